@@ -1,0 +1,97 @@
+/*
+ * msmpb_hip.h -- entry points of libmsmpb_hip.so beyond forcefield.h.
+ *
+ * These have no counterpart in hkayabilisim/msmpb; they serve callers that
+ * keep positions and charges in GPU memory (the PyTorch autograd binding in
+ * msmpb_amd/) and callers that choose the precision or the reference HEAD
+ * index layout.  All pointers named d_* are device pointers (HBM) of the
+ * current HIP device; `stream` is a hipStream_t (NULL = the FF's own
+ * stream).  Every int-returning call returns 0 on success and a negative
+ * code on failure, with a message available from FF_last_error().
+ */
+#ifndef MSMPB_HIP_EXT_H
+#define MSMPB_HIP_EXT_H
+
+#include "forcefield.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  MSM_OK = 0,
+  MSM_ERR_ARG = -1,      /* invalid argument or parameter outside the domain */
+  MSM_ERR_STATE = -2,    /* called before FF_build, or after a failed build */
+  MSM_ERR_HIP = -3,      /* HIP runtime failure (message names the call) */
+  MSM_ERR_ALLOC = -4     /* device allocation failed */
+};
+
+/* Arithmetic of the evaluation. */
+enum {
+  MSM_PREC_FP64 = 0,     /* FP64 throughout (parity mode) */
+  MSM_PREC_MIXED = 1     /* FP32 short-range pairs with FP64 accumulation,
+                            BF16 hi/lo-split MFMA level stencils with FP32
+                            accumulation; FP64 elsewhere */
+};
+
+/* Index layout of the grid levels. */
+enum {
+  MSM_LAYOUT_FIXED = 0,  /* corrected layout (reproduces the published
+                            results/ energies of the reference) */
+  MSM_LAYOUT_ASIS = 1    /* reference HEAD: L = 1 grid energy read at
+                            unpadded indices (forcefield1.c:196-212); L >= 2
+                            top grid read padded (forcefield1.c:504-515) */
+};
+
+/* Last error message of the calling thread ("" if none). */
+const char *FF_last_error(void);
+
+/* Set before FF_build (or between evaluations). */
+int FF_set_precision(FF *ff, int precision);
+int FF_get_precision(FF *ff);
+int FF_set_layout_compat(FF *ff, int layout);
+/* Record per-stage GPU time into the FF time_* fields on FF_energy. */
+int FF_set_stage_timing(FF *ff, int enable);
+
+/* Energy split of the last evaluation: {constant part, short-range pairs,
+ * grid part}; their sum is the returned energy. */
+int FF_get_energy_terms(FF *ff, double terms[3]);
+
+/* Same as FF_energy but also returns phi_i = dE/dq_i (N doubles, host). */
+double FF_energy_potential(FF *ff, double (*force)[3], double (*position)[3],
+                           double *potential);
+
+/* Device-resident evaluation.  d_pos: N x 3 (row-major xyz), d_q: N.
+ * d_energy (device, >= 4 doubles) receives {E, constant, pairs, grid}.
+ * d_force (N x 3) and d_phi (N) are optional (NULL = not computed).
+ * The call is stream-ordered: it returns before the GPU finishes. */
+int msm_eval_device(FF *ff, const double *d_pos, const double *d_q, double *d_energy,
+                    double *d_force, double *d_phi, void *stream);
+
+/* Split evaluation for automatic differentiation.
+ * msm_forward_device computes the energy only; keep != 0 keeps the
+ * level-1 potential grid and the atom ordering for msm_adjoint_device.
+ * msm_adjoint_device writes d_force = -dE/dr and d_phi = dE/dq; when the
+ * forward state was not kept (or recompute != 0) it recomputes the grid
+ * levels from d_pos and d_q first. */
+int msm_forward_device(FF *ff, const double *d_pos, const double *d_q, double *d_energy,
+                       int keep, void *stream);
+int msm_adjoint_device(FF *ff, const double *d_pos, const double *d_q, double *d_force,
+                       double *d_phi, int recompute, void *stream);
+
+/* Diagnostics for the stage-by-stage parity tests: copy the last
+ * evaluation's periodic level-l grid (which = 0: charges q^l, 1:
+ * potentials e^l; x-major, z fastest) or the level-l stencil to host
+ * memory.  Dims via msm_grid_dims / msm_stencil_dims. */
+int msm_grid_dims(FF *ff, int level, int dims[3]);
+int msm_stencil_dims(FF *ff, int level, int dims[3]);
+int msm_copy_grid(FF *ff, int level, int which, double *host_out);
+int msm_copy_stencil(FF *ff, int level, double *host_out);
+/* Number of atom pairs i < j within the cutoff (nearest image), counted on
+ * the GPU for the given positions (host N x 3). */
+int msm_count_pairs(FF *ff, double (*position)[3], long long *count);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,879 @@
+// msm_engine.hip -- the C ABI of libmsmpb_hip.so: forcefield.h entry points
+// (hkayabilisim/msmpb code/forcefield.h:70-112) and the device-pointer
+// extensions of msmpb_hip.h, on top of the HIP kernel chain.
+//
+// One FF owns one Engine: the resolved plan, its device buffers (allocated
+// once in FF_build, sized for N atoms) and a HIP stream.  FF_energy stages
+// host positions and charges into HBM, runs the chain and copies forces
+// back; the *_device entry points work on caller-owned device memory and
+// are stream-ordered.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/msmpb_hip.h"
+// forcefield.h defines GNU max/min macros for C callers; not wanted here
+#undef max
+#undef min
+#include "msm_kernels.h"
+#include "msm_plan.h"
+
+// The reference's callers define `struct O o`; keep programs that do not
+// define it linkable.
+__attribute__((weak)) struct O o = {(char *)"csCl"};
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIP_TRY(call)                                                                  \
+  do {                                                                                 \
+    hipError_t e_ = (call);                                                            \
+    if (e_ != hipSuccess)                                                              \
+      return fail(MSM_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_));     \
+  } while (0)
+
+struct Timing {
+  bool on = false;
+  std::vector<hipEvent_t> ev;
+  std::vector<std::string> tag;
+  void mark(const char *t, hipStream_t s) {
+    if (!on) return;
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return;
+    hipEventRecord(e, s);
+    ev.push_back(e);
+    tag.push_back(t);
+  }
+  void clear() {
+    for (auto e : ev) hipEventDestroy(e);
+    ev.clear();
+    tag.clear();
+  }
+};
+
+struct Engine {
+  msm::Plan plan;
+  msm::DevParams P{};
+  msm::GridDims gd[msm::kMaxLevel + 1]{}, sd[msm::kMaxLevel + 1]{};
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int nsten = 0;
+  // device buffers
+  double *d_pos = nullptr, *d_q = nullptr, *d_F = nullptr, *d_phi = nullptr;
+  double *d_out = nullptr, *d_qpart = nullptr, *d_scal = nullptr;
+  double *d_epair = nullptr, *d_egrid = nullptr;
+  int *d_cell_of = nullptr, *d_cnt = nullptr, *d_start = nullptr, *d_sidx = nullptr;
+  int *d_scell = nullptr, *d_flags = nullptr;
+  double4 *d_wpos = nullptr, *d_spos = nullptr;
+  float4 *d_lpos = nullptr;
+  int4 *d_sten = nullptr;
+  unsigned long long *d_count = nullptr;
+  double *d_ql[msm::kMaxLevel + 1] = {}, *d_el[msm::kMaxLevel + 1] = {};
+  double *d_khat[msm::kMaxLevel + 1] = {};
+  void *d_ksplit[msm::kMaxLevel + 1] = {};
+  void *d_qsplit = nullptr;
+  double *d_tw = nullptr, *d_topwork = nullptr;
+  // forward state kept for a following adjoint
+  bool kept = false;
+  const double *kept_pos = nullptr, *kept_q = nullptr;
+  double last_terms[3] = {0, 0, 0};
+
+  ~Engine() { release(); }
+  void release() {
+    void *ptrs[] = {d_pos,  d_q,    d_F,    d_phi,   d_out,  d_qpart, d_scal,  d_epair,
+                    d_egrid, d_cell_of, d_cnt, d_start, d_sidx, d_scell, d_flags, d_wpos,
+                    d_spos, d_lpos, d_sten, d_count, d_qsplit, d_tw, d_topwork};
+    for (void *p : ptrs)
+      if (p) hipFree(p);
+    for (int l = 0; l <= msm::kMaxLevel; ++l) {
+      if (d_ql[l]) hipFree(d_ql[l]);
+      if (d_el[l]) hipFree(d_el[l]);
+      if (d_khat[l]) hipFree(d_khat[l]);
+      if (d_ksplit[l]) hipFree(d_ksplit[l]);
+      d_ql[l] = d_el[l] = d_khat[l] = nullptr;
+      d_ksplit[l] = nullptr;
+    }
+    d_pos = d_q = d_F = d_phi = d_out = d_qpart = d_scal = d_epair = d_egrid = nullptr;
+    d_cell_of = d_cnt = d_start = d_sidx = d_scell = d_flags = nullptr;
+    d_wpos = d_spos = nullptr;
+    d_lpos = nullptr;
+    d_sten = nullptr;
+    d_count = nullptr;
+    d_qsplit = nullptr;
+    d_tw = d_topwork = nullptr;
+    if (stream) hipStreamDestroy(stream);
+    stream = nullptr;
+  }
+};
+
+struct FFX {
+  FF ff;  // first member: FF* and FFX* alias
+  Engine *eng = nullptr;
+  int precision = MSM_PREC_FP64;
+  int layout = MSM_LAYOUT_FIXED;
+  int timing = 0;
+  bool built = false;
+  // host copies behind the public FF pointers
+  std::vector<double> h_tau, h_Q, h_J, h_aCut, h_cL[3];
+  std::vector<double *> h_khat_ptr;
+  std::vector<std::vector<double>> h_khat;
+  double *(*h_omegap)[3] = nullptr;
+  std::vector<std::vector<double>> h_omegap_store;
+};
+
+FFX *X(FF *ff) { return reinterpret_cast<FFX *>(ff); }
+
+template <typename T>
+int dalloc(T **p, size_t n) {
+  if (n == 0) n = 1;
+  hipError_t e = hipMalloc((void **)p, n * sizeof(T));
+  if (e != hipSuccess)
+    return fail(MSM_ERR_ALLOC, std::string("hipMalloc of ") + std::to_string(n * sizeof(T)) +
+                                   " bytes: " + hipGetErrorString(e));
+  return MSM_OK;
+}
+
+#define TRY(x)              \
+  do {                      \
+    int r_ = (x);           \
+    if (r_ != MSM_OK) return r_; \
+  } while (0)
+
+// Cells of width >= a0/2 and the list of neighbour-cell offsets whose
+// nearest points lie inside the cutoff.
+void pair_geometry(const msm::Plan &p, msm::DevParams &P, std::vector<int4> &sten) {
+  double a0 = p.aCut[0];
+  int lim[3];
+  for (int a = 0; a < 3; ++a) {
+    double L = p.A[a][a];
+    int n = (int)std::floor(L / (0.5 * a0));
+    if (n < 1) n = 1;
+    if (n > 256) n = 256;
+    P.nc[a] = n;
+    P.cw[a] = L / n;
+    lim[a] = (int)std::ceil(a0 / P.cw[a]);
+  }
+  P.ncell = P.nc[0] * P.nc[1] * P.nc[2];
+  sten.clear();
+  for (int dx = -lim[0]; dx <= lim[0]; ++dx)
+    for (int dy = -lim[1]; dy <= lim[1]; ++dy)
+      for (int dz = -lim[2]; dz <= lim[2]; ++dz) {
+        int d[3] = {dx, dy, dz};
+        double m2 = 0;
+        for (int a = 0; a < 3; ++a) {
+          double g = (std::abs(d[a]) - 1) * P.cw[a];
+          if (g > 0) m2 += g * g;
+        }
+        if (m2 < a0 * a0) sten.push_back(make_int4(dx, dy, dz, 0));
+      }
+}
+
+void fill_params(const msm::Plan &p, int N, msm::DevParams &P) {
+  for (int a = 0; a < 3; ++a) {
+    P.box[a] = p.A[a][a];
+    P.ibox[a] = p.Ai[a][a];
+  }
+  double a0 = p.aCut[0];
+  P.a0 = a0;
+  P.a02 = a0 * a0;
+  P.inv_a0 = 1.0 / a0;
+  P.inv_a02 = 1.0 / (a0 * a0);
+  P.two_inv_a03 = 2.0 / (a0 * a0 * a0);
+  P.nu = p.nu;
+  P.N = N;
+  std::memset(P.tau, 0, sizeof P.tau);
+  std::memset(P.Q, 0, sizeof P.Q);
+  std::memset(P.J, 0, sizeof P.J);
+  for (int k = 0; k < p.nu; ++k) P.tau[k] = p.tau[k];
+  for (int k = 0; k < p.nu / 2 * p.nu; ++k) P.Q[k] = p.Q[k];
+  for (int k = 0; k <= p.nu; ++k) P.J[k] = p.J[k];
+  P.coeff1 = p.coeff1;
+  P.coeff2 = p.coeff2;
+  P.gamma0 = p.gamma0;
+  P.coeff_self = p.coeff1 - p.gamma0 / a0;
+}
+
+// Uploads the box-dependent device data (stencils, twiddles, parameters).
+int upload_box(Engine &E) {
+  msm::Plan &p = E.plan;
+  fill_params(p, p.N, E.P);
+  std::vector<int4> sten;
+  pair_geometry(p, E.P, sten);
+  E.nsten = (int)sten.size();
+  if (E.d_sten) hipFree(E.d_sten), E.d_sten = nullptr;
+  TRY(dalloc(&E.d_sten, sten.size()));
+  HIP_TRY(hipMemcpy(E.d_sten, sten.data(), sten.size() * sizeof(int4), hipMemcpyHostToDevice));
+  for (int l = 1; l <= p.L; ++l) {
+    if (E.d_khat[l]) hipFree(E.d_khat[l]), E.d_khat[l] = nullptr;
+    TRY(dalloc(&E.d_khat[l], p.khat[l].size()));
+    HIP_TRY(hipMemcpy(E.d_khat[l], p.khat[l].data(), p.khat[l].size() * sizeof(double),
+                      hipMemcpyHostToDevice));
+  }
+  const int *gt = p.M[p.L];
+  std::vector<double> tw;
+  for (int a = 0; a < 3; ++a)
+    for (int j = 0; j < gt[a]; ++j) {
+      double ang = 2.0 * M_PI * (double)j / (double)gt[a];
+      tw.push_back(std::cos(ang));
+      tw.push_back(std::sin(ang));
+    }
+  if (E.d_tw) hipFree(E.d_tw), E.d_tw = nullptr;
+  TRY(dalloc(&E.d_tw, tw.size()));
+  HIP_TRY(hipMemcpy(E.d_tw, tw.data(), tw.size() * sizeof(double), hipMemcpyHostToDevice));
+  return MSM_OK;
+}
+
+int create_engine(FFX *x, const msm::Plan &plan) {
+  Engine *E = new Engine();
+  E->plan = plan;
+  x->eng = E;
+  HIP_TRY(hipGetDevice(&E->device));
+  HIP_TRY(hipStreamCreateWithFlags(&E->stream, hipStreamNonBlocking));
+  const msm::Plan &p = E->plan;
+  for (int l = 1; l <= p.L; ++l) {
+    E->gd[l] = {p.M[l][0], p.M[l][1], p.M[l][2]};
+    E->sd[l] = {p.S[l][0], p.S[l][1], p.S[l][2]};
+  }
+  E->sd[p.L] = E->gd[p.L];
+  TRY(upload_box(*E));
+  long N = p.N;
+  int nb = msm::bin_blocks((int)N), np = msm::pair_blocks((int)N);
+  TRY(dalloc(&E->d_pos, 3 * N));
+  TRY(dalloc(&E->d_q, N));
+  TRY(dalloc(&E->d_F, 3 * N));
+  TRY(dalloc(&E->d_phi, N));
+  TRY(dalloc(&E->d_out, 8));
+  TRY(dalloc(&E->d_qpart, 2 * nb));
+  TRY(dalloc(&E->d_scal, 4));
+  TRY(dalloc(&E->d_epair, np));
+  TRY(dalloc(&E->d_egrid, msm::grid_blocks(E->gd[1].size())));
+  TRY(dalloc(&E->d_cell_of, N));
+  TRY(dalloc(&E->d_cnt, E->P.ncell));
+  TRY(dalloc(&E->d_start, E->P.ncell + 1));
+  TRY(dalloc(&E->d_sidx, N));
+  TRY(dalloc(&E->d_scell, N));
+  TRY(dalloc(&E->d_flags, 4));
+  TRY(dalloc(&E->d_wpos, N));
+  TRY(dalloc(&E->d_spos, N));
+  TRY(dalloc(&E->d_lpos, N));
+  TRY(dalloc(&E->d_count, 1));
+  for (int l = 1; l <= p.L; ++l) {
+    TRY(dalloc(&E->d_ql[l], E->gd[l].size()));
+    TRY(dalloc(&E->d_el[l], E->gd[l].size()));
+  }
+  TRY(dalloc(&E->d_topwork, 4 * E->gd[p.L].size()));
+  return MSM_OK;
+}
+
+// ---- the evaluation chain ------------------------------------------------
+// Binning + counting sort of atoms into cells (replaces the stored
+// neighbour list of forcefield0.c:21-164).
+void stage_prepare(Engine &E, const double *pos, const double *q, hipStream_t s) {
+  msm::launch_fill_int(E.d_cnt, E.P.ncell, 0, s);
+  msm::launch_fill_int(E.d_flags, 1, 0, s);
+  msm::launch_bin(E.P, pos, q, E.d_cell_of, E.d_cnt, E.d_wpos, E.d_qpart, E.d_flags, s);
+  msm::launch_scan(E.P, E.d_cnt, E.d_start, E.d_qpart, msm::bin_blocks(E.P.N), E.d_scal, s);
+  msm::launch_scatter(E.P, E.d_cell_of, E.d_wpos, E.d_start, E.d_cnt, E.d_spos, E.d_sidx,
+                      E.d_scell, s);
+  msm::launch_cell_sort(E.P, E.d_start, E.d_spos, E.d_sidx, E.d_lpos, s);
+}
+
+void stage_pairs(Engine &E, int precision, double *F, double *phi, hipStream_t s) {
+  if (precision == MSM_PREC_MIXED)
+    msm::launch_pair_fp32(E.P, E.d_lpos, E.d_sidx, E.d_scell, E.d_start, E.d_sten, E.nsten, F, phi,
+                          E.d_epair, s);
+  else
+    msm::launch_pair_fp64(E.P, E.d_spos, E.d_sidx, E.d_scell, E.d_start, E.d_sten, E.nsten, F, phi,
+                          E.d_epair, s);
+}
+
+// Spread, restrict to the top, top-level DFT convolution, then prolongate
+// and add each level's stencil pass down to e^1.
+void stage_grids(Engine &E, int precision, int layout, Timing &T, hipStream_t s) {
+  const msm::Plan &p = E.plan;
+  int L = p.L;
+  msm::launch_spread(E.P, E.gd[1], E.d_spos, E.d_ql[1], s);
+  T.mark("anterpolation", s);
+  for (int l = 2; l <= L; ++l) {
+    msm::launch_restrict(E.P, E.gd[l - 1], E.gd[l], E.d_ql[l - 1], E.d_ql[l], s);
+    T.mark("restriction", s);
+  }
+  msm::launch_top(E.P, E.gd[L], E.d_ql[L], E.d_khat[L], E.d_tw, E.d_topwork, E.d_el[L],
+                  layout == MSM_LAYOUT_ASIS && L >= 2, s);
+  T.mark("grid2grid_top", s);
+  for (int l = L - 1; l >= 1; --l) {
+    (void)precision;
+    msm::launch_level_fp64(E.P, E.gd[l], E.gd[l + 1], E.sd[l], E.d_khat[l], E.d_ql[l],
+                           E.d_el[l + 1], E.d_el[l], s);
+    T.mark("grid2grid", s);
+  }
+}
+
+void stage_energy(Engine &E, int layout, hipStream_t s) {
+  msm::launch_grid_energy(E.P, E.gd[1], E.d_ql[1], E.d_el[1], E.d_egrid,
+                          layout == MSM_LAYOUT_ASIS && E.plan.L == 1, s);
+  msm::launch_finalize(E.P, E.d_epair, msm::pair_blocks(E.P.N), E.d_egrid,
+                       msm::grid_blocks(E.gd[1].size()), E.d_scal, E.d_out, s);
+}
+
+int check_built(FF *ff) {
+  if (!ff) return fail(MSM_ERR_ARG, "FF is NULL");
+  FFX *x = X(ff);
+  if (!x->built || !x->eng) return fail(MSM_ERR_STATE, "FF_build has not completed successfully");
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  if (dev != x->eng->device) HIP_TRY(hipSetDevice(x->eng->device));
+  return MSM_OK;
+}
+
+// Full evaluation on device pointers; out receives {E, const, pair, grid}.
+int eval_chain(FFX *x, const double *pos, const double *q, double *out, double *F, double *phi,
+               hipStream_t s, Timing &T) {
+  Engine &E = *x->eng;
+  T.mark("begin", s);
+  stage_prepare(E, pos, q, s);
+  T.mark("nlist", s);
+  stage_pairs(E, x->precision, F, phi, s);
+  T.mark("partcl2partcl", s);
+  stage_grids(E, x->precision, x->layout, T, s);
+  stage_energy(E, x->layout, s);
+  if (F || phi) msm::launch_gather(E.P, E.gd[1], E.d_spos, E.d_sidx, E.d_el[1], E.d_scal, F, phi, s);
+  T.mark("interpolation", s);
+  if (out != E.d_out) HIP_TRY(hipMemcpyAsync(out, E.d_out, 4 * sizeof(double), hipMemcpyDeviceToDevice, s));
+  HIP_TRY(hipGetLastError());
+  E.kept = false;
+  return MSM_OK;
+}
+
+void publish_timing(FFX *x, Timing &T) {
+  if (!T.on || T.ev.size() < 2) return;
+  FF *ff = &x->ff;
+  ff->time_partcl2partcl = ff->time_anterpolation = ff->time_interpolation = 0;
+  ff->time_nlist = 0;
+  for (int l = 0; l < 10; ++l)
+    ff->time_grid2grid[l] = ff->time_restriction[l] = ff->time_prolongation[l] =
+        ff->time_padding[l] = 0;
+  int L = x->eng->plan.L, lr = 2, lg = L - 1;
+  for (size_t k = 1; k < T.ev.size(); ++k) {
+    float ms = 0;
+    hipEventElapsedTime(&ms, T.ev[k - 1], T.ev[k]);
+    double sec = ms * 1e-3;
+    const std::string &t = T.tag[k];
+    if (t == "nlist") ff->time_nlist = sec;
+    else if (t == "partcl2partcl") ff->time_partcl2partcl = sec;
+    else if (t == "anterpolation") ff->time_anterpolation = sec;
+    else if (t == "restriction") { if (lr <= 9) ff->time_restriction[lr] = sec; ++lr; }
+    else if (t == "grid2grid_top") { if (L <= 9) ff->time_grid2grid[L] = sec; }
+    else if (t == "grid2grid") { if (lg >= 1 && lg <= 9) ff->time_grid2grid[lg] = sec; --lg; }
+    else if (t == "interpolation") ff->time_interpolation = sec;
+  }
+}
+
+void publish_plan(FFX *x) {
+  FF *ff = &x->ff;
+  const msm::Plan &p = x->eng->plan;
+  std::memcpy(ff->A, p.A, sizeof ff->A);
+  std::memcpy(ff->Ai, p.Ai, sizeof ff->Ai);
+  ff->detA = p.detA;
+  ff->relCutoff = p.relCutoff;
+  ff->nLim = p.nLim;
+  ff->ntc = 7;
+  ff->tmax = 4;
+  ff->estErr = p.estErr;
+  ff->orderAcc = p.nu;
+  ff->maxLevel = p.L;
+  for (int a = 0; a < 3; ++a) ff->topGridDim[a] = p.M[p.L][a];
+  ff->cutoff = p.aCut[0];
+  ff->coeff1 = p.coeff1;
+  ff->coeff2 = p.coeff2;
+  ff->nlist = nullptr;
+  ff->nlist_len = 0;
+  x->h_tau.assign(p.tau, p.tau + p.nu);
+  x->h_Q.assign(p.Q, p.Q + p.nu / 2 * p.nu);
+  x->h_J.assign(p.J, p.J + p.nu + 1);
+  x->h_aCut.assign(p.aCut, p.aCut + p.L + 1);
+  ff->tau = x->h_tau.data();
+  ff->Q = x->h_Q.data();
+  ff->J = x->h_J.data();
+  ff->aCut = x->h_aCut.data();
+  ff->dsigma = nullptr;
+  ff->sigmad = nullptr;
+  for (int a = 0; a < 3; ++a) {
+    x->h_cL[a] = p.cL[a];
+    ff->cL[a] = x->h_cL[a].data();
+  }
+  x->h_khat.assign(p.L + 1, std::vector<double>());
+  x->h_khat_ptr.assign(p.L + 1, nullptr);
+  for (int l = 1; l <= p.L; ++l) {
+    x->h_khat[l] = p.khat[l];
+    x->h_khat_ptr[l] = x->h_khat[l].data();
+  }
+  ff->khat = x->h_khat_ptr.data();
+  free(x->h_omegap);
+  x->h_omegap = (double *(*)[3])calloc(p.L + 1, sizeof(double *[3]));
+  x->h_omegap_store.clear();
+  x->h_omegap_store.reserve(3 * (p.L + 1));
+  for (int l = 1; l <= p.L; ++l)
+    for (int a = 0; a < 3; ++a) {
+      x->h_omegap_store.push_back(p.omegap[l][a]);
+      x->h_omegap[l][a] = x->h_omegap_store.back().data();
+    }
+  ff->omegap = x->h_omegap;
+}
+
+int count_pairs_dev(FFX *x, const double *d_pos, const double *d_q, long long *count) {
+  Engine &E = *x->eng;
+  hipStream_t s = E.stream;
+  stage_prepare(E, d_pos, d_q, s);
+  HIP_TRY(hipMemsetAsync(E.d_count, 0, sizeof(unsigned long long), s));
+  msm::launch_pair_count(E.P, E.d_spos, E.d_scell, E.d_start, E.d_sten, E.nsten, E.d_count, s);
+  unsigned long long c = 0;
+  HIP_TRY(hipMemcpyAsync(&c, E.d_count, sizeof c, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  *count = (long long)c;
+  return MSM_OK;
+}
+
+int stage_host_inputs(FFX *x, double (*position)[3]) {
+  Engine &E = *x->eng;
+  size_t N = E.plan.N;
+  HIP_TRY(hipMemcpyAsync(E.d_pos, position, 3 * N * sizeof(double), hipMemcpyHostToDevice, E.stream));
+  HIP_TRY(hipMemcpyAsync(E.d_q, x->ff.q, N * sizeof(double), hipMemcpyHostToDevice, E.stream));
+  return MSM_OK;
+}
+
+int build_impl(FF *ff, int N, double (*position)[3], double edges[3][3]) {
+  FFX *x = X(ff);
+  x->built = false;
+  g_err.clear();
+  if (x->eng) {
+    delete x->eng;
+    x->eng = nullptr;
+  }
+  if (N != ff->N) return fail(MSM_ERR_ARG, "FF_build: N differs from the N given to FF_new");
+  if (!ff->q) return fail(MSM_ERR_ARG, "FF_build: charges are NULL");
+  if (!position) return fail(MSM_ERR_ARG, "FF_build: positions are NULL");
+  msm::BuildRequest r;
+  r.N = N;
+  r.q = ff->q;
+  std::memcpy(r.edges, edges, sizeof r.edges);
+  r.nu = ff->orderAcc;
+  r.L = ff->maxLevel;
+  for (int a = 0; a < 3; ++a) r.Mtop[a] = ff->topGridDim[0] > 0 ? ff->topGridDim[a] : -1;
+  r.a0 = ff->cutoff;
+  r.estErr = ff->estErr;
+  msm::Plan plan;
+  std::string err;
+  if (!ff->FFT) return fail(MSM_ERR_ARG, "FF_build: only the FFT top level is supported");
+  auto t0 = std::chrono::steady_clock::now();
+  if (!msm::build_plan(r, plan, err)) return fail(MSM_ERR_ARG, "FF_build: " + err);
+  double tstencil =
+      std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  TRY(create_engine(x, plan));
+  publish_plan(x);
+  x->built = true;
+  for (int l = 1; l <= plan.L && l < 10; ++l) ff->time_stencil[l] = tstencil / plan.L;
+  // neighbour statistics for the reference's JSON fields
+  TRY(stage_host_inputs(x, position));
+  long long cnt = 0;
+  auto t1 = std::chrono::steady_clock::now();
+  TRY(count_pairs_dev(x, x->eng->d_pos, x->eng->d_q, &cnt));
+  ff->time_nlist = std::chrono::duration<double>(std::chrono::steady_clock::now() - t1).count();
+  ff->nlist_interaction_count = (int)cnt;
+  return MSM_OK;
+}
+
+int energy_host(FF *ff, double (*force)[3], double (*position)[3], double *phi_out, double *E_out) {
+  TRY(check_built(ff));
+  FFX *x = X(ff);
+  Engine &E = *x->eng;
+  if (!force || !position) return fail(MSM_ERR_ARG, "FF_energy: force and position must be non-NULL");
+  Timing T;
+  T.on = x->timing != 0;
+  TRY(stage_host_inputs(x, position));
+  int rc = eval_chain(x, E.d_pos, E.d_q, E.d_out, E.d_F, phi_out ? E.d_phi : nullptr, E.stream, T);
+  if (rc != MSM_OK) {
+    T.clear();
+    return rc;
+  }
+  size_t N = E.plan.N;
+  double out[4];
+  int flags = 0;
+  HIP_TRY(hipMemcpyAsync(force, E.d_F, 3 * N * sizeof(double), hipMemcpyDeviceToHost, E.stream));
+  if (phi_out) HIP_TRY(hipMemcpyAsync(phi_out, E.d_phi, N * sizeof(double), hipMemcpyDeviceToHost, E.stream));
+  HIP_TRY(hipMemcpyAsync(out, E.d_out, sizeof out, hipMemcpyDeviceToHost, E.stream));
+  HIP_TRY(hipMemcpyAsync(&flags, E.d_flags, sizeof flags, hipMemcpyDeviceToHost, E.stream));
+  HIP_TRY(hipStreamSynchronize(E.stream));
+  publish_timing(x, T);
+  T.clear();
+  if (flags) return fail(MSM_ERR_ARG, "FF_energy: non-finite position or charge");
+  std::memcpy(E.last_terms, out + 1, 3 * sizeof(double));
+  *E_out = out[0];
+  return MSM_OK;
+}
+
+double my_errest(FF *ff, int nu) {
+  // _FF_get_errEst (forcefield0.c:553-582) on the public fields
+  double ax = std::sqrt(ff->A[0][0] * ff->A[0][0] + ff->A[0][1] * ff->A[0][1] + ff->A[0][2] * ff->A[0][2]);
+  double ay = std::sqrt(ff->A[1][0] * ff->A[1][0] + ff->A[1][1] * ff->A[1][1] + ff->A[1][2] * ff->A[1][2]);
+  double az = std::sqrt(ff->A[2][0] * ff->A[2][0] + ff->A[2][1] * ff->A[2][1] + ff->A[2][2] * ff->A[2][2]);
+  double M1[3];
+  for (int a = 0; a < 3; ++a) M1[a] = ff->topGridDim[a] * std::pow(2.0, ff->maxLevel - 1);
+  double Q2 = 0;
+  for (int i = 0; i < ff->N; ++i) Q2 += ff->q[i] * ff->q[i];
+  double Fref = (Q2 / (double)ff->N) / std::pow(ff->detA / (double)ff->N, 5. / 6.);
+  double h1 = std::fmax(std::fmax(ax / M1[0], ay / M1[1]), az / M1[2]);
+  double dF = Q2 * (1 - std::pow(4.0, -ff->maxLevel)) * std::pow(h1, nu - 1) *
+              std::pow(ff->detA, -1. / 3.) * std::pow(ff->N, -2.0 / 3.0);
+  return dF / Fref;
+}
+
+}  // namespace
+
+// ============================================================ forcefield.h
+extern "C" {
+
+FF *FF_new(int N, double *q) {
+  FFX *x = new FFX();
+  std::memset(&x->ff, 0, sizeof(FF));
+  FF *ff = &x->ff;
+  ff->FFT = true;
+  ff->N = N;
+  ff->q = q;
+  ff->maxLevel = -1;
+  ff->orderAcc = -1;
+  ff->topGridDim[0] = ff->topGridDim[1] = ff->topGridDim[2] = -1;
+  ff->cutoff = -1;
+  ff->estErr = -1;
+  return ff;
+}
+
+void FF_set_orderAcc(FF *ff, int orderAcc) { ff->orderAcc = orderAcc; }
+void FF_set_maxLevel(FF *ff, int maxLevel) { ff->maxLevel = maxLevel; }
+void FF_set_topGridDim(FF *ff, int topGridDim[3]) {
+  for (int a = 0; a < 3; ++a) ff->topGridDim[a] = topGridDim[a];
+}
+void FF_set_estErr(FF *ff, double estErr) { ff->estErr = estErr; }
+double FF_get_estErr(FF *ff) { return ff->estErr; }
+void FF_set_cutoff(FF *ff, double cutoff) { ff->cutoff = cutoff; }
+double FF_get_cutoff(FF *ff) { return ff->cutoff; }
+void FF_set_FFT(FF *ff, bool FFT) {
+  if (!FFT) std::fprintf(stderr, "msmpb_hip: only the FFT top level is implemented\n");
+  ff->FFT = FFT;
+}
+int FF_get_orderAcc(FF *ff) { return ff->orderAcc; }
+int FF_get_maxLevel(FF *ff) { return ff->maxLevel; }
+void FF_get_topGridDim(FF *ff, int topGridDim[3]) {
+  for (int a = 0; a < 3; ++a) topGridDim[a] = ff->topGridDim[a];
+}
+bool FF_get_FFT(FF *ff) { return ff->FFT; }
+
+void FF_build(FF *ff, int N, double (*position)[3], double edges[3][3], double margin) {
+  (void)margin;  // pairs are re-binned every evaluation
+  if (!ff) return;
+  if (build_impl(ff, N, position, edges) != MSM_OK)
+    std::fprintf(stderr, "msmpb_hip: %s\n", g_err.c_str());
+}
+
+void FF_rebuild(FF *ff, double edges[3][3], double (*position)[3], double margin) {
+  (void)margin;
+  if (check_built(ff) != MSM_OK) {
+    std::fprintf(stderr, "msmpb_hip: %s\n", g_err.c_str());
+    return;
+  }
+  FFX *x = X(ff);
+  std::string err;
+  Engine &E = *x->eng;
+  if (!msm::rebuild_plan(E.plan, edges, err) || upload_box(E) != MSM_OK) {
+    if (!err.empty()) g_err = "FF_rebuild: " + err;
+    x->built = false;
+    std::fprintf(stderr, "msmpb_hip: %s\n", g_err.c_str());
+    return;
+  }
+  publish_plan(x);
+  if (position) {
+    long long cnt = 0;
+    if (stage_host_inputs(x, position) == MSM_OK &&
+        count_pairs_dev(x, E.d_pos, E.d_q, &cnt) == MSM_OK)
+      ff->nlist_interaction_count = (int)cnt;
+  }
+}
+
+double FF_energy(FF *ff, double (*force)[3], double (*position)[3], double *weight) {
+  if (weight && ff && ff->maxLevel > 0) {
+    for (int l = 0; l <= ff->maxLevel; ++l)
+      if (weight[l] != 1.0) {
+        fail(MSM_ERR_ARG, "FF_energy: level weights other than 1 are not supported");
+        std::fprintf(stderr, "msmpb_hip: %s\n", g_err.c_str());
+        return NAN;
+      }
+  }
+  double E = NAN;
+  if (energy_host(ff, force, position, nullptr, &E) != MSM_OK) {
+    std::fprintf(stderr, "msmpb_hip: %s\n", g_err.c_str());
+    return NAN;
+  }
+  ff->errEst = FF_get_errEst(ff);
+  return E;
+}
+
+void FF_delete(FF *ff) {
+  if (!ff) return;
+  FFX *x = X(ff);
+  delete x->eng;
+  free(x->h_omegap);
+  delete x;
+}
+
+double FF_get_deltaF(FF *ff) {
+  // forcefield0.c:521-551: the same a-priori model with fitted constants C
+  static const double C[4] = {0.825169, 5.16701, 73.6508, 1977.23};
+  int nu = ff->orderAcc;
+  double Q2 = 0;
+  for (int i = 0; i < ff->N; ++i) Q2 += ff->q[i] * ff->q[i];
+  double Fref = (Q2 / (double)ff->N) / std::pow(ff->detA / (double)ff->N, 5. / 6.);
+  return C[(nu - 4) / 2] * my_errest(ff, nu) * Fref / std::pow(ff->cutoff, nu);
+}
+
+double _FF_get_errEst(FF *ff, int nu) { return my_errest(ff, nu); }
+
+double FF_get_errEst(FF *ff) { return my_errest(ff, ff->orderAcc) / std::pow(ff->cutoff, ff->orderAcc); }
+
+// tic/toc stack of CPU-clock timers (forcefield1.c:35-56)
+double msm4g_tictocmanager(int push) {
+  static clock_t stack[100];
+  static int top = 0;
+  if (push) {
+    if (top < 100) stack[top++] = clock();
+    return 0.0;
+  }
+  if (top == 0) return 0.0;
+  clock_t now = clock();
+  return (double)(now - stack[--top]) / CLOCKS_PER_SEC;
+}
+void msm4g_tic(void) { msm4g_tictocmanager(1); }
+double msm4g_toc(void) { return msm4g_tictocmanager(0); }
+
+double invert(Matrix *A) {
+  double a[3][3], ai[3][3];
+  std::memcpy(a, A, sizeof a);
+  double d = msm::invert3(a, ai);
+  std::memcpy(A, ai, sizeof ai);
+  return d;
+}
+
+void omegap(FF *ff) {
+  // recompute the anti-blurring weights of a built FF (forcefield0.c:726-767)
+  if (check_built(ff) != MSM_OK) return;
+  FFX *x = X(ff);
+  std::string err;
+  msm::Plan &p = x->eng->plan;
+  double edges[3][3];
+  std::memcpy(edges, p.A, sizeof edges);
+  if (msm::rebuild_plan(p, edges, err)) publish_plan(x);
+}
+
+Vector prod(Matrix m, Vector v) {
+  Vector r = {m.xx * v.x + m.xy * v.y + m.xz * v.z, m.yx * v.x + m.yy * v.y + m.yz * v.z,
+              m.zx * v.x + m.zy * v.y + m.zz * v.z};
+  return r;
+}
+
+Vector prodT(Matrix m, Vector v) {
+  Vector r = {m.xx * v.x + m.yx * v.y + m.zx * v.z, m.xy * v.x + m.yy * v.y + m.zy * v.z,
+              m.xz * v.x + m.yz * v.y + m.zz * v.z};
+  return r;
+}
+
+void grid2grid(FF *ff, const int l, const Triple gd, double *el, const double *ql, const Triple sd,
+               const double *kh) {
+  // reference layout (forcefield0.c:898-931): ql padded by
+  // max(min(dmax, M), nu) per axis with the interior at max(pad/2, nu/2)
+  if (check_built(ff) != MSM_OK) return;
+  FFX *x = X(ff);
+  Engine &E = *x->eng;
+  int nu = E.plan.nu, dmax = 2 * E.plan.nLim + 1;
+  int g[3] = {gd.x, gd.y, gd.z}, pad[3], of[3];
+  for (int a = 0; a < 3; ++a) {
+    pad[a] = std::max(std::min(dmax, g[a]), nu);
+    of[a] = std::max(pad[a] / 2, nu / 2);
+  }
+  long n = (long)g[0] * g[1] * g[2], ns = (long)sd.x * sd.y * sd.z;
+  std::vector<double> q(n), e(n);
+  for (int i = 0; i < g[0]; ++i)
+    for (int j = 0; j < g[1]; ++j)
+      for (int k = 0; k < g[2]; ++k)
+        q[((long)i * g[1] + j) * g[2] + k] =
+            ql[((long)(i + of[0]) * (g[1] + pad[1]) + (j + of[1])) * (g[2] + pad[2]) + k + of[2]];
+  double *dq = nullptr, *de = nullptr, *dk = nullptr;
+  bool ok = hipMalloc(&dq, n * 8) == hipSuccess && hipMalloc(&de, n * 8) == hipSuccess &&
+            hipMalloc(&dk, ns * 8) == hipSuccess;
+  if (ok) {
+    hipMemcpy(dq, q.data(), n * 8, hipMemcpyHostToDevice);
+    hipMemcpy(dk, kh, ns * 8, hipMemcpyHostToDevice);
+    msm::launch_level_fp64(E.P, {g[0], g[1], g[2]}, {1, 1, 1}, {sd.x, sd.y, sd.z}, dk, dq, nullptr,
+                           de, E.stream);
+    hipMemcpyAsync(e.data(), de, n * 8, hipMemcpyDeviceToHost, E.stream);
+    ok = hipStreamSynchronize(E.stream) == hipSuccess;
+  }
+  if (dq) hipFree(dq);
+  if (de) hipFree(de);
+  if (dk) hipFree(dk);
+  if (!ok) {
+    std::fprintf(stderr, "msmpb_hip: grid2grid failed\n");
+    return;
+  }
+  for (long m = 0; m < n; ++m) el[m] += e[m];
+  (void)l;
+}
+
+// ============================================================ msmpb_hip.h
+const char *FF_last_error(void) { return g_err.c_str(); }
+
+int FF_set_precision(FF *ff, int precision) {
+  if (!ff) return fail(MSM_ERR_ARG, "FF is NULL");
+  if (precision != MSM_PREC_FP64 && precision != MSM_PREC_MIXED)
+    return fail(MSM_ERR_ARG, "precision must be MSM_PREC_FP64 or MSM_PREC_MIXED");
+  X(ff)->precision = precision;
+  return MSM_OK;
+}
+int FF_get_precision(FF *ff) { return ff ? X(ff)->precision : -1; }
+
+int FF_set_layout_compat(FF *ff, int layout) {
+  if (!ff) return fail(MSM_ERR_ARG, "FF is NULL");
+  if (layout != MSM_LAYOUT_FIXED && layout != MSM_LAYOUT_ASIS)
+    return fail(MSM_ERR_ARG, "layout must be MSM_LAYOUT_FIXED or MSM_LAYOUT_ASIS");
+  X(ff)->layout = layout;
+  return MSM_OK;
+}
+
+int FF_set_stage_timing(FF *ff, int enable) {
+  if (!ff) return fail(MSM_ERR_ARG, "FF is NULL");
+  X(ff)->timing = enable;
+  return MSM_OK;
+}
+
+int FF_get_energy_terms(FF *ff, double terms[3]) {
+  TRY(check_built(ff));
+  std::memcpy(terms, X(ff)->eng->last_terms, 3 * sizeof(double));
+  return MSM_OK;
+}
+
+double FF_energy_potential(FF *ff, double (*force)[3], double (*position)[3], double *potential) {
+  double E = NAN;
+  if (!potential) {
+    fail(MSM_ERR_ARG, "FF_energy_potential: potential is NULL");
+    return NAN;
+  }
+  if (energy_host(ff, force, position, potential, &E) != MSM_OK) return NAN;
+  return E;
+}
+
+int msm_eval_device(FF *ff, const double *d_pos, const double *d_q, double *d_energy,
+                    double *d_force, double *d_phi, void *stream) {
+  TRY(check_built(ff));
+  if (!d_pos || !d_q || !d_energy) return fail(MSM_ERR_ARG, "msm_eval_device: NULL input");
+  FFX *x = X(ff);
+  hipStream_t s = stream ? (hipStream_t)stream : x->eng->stream;
+  Timing T;
+  TRY(eval_chain(x, d_pos, d_q, d_energy, d_force, d_phi, s, T));
+  return MSM_OK;
+}
+
+int msm_forward_device(FF *ff, const double *d_pos, const double *d_q, double *d_energy, int keep,
+                       void *stream) {
+  TRY(check_built(ff));
+  if (!d_pos || !d_q || !d_energy) return fail(MSM_ERR_ARG, "msm_forward_device: NULL input");
+  FFX *x = X(ff);
+  Engine &E = *x->eng;
+  hipStream_t s = stream ? (hipStream_t)stream : E.stream;
+  Timing T;
+  stage_prepare(E, d_pos, d_q, s);
+  stage_pairs(E, x->precision, nullptr, nullptr, s);
+  stage_grids(E, x->precision, x->layout, T, s);
+  stage_energy(E, x->layout, s);
+  HIP_TRY(hipMemcpyAsync(d_energy, E.d_out, 4 * sizeof(double), hipMemcpyDeviceToDevice, s));
+  HIP_TRY(hipGetLastError());
+  E.kept = keep != 0;
+  E.kept_pos = d_pos;
+  E.kept_q = d_q;
+  return MSM_OK;
+}
+
+int msm_adjoint_device(FF *ff, const double *d_pos, const double *d_q, double *d_force,
+                       double *d_phi, int recompute, void *stream) {
+  TRY(check_built(ff));
+  if (!d_pos || !d_q) return fail(MSM_ERR_ARG, "msm_adjoint_device: NULL input");
+  FFX *x = X(ff);
+  Engine &E = *x->eng;
+  hipStream_t s = stream ? (hipStream_t)stream : E.stream;
+  Timing T;
+  if (recompute || !E.kept || E.kept_pos != d_pos || E.kept_q != d_q) {
+    // grid levels are not stored: rebuild the ordering and e^1 from scratch
+    stage_prepare(E, d_pos, d_q, s);
+    stage_grids(E, x->precision, x->layout, T, s);
+  }
+  stage_pairs(E, x->precision, d_force, d_phi, s);
+  if (d_force || d_phi)
+    msm::launch_gather(E.P, E.gd[1], E.d_spos, E.d_sidx, E.d_el[1], E.d_scal, d_force, d_phi, s);
+  HIP_TRY(hipGetLastError());
+  return MSM_OK;
+}
+
+int msm_grid_dims(FF *ff, int level, int dims[3]) {
+  TRY(check_built(ff));
+  Engine &E = *X(ff)->eng;
+  if (level < 1 || level > E.plan.L) return fail(MSM_ERR_ARG, "level out of range");
+  dims[0] = E.gd[level].x, dims[1] = E.gd[level].y, dims[2] = E.gd[level].z;
+  return MSM_OK;
+}
+
+int msm_stencil_dims(FF *ff, int level, int dims[3]) {
+  TRY(check_built(ff));
+  Engine &E = *X(ff)->eng;
+  if (level < 1 || level > E.plan.L) return fail(MSM_ERR_ARG, "level out of range");
+  dims[0] = E.sd[level].x, dims[1] = E.sd[level].y, dims[2] = E.sd[level].z;
+  return MSM_OK;
+}
+
+int msm_copy_grid(FF *ff, int level, int which, double *host_out) {
+  TRY(check_built(ff));
+  Engine &E = *X(ff)->eng;
+  if (level < 1 || level > E.plan.L) return fail(MSM_ERR_ARG, "level out of range");
+  const double *src = which ? E.d_el[level] : E.d_ql[level];
+  HIP_TRY(hipMemcpyAsync(host_out, src, E.gd[level].size() * sizeof(double),
+                         hipMemcpyDeviceToHost, E.stream));
+  HIP_TRY(hipStreamSynchronize(E.stream));
+  return MSM_OK;
+}
+
+int msm_copy_stencil(FF *ff, int level, double *host_out) {
+  TRY(check_built(ff));
+  Engine &E = *X(ff)->eng;
+  if (level < 1 || level > E.plan.L) return fail(MSM_ERR_ARG, "level out of range");
+  const std::vector<double> &k = E.plan.khat[level];
+  std::memcpy(host_out, k.data(), k.size() * sizeof(double));
+  return MSM_OK;
+}
+
+int msm_count_pairs(FF *ff, double (*position)[3], long long *count) {
+  TRY(check_built(ff));
+  if (!position || !count) return fail(MSM_ERR_ARG, "msm_count_pairs: NULL argument");
+  FFX *x = X(ff);
+  TRY(stage_host_inputs(x, position));
+  return count_pairs_dev(x, x->eng->d_pos, x->eng->d_q, count);
+}
+
+}  // extern "C"

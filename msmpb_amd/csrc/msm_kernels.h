@@ -1,0 +1,103 @@
+// msm_kernels.h -- device-side parameter block and launchers of the MSM
+// evaluation chain (implemented in msm_kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace msm {
+
+constexpr int kMaxNu = 10;
+
+// Passed by value to every kernel (lives in the kernarg segment, read with
+// scalar loads).  Orthorhombic boxes only: A and Ai are diagonal.
+struct DevParams {
+  double box[3];       // edge lengths (diagonal of A)
+  double ibox[3];      // 1 / edge lengths (diagonal of Ai)
+  int nc[3];           // pair-search cells per axis
+  int ncell;
+  double cw[3];        // cell widths
+  double a0, a02, inv_a0, inv_a02, two_inv_a03;
+  int nu;
+  int N;
+  double tau[kMaxNu];
+  double Q[kMaxNu * kMaxNu / 2];
+  double J[kMaxNu + 1];
+  double coeff_self;   // coeff1 - gamma(0)/a0 (per-charge constant potential)
+  double coeff1, coeff2, gamma0;
+};
+
+struct GridDims {
+  int x, y, z;
+  __host__ __device__ long size() const { return (long)x * y * z; }
+};
+
+// ---- launchers (stream-ordered, no synchronisation) ----
+void launch_fill_int(int *p, long n, int v, hipStream_t s);
+void launch_fill_dbl(double *p, long n, double v, hipStream_t s);
+
+// Binning: wrap positions into the box, count atoms per cell, partial sums
+// of q and q^2 per block (2 doubles per block into qpart).
+int bin_blocks(int N);
+void launch_bin(const DevParams &P, const double *pos, const double *q, int *cell_of, int *cnt,
+                double4 *wpos, double *qpart, int *flags, hipStream_t s);
+// Exclusive scan of cnt into start[0..ncell] (cnt is zeroed for reuse as fill
+// counters) and reduction of qpart into scal[0] = sum q, scal[1] = sum q^2.
+void launch_scan(const DevParams &P, int *cnt, int *start, const double *qpart, int nqpart,
+                 double *scal, hipStream_t s);
+void launch_scatter(const DevParams &P, const int *cell_of, const double4 *wpos, const int *start,
+                    int *fill, double4 *spos, int *sidx, int *scell, hipStream_t s);
+// Orders atoms inside each cell by original index (deterministic results)
+// and writes the cell-local FP32 coordinates used by the mixed pair kernel.
+void launch_cell_sort(const DevParams &P, const int *start, double4 *spos, int *sidx,
+                      float4 *lpos, hipStream_t s);
+
+// Short-range pairs, full neighbour shell, one thread per atom.  Writes
+// force (if F) and potential (if phi) at the original atom index, and one
+// energy partial per block into epart.
+int pair_blocks(int N);
+void launch_pair_fp64(const DevParams &P, const double4 *spos, const int *sidx, const int *scell,
+                      const int *start, const int4 *sten, int nsten, double *F, double *phi,
+                      double *epart, hipStream_t s);
+void launch_pair_fp32(const DevParams &P, const float4 *lpos, const int *sidx, const int *scell,
+                      const int *start, const int4 *sten, int nsten, double *F, double *phi,
+                      double *epart, hipStream_t s);
+// Pair count within the cutoff (i < j, all images counted once per pair).
+void launch_pair_count(const DevParams &P, const double4 *spos, const int *scell, const int *start,
+                       const int4 *sten, int nsten, unsigned long long *count, hipStream_t s);
+
+// B-spline spreading of charges onto the level-1 grid (grid zeroed here).
+void launch_spread(const DevParams &P, GridDims g, const double4 *spos, double *q1, hipStream_t s);
+// q^l = R q^{l-1} (full weighting with the two-scale stencil).
+void launch_restrict(const DevParams &P, GridDims fine, GridDims coarse, const double *qf,
+                     double *qc, hipStream_t s);
+// Top level: e^L = IDFT(DFT(q^L) * khat) / M^3.  work: 4 * M^3 doubles.
+// asis != 0 reads q^L with the reference HEAD's padded index map.
+void launch_top(const DevParams &P, GridDims g, const double *qL, const double *khat,
+                const double *tw, double *work, double *eL, int asis, hipStream_t s);
+// e^l = P e^{l+1} + K_l * q^l (prolongation fused with the stencil pass);
+// FP64 on the vector ALUs.
+void launch_level_fp64(const DevParams &P, GridDims g, GridDims gc, GridDims sd,
+                       const double *khat, const double *ql, const double *ec, double *el,
+                       hipStream_t s);
+// Same with the stencil correlation as BF16 hi/lo split MFMA products
+// accumulated in FP32 (khat_split: prepared by prepare_split_stencil).
+void launch_level_mfma(const DevParams &P, GridDims g, GridDims gc, GridDims sd,
+                       const void *khat_split, const double *ql, const double *ec, double *el,
+                       void *qsplit, hipStream_t s);
+size_t split_stencil_bytes(GridDims sd);
+size_t split_grid_bytes(GridDims g, GridDims sd);
+void prepare_split_stencil(GridDims sd, const double *khat, void *host_out);
+
+// Grid energy partials: sum q1*e1 per block (asis != 0: the reference
+// HEAD's L = 1 index map).
+int grid_blocks(long n);
+void launch_grid_energy(const DevParams &P, GridDims g, const double *q1, const double *e1,
+                        double *epart, int asis, hipStream_t s);
+// Interpolation: F += q E and phi += grid potential + constant self terms.
+void launch_gather(const DevParams &P, GridDims g, const double4 *spos, const int *sidx,
+                   const double *e1, const double *scal, double *F, double *phi, hipStream_t s);
+// Final energy: out = {E, constant part, pair part, grid part}.
+void launch_finalize(const DevParams &P, const double *epair, int npair, const double *egrid,
+                     int ngrid, const double *scal, double *out, hipStream_t s);
+
+}  // namespace msm

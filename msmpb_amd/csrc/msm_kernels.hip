@@ -1,0 +1,924 @@
+// msm_kernels.hip -- HIP kernels of one MSM energy/force evaluation on
+// gfx950 (MI355X).
+//
+// The chain restates FF_energy of hkayabilisim/msmpb
+// (code/forcefield1.c:101-235) for a GPU: atoms are binned into cells and
+// sorted every call (instead of walking a stored neighbour list), the pair
+// sum runs over a full neighbour-cell shell (no atomics on forces), the
+// level grids are periodic and unpadded, and prolongation is fused into the
+// level stencil pass.  Wave size is 64.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+
+#include "msm_kernels.h"
+
+namespace msm {
+namespace {
+
+__device__ __forceinline__ int wrapi(int v, int m) {
+  int r = v % m;
+  return r < 0 ? r + m : r;
+}
+
+__device__ __forceinline__ int floordiv(int v, int m) {
+  return v >= 0 ? v / m : -((-v + m - 1) / m);
+}
+
+// Sum over a 64-wide wavefront, then over the waves of a block through LDS.
+// Returns the block total in thread 0.
+template <int BS>
+__device__ __forceinline__ double block_sum(double v) {
+  __shared__ double red[BS / 64];
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) red[w] = v;
+  __syncthreads();
+  double t = 0;
+  if (threadIdx.x == 0)
+    for (int k = 0; k < BS / 64; ++k) t += red[k];
+  return t;
+}
+
+// Weights of the order-NU B-spline at the NU grid points around t in [0,1):
+// w[i] = Q_NU(i + t) from the stored first NU/2 pieces, the other half by
+// symmetry (forcefield1.c:609-637).  dw = d/dt.
+template <int NU, bool DERIV>
+__device__ __forceinline__ void bspline(const double *Q, double t, double *w, double *dw) {
+#pragma unroll
+  for (int i = 0; i < NU; ++i) {
+    int p = i < NU / 2 ? i : NU - 1 - i;
+    double u = i < NU / 2 ? t : 1.0 - t;
+    const double *c = Q + p * NU;
+    double v = c[NU - 1], d = 0.0;
+#pragma unroll
+    for (int k = NU - 2; k >= 0; --k) {
+      if (DERIV) d = v + u * d;
+      v = c[k] + u * v;
+    }
+    w[i] = v;
+    if (DERIV) dw[i] = i < NU / 2 ? d : -d;
+  }
+}
+
+// One weight Q_NU(i + t) (and its t-derivative) for a uniform index i.
+template <int NU, bool DERIV>
+__device__ __forceinline__ double bspline_one(const double *Q, int i, double t, double *dw) {
+  int p = i < NU / 2 ? i : NU - 1 - i;
+  double u = i < NU / 2 ? t : 1.0 - t;
+  const double *c = Q + p * NU;
+  double v = c[NU - 1], d = 0.0;
+#pragma unroll
+  for (int k = NU - 2; k >= 0; --k) {
+    if (DERIV) d = v + u * d;
+    v = c[k] + u * v;
+  }
+  if (DERIV) *dw = i < NU / 2 ? d : -d;
+  return v;
+}
+
+// ---------------------------------------------------------------- utilities
+__global__ void k_fill_int(int *p, long n, int v) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+__global__ void k_fill_dbl(double *p, long n, double v) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// ---------------------------------------------------------------- binning
+constexpr int kBinBS = 256;
+
+__global__ __launch_bounds__(kBinBS) void k_bin(DevParams P, const double *__restrict__ pos,
+                                                const double *__restrict__ q,
+                                                int *__restrict__ cell_of, int *__restrict__ cnt,
+                                                double4 *__restrict__ wpos,
+                                                double *__restrict__ qpart, int *__restrict__ flags) {
+  int i = blockIdx.x * kBinBS + threadIdx.x;
+  double qi = 0.0;
+  if (i < P.N) {
+    double r[3] = {pos[3 * (long)i], pos[3 * (long)i + 1], pos[3 * (long)i + 2]};
+    qi = q[i];
+    int c[3];
+    double rw[3];
+    bool ok = isfinite(qi);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      double s = r[a] * P.ibox[a];
+      double f = floor(s);
+      double sw = s - f;
+      ok = ok && isfinite(s);
+      // s - floor(s) may round up to 1 for tiny negative s; keep in range
+      int ci = (int)(sw * P.nc[a]);
+      if (!(sw >= 0.0) || ci < 0) ci = 0;
+      if (ci >= P.nc[a]) ci = P.nc[a] - 1;
+      c[a] = ci;
+      rw[a] = r[a] - f * P.box[a];
+      if (!isfinite(rw[a])) rw[a] = 0.0;
+    }
+    if (!ok) atomicAdd(flags, 1);
+    int cell = (c[0] * P.nc[1] + c[1]) * P.nc[2] + c[2];
+    cell_of[i] = cell;
+    atomicAdd(cnt + cell, 1);
+    wpos[i] = make_double4(rw[0], rw[1], rw[2], qi);
+  }
+  double s1 = block_sum<kBinBS>(qi);
+  __syncthreads();
+  double s2 = block_sum<kBinBS>(qi * qi);
+  if (threadIdx.x == 0) {
+    qpart[2 * blockIdx.x] = s1;
+    qpart[2 * blockIdx.x + 1] = s2;
+  }
+}
+
+constexpr int kScanBS = 1024;
+
+__global__ __launch_bounds__(kScanBS) void k_scan(int n, int *__restrict__ cnt,
+                                                  int *__restrict__ start,
+                                                  const double *__restrict__ qpart, int nq,
+                                                  double *__restrict__ scal) {
+  __shared__ int part[kScanBS];
+  int t = threadIdx.x;
+  int chunk = (n + kScanBS - 1) / kScanBS;
+  int b = t * chunk, e = min(n, b + chunk);
+  int s = 0;
+  for (int k = b; k < e; ++k) s += cnt[k];
+  part[t] = s;
+  __syncthreads();
+  for (int off = 1; off < kScanBS; off <<= 1) {  // inclusive Hillis-Steele scan
+    int v = t >= off ? part[t - off] : 0;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  int run = part[t] - s;
+  for (int k = b; k < e; ++k) {
+    int c = cnt[k];
+    start[k] = run;
+    run += c;
+    cnt[k] = 0;
+  }
+  if (t == kScanBS - 1) start[n] = part[t];
+  // charge sums for the constant energy terms
+  double q1 = 0, q2 = 0;
+  for (int k = t; k < nq; k += kScanBS) q1 += qpart[2 * k], q2 += qpart[2 * k + 1];
+  __syncthreads();
+  double a = block_sum<kScanBS>(q1);
+  __syncthreads();
+  double c2 = block_sum<kScanBS>(q2);
+  if (t == 0) scal[0] = a, scal[1] = c2;
+}
+
+__global__ __launch_bounds__(256) void k_scatter(int N, const int *__restrict__ cell_of,
+                                                 const double4 *__restrict__ wpos,
+                                                 const int *__restrict__ start,
+                                                 int *__restrict__ fill,
+                                                 double4 *__restrict__ spos,
+                                                 int *__restrict__ sidx,
+                                                 int *__restrict__ scell) {
+  int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= N) return;
+  int c = cell_of[i];
+  int slot = start[c] + atomicAdd(fill + c, 1);
+  spos[slot] = wpos[i];
+  sidx[slot] = i;
+  scell[slot] = c;
+}
+
+__global__ __launch_bounds__(256) void k_cell_sort(DevParams P, const int *__restrict__ start,
+                                                   double4 *__restrict__ spos,
+                                                   int *__restrict__ sidx,
+                                                   float4 *__restrict__ lpos) {
+  int c = blockIdx.x * 256 + threadIdx.x;
+  if (c >= P.ncell) return;
+  int b = start[c], e = start[c + 1];
+  for (int k = b + 1; k < e; ++k) {  // insertion sort by original index
+    int key = sidx[k];
+    double4 v = spos[k];
+    int m = k - 1;
+    while (m >= b && sidx[m] > key) {
+      sidx[m + 1] = sidx[m];
+      spos[m + 1] = spos[m];
+      --m;
+    }
+    sidx[m + 1] = key;
+    spos[m + 1] = v;
+  }
+  int cz = c % P.nc[2], cy = (c / P.nc[2]) % P.nc[1], cx = c / (P.nc[1] * P.nc[2]);
+  double ox = cx * P.cw[0], oy = cy * P.cw[1], oz = cz * P.cw[2];
+  for (int k = b; k < e; ++k) {
+    double4 v = spos[k];
+    lpos[k] = make_float4((float)(v.x - ox), (float)(v.y - oy), (float)(v.z - oz), (float)v.w);
+  }
+}
+
+// ---------------------------------------------------------------- pairs
+// partcl2partcl (forcefield1.c:240-321) as a full-shell gather: each atom
+// sums q_j g(r) over every neighbour j != i and every periodic image inside
+// the cutoff, with g(r) = 1/r - tau(r^2/a0^2 - 1)/a0.  Neighbour cells are
+// visited through a stencil of cell offsets; an offset that leaves the box
+// carries the image shift, so small boxes get every image exactly once.
+constexpr int kPairBS = 256;
+
+__global__ __launch_bounds__(kPairBS) void k_pair64(
+    DevParams P, const double4 *__restrict__ spos, const int *__restrict__ sidx,
+    const int *__restrict__ scell, const int *__restrict__ start, const int4 *__restrict__ sten,
+    int nsten, double *__restrict__ F, double *__restrict__ phi, double *__restrict__ epart) {
+  int i = blockIdx.x * kPairBS + threadIdx.x;
+  double acc = 0.0, fx = 0.0, fy = 0.0, fz = 0.0, qi = 0.0;
+  if (i < P.N) {
+    double4 ai = spos[i];
+    qi = ai.w;
+    int c = scell[i];
+    int cz = c % P.nc[2], cy = (c / P.nc[2]) % P.nc[1], cx = c / (P.nc[1] * P.nc[2]);
+    const int nu = P.nu;
+    for (int o = 0; o < nsten; ++o) {
+      int4 d = sten[o];
+      int ox = cx + d.x, oy = cy + d.y, oz = cz + d.z;
+      int kx = floordiv(ox, P.nc[0]), ky = floordiv(oy, P.nc[1]), kz = floordiv(oz, P.nc[2]);
+      ox -= kx * P.nc[0], oy -= ky * P.nc[1], oz -= kz * P.nc[2];
+      double bx = kx * P.box[0] - ai.x, by = ky * P.box[1] - ai.y, bz = kz * P.box[2] - ai.z;
+      int cn = (ox * P.nc[1] + oy) * P.nc[2] + oz;
+      int jb = start[cn], je = start[cn + 1];
+      for (int j = jb; j < je; ++j) {
+        double4 aj = spos[j];
+        double dx = aj.x + bx, dy = aj.y + by, dz = aj.z + bz;
+        double r2 = dx * dx + dy * dy + dz * dz;
+        if (r2 < P.a02 && j != i) {
+          double s = r2 * P.inv_a02 - 1.0;
+          double ts = P.tau[nu - 1], t1 = 0.0;
+          for (int k = nu - 2; k >= 0; --k) {
+            t1 = ts + s * t1;
+            ts = P.tau[k] + s * ts;
+          }
+          double rinv = 1.0 / sqrt(r2);
+          double e = rinv - ts * P.inv_a0;
+          double cf = -rinv * rinv * rinv - t1 * P.two_inv_a03;
+          acc += aj.w * e;
+          double qc = aj.w * cf;
+          fx += qc * dx, fy += qc * dy, fz += qc * dz;
+        }
+      }
+    }
+    int oi = sidx[i];
+    if (F) {
+      F[3 * (long)oi] = qi * fx;
+      F[3 * (long)oi + 1] = qi * fy;
+      F[3 * (long)oi + 2] = qi * fz;
+    }
+    if (phi) phi[oi] = acc;
+  }
+  double t = block_sum<kPairBS>(0.5 * qi * acc);
+  if (threadIdx.x == 0) epart[blockIdx.x] = t;
+}
+
+// Mixed precision: the same sum with FP32 pair arithmetic on cell-local
+// coordinates (|x| < cell width, so differences lose no bits to the box
+// size) and FP64 accumulation per atom.
+__global__ __launch_bounds__(kPairBS) void k_pair32(
+    DevParams P, const float4 *__restrict__ lpos, const int *__restrict__ sidx,
+    const int *__restrict__ scell, const int *__restrict__ start, const int4 *__restrict__ sten,
+    int nsten, double *__restrict__ F, double *__restrict__ phi, double *__restrict__ epart) {
+  int i = blockIdx.x * kPairBS + threadIdx.x;
+  double acc = 0.0, fx = 0.0, fy = 0.0, fz = 0.0, qi = 0.0;
+  if (i < P.N) {
+    float4 ai = lpos[i];
+    qi = ai.w;
+    int c = scell[i];
+    int cz = c % P.nc[2], cy = (c / P.nc[2]) % P.nc[1], cx = c / (P.nc[1] * P.nc[2]);
+    const int nu = P.nu;
+    const float a02 = (float)P.a02, ia02 = (float)P.inv_a02, ia0 = (float)P.inv_a0;
+    const float c3 = (float)P.two_inv_a03;
+    float tau[kMaxNu];
+#pragma unroll
+    for (int k = 0; k < kMaxNu; ++k) tau[k] = (float)P.tau[k];
+    const float cwx = (float)P.cw[0], cwy = (float)P.cw[1], cwz = (float)P.cw[2];
+    for (int o = 0; o < nsten; ++o) {
+      int4 d = sten[o];
+      int ox = cx + d.x, oy = cy + d.y, oz = cz + d.z;
+      int kx = floordiv(ox, P.nc[0]), ky = floordiv(oy, P.nc[1]), kz = floordiv(oz, P.nc[2]);
+      ox -= kx * P.nc[0], oy -= ky * P.nc[1], oz -= kz * P.nc[2];
+      // neighbour-cell origin relative to the home-cell origin
+      float bx = d.x * cwx - ai.x, by = d.y * cwy - ai.y, bz = d.z * cwz - ai.z;
+      int cn = (ox * P.nc[1] + oy) * P.nc[2] + oz;
+      int jb = start[cn], je = start[cn + 1];
+      for (int j = jb; j < je; ++j) {
+        float4 aj = lpos[j];
+        float dx = aj.x + bx, dy = aj.y + by, dz = aj.z + bz;
+        float r2 = dx * dx + dy * dy + dz * dz;
+        if (r2 < a02 && j != i) {
+          float s = r2 * ia02 - 1.0f;
+          float ts = tau[kMaxNu - 1], t1 = 0.0f;
+          // tau holds zeros above nu-1, so the fixed-length Horner is exact
+#pragma unroll
+          for (int k = kMaxNu - 2; k >= 0; --k) {
+            t1 = ts + s * t1;
+            ts = tau[k] + s * ts;
+          }
+          float rinv = rsqrtf(r2);
+          float e = rinv - ts * ia0;
+          float cf = -rinv * rinv * rinv - t1 * c3;
+          float qc = aj.w * cf;
+          acc += (double)(aj.w * e);
+          fx += (double)(qc * dx), fy += (double)(qc * dy), fz += (double)(qc * dz);
+        }
+      }
+    }
+    (void)nu;
+    int oi = sidx[i];
+    if (F) {
+      F[3 * (long)oi] = qi * fx;
+      F[3 * (long)oi + 1] = qi * fy;
+      F[3 * (long)oi + 2] = qi * fz;
+    }
+    if (phi) phi[oi] = acc;
+  }
+  double t = block_sum<kPairBS>(0.5 * qi * acc);
+  if (threadIdx.x == 0) epart[blockIdx.x] = t;
+}
+
+__global__ __launch_bounds__(kPairBS) void k_pair_count(
+    DevParams P, const double4 *__restrict__ spos, const int *__restrict__ scell,
+    const int *__restrict__ start, const int4 *__restrict__ sten, int nsten,
+    unsigned long long *__restrict__ count) {
+  int i = blockIdx.x * kPairBS + threadIdx.x;
+  double n = 0;
+  if (i < P.N) {
+    double4 ai = spos[i];
+    int c = scell[i];
+    int cz = c % P.nc[2], cy = (c / P.nc[2]) % P.nc[1], cx = c / (P.nc[1] * P.nc[2]);
+    for (int o = 0; o < nsten; ++o) {
+      int4 d = sten[o];
+      int ox = cx + d.x, oy = cy + d.y, oz = cz + d.z;
+      int kx = floordiv(ox, P.nc[0]), ky = floordiv(oy, P.nc[1]), kz = floordiv(oz, P.nc[2]);
+      ox -= kx * P.nc[0], oy -= ky * P.nc[1], oz -= kz * P.nc[2];
+      double bx = kx * P.box[0] - ai.x, by = ky * P.box[1] - ai.y, bz = kz * P.box[2] - ai.z;
+      int cn = (ox * P.nc[1] + oy) * P.nc[2] + oz;
+      for (int j = start[cn]; j < start[cn + 1]; ++j) {
+        if (j <= i) continue;
+        double4 aj = spos[j];
+        double dx = aj.x + bx, dy = aj.y + by, dz = aj.z + bz;
+        // count each pair once, at its nearest image
+        if (rint(dx * P.ibox[0]) != 0.0 || rint(dy * P.ibox[1]) != 0.0 ||
+            rint(dz * P.ibox[2]) != 0.0)
+          continue;
+        if (dx * dx + dy * dy + dz * dz < P.a02) n += 1.0;
+      }
+    }
+  }
+  double t = block_sum<kPairBS>(n);
+  if (threadIdx.x == 0) atomicAdd(count, (unsigned long long)t);
+}
+
+// ---------------------------------------------------------------- spread
+// anterpolate (forcefield1.c:324-368): q^1_{m-i+NU/2} += q Q(i+t) per axis.
+// Atoms arrive sorted by cell, so a wavefront's atomics hit a compact
+// region of the grid that stays resident in L2.
+template <int NU>
+__global__ __launch_bounds__(256) void k_spread(DevParams P, GridDims g,
+                                                const double4 *__restrict__ spos,
+                                                double *__restrict__ q1) {
+  int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= P.N) return;
+  double4 a = spos[i];
+  double r[3] = {a.x, a.y, a.z};
+  int dims[3] = {g.x, g.y, g.z};
+  double t[3];
+  int m[3];
+#pragma unroll
+  for (int ax = 0; ax < 3; ++ax) {
+    double s = r[ax] * P.ibox[ax];
+    s = s - floor(s);
+    double tt = dims[ax] * s, mm = floor(tt);
+    t[ax] = tt - mm;
+    m[ax] = (int)mm;
+  }
+  double wz[NU];
+  int iz[NU];
+  bspline<NU, false>(P.Q, t[2], wz, nullptr);
+#pragma unroll
+  for (int k = 0; k < NU; ++k) iz[k] = wrapi(m[2] - k + NU / 2, g.z);
+#pragma unroll 1
+  for (int x = 0; x < NU; ++x) {
+    double wx = a.w * bspline_one<NU, false>(P.Q, x, t[0], nullptr);
+    long xo = (long)wrapi(m[0] - x + NU / 2, g.x) * g.y;
+#pragma unroll 1
+    for (int y = 0; y < NU; ++y) {
+      double wxy = wx * bspline_one<NU, false>(P.Q, y, t[1], nullptr);
+      double *row = q1 + (xo + wrapi(m[1] - y + NU / 2, g.y)) * g.z;
+#pragma unroll
+      for (int z = 0; z < NU; ++z) unsafeAtomicAdd(row + iz[z], wxy * wz[z]);
+    }
+  }
+}
+
+// ---------------------------------------------------------------- restrict
+// restrict_ (forcefield1.c:401-477): q^l_m = sum_n J_nx J_ny J_nz
+// q^{l-1}_{2m+n}, n in [-NU/2, NU/2]^3, periodic.
+template <int NU>
+__global__ __launch_bounds__(256) void k_restrict(DevParams P, GridDims f, GridDims c,
+                                                  const double *__restrict__ qf,
+                                                  double *__restrict__ qc) {
+  long id = (long)blockIdx.x * 256 + threadIdx.x;
+  if (id >= c.size()) return;
+  int z = id % c.z, y = (id / c.z) % c.y, x = id / ((long)c.z * c.y);
+  const int H = NU / 2;
+  int iz[NU + 1];
+#pragma unroll
+  for (int n = 0; n <= NU; ++n) iz[n] = wrapi(2 * z + n - H, f.z);
+  double acc = 0.0;
+  for (int nx = 0; nx <= NU; ++nx) {
+    long xo = (long)wrapi(2 * x + nx - H, f.x) * f.y;
+    double sy = 0.0;
+    for (int ny = 0; ny <= NU; ++ny) {
+      const double *row = qf + (xo + wrapi(2 * y + ny - H, f.y)) * f.z;
+      double sz = 0.0;
+#pragma unroll
+      for (int nz = 0; nz <= NU; ++nz) sz += P.J[nz] * row[iz[nz]];
+      sy += P.J[ny] * sz;
+    }
+    acc += P.J[nx] * sy;
+  }
+  qc[id] = acc;
+}
+
+// ---------------------------------------------------------------- top level
+// FFTg2g (forcefield1.c:499-528): separable DFT along z, y, x (sign -1),
+// pointwise product with khat, inverse (sign +1), real part / M^3.
+// tw holds cos/sin(2 pi j / M_axis) for the three axes back to back.
+__device__ __forceinline__ double top_input(GridDims g, const double *qL, int x, int y, int z,
+                                            int asis, int nu) {
+  if (!asis) return qL[((long)x * g.y + y) * g.z + z];
+  // reference HEAD (forcefield1.c:504-515 vs :406-415): unpadded data read
+  // with the padded stride M + nu - 1 and offset nu/2 - 1
+  int o = nu / 2 - 1;
+  long f = ((long)(x + o) * (g.y + nu - 1) + (y + o)) * (g.z + nu - 1) + (z + o);
+  return f < g.size() ? qL[f] : 0.0;
+}
+
+constexpr int kTopBS = 1024;
+constexpr int kTopSmall = 2048;  // largest M^3 done in one workgroup (LDS)
+
+__device__ void dft_pass_lds(GridDims g, int axis, double sign, double *re, double *im,
+                             const double *tw) {
+  int M = axis == 0 ? g.x : axis == 1 ? g.y : g.z;
+  const double *c = tw + 2 * (axis == 0 ? 0 : axis == 1 ? g.x : g.x + g.y);
+  long stride = axis == 0 ? (long)g.y * g.z : axis == 1 ? g.z : 1;
+  long n = g.size();
+  double orr[kTopSmall / kTopBS], oii[kTopSmall / kTopBS];
+  int cnt = 0;
+  for (long e = threadIdx.x; e < n; e += kTopBS, ++cnt) {
+    int k = axis == 0 ? e / ((long)g.y * g.z) : axis == 1 ? (e / g.z) % g.y : e % g.z;
+    long base = e - k * stride;
+    double sr = 0, si = 0;
+    for (int m = 0; m < M; ++m) {
+      int km = (int)(((long)k * m) % M);
+      double cr = c[2 * km], ci = sign * c[2 * km + 1];
+      double xr = re[base + m * stride], xi = im[base + m * stride];
+      sr += xr * cr - xi * ci;
+      si += xr * ci + xi * cr;
+    }
+    orr[cnt] = sr, oii[cnt] = si;
+  }
+  __syncthreads();
+  cnt = 0;
+  for (long e = threadIdx.x; e < n; e += kTopBS, ++cnt) re[e] = orr[cnt], im[e] = oii[cnt];
+  __syncthreads();
+}
+
+__global__ __launch_bounds__(kTopBS) void k_top_small(DevParams P, GridDims g,
+                                                      const double *__restrict__ qL,
+                                                      const double *__restrict__ khat,
+                                                      const double *__restrict__ tw,
+                                                      double *__restrict__ eL, int asis) {
+  __shared__ double re[kTopSmall], im[kTopSmall];
+  long n = g.size();
+  for (long e = threadIdx.x; e < n; e += kTopBS) {
+    int z = e % g.z, y = (e / g.z) % g.y, x = e / ((long)g.y * g.z);
+    re[e] = top_input(g, qL, x, y, z, asis, P.nu);
+    im[e] = 0.0;
+  }
+  __syncthreads();
+  dft_pass_lds(g, 2, -1.0, re, im, tw);
+  dft_pass_lds(g, 1, -1.0, re, im, tw);
+  dft_pass_lds(g, 0, -1.0, re, im, tw);
+  for (long e = threadIdx.x; e < n; e += kTopBS) re[e] *= khat[e], im[e] *= khat[e];
+  __syncthreads();
+  dft_pass_lds(g, 0, 1.0, re, im, tw);
+  dft_pass_lds(g, 1, 1.0, re, im, tw);
+  dft_pass_lds(g, 2, 1.0, re, im, tw);
+  for (long e = threadIdx.x; e < n; e += kTopBS) eL[e] = re[e] / (double)n;
+}
+
+// Large top grids (L = 1 with a fine grid): one launch per pass, global memory.
+__global__ __launch_bounds__(256) void k_top_pack(DevParams P, GridDims g,
+                                                  const double *__restrict__ qL,
+                                                  double *__restrict__ re,
+                                                  double *__restrict__ im, int asis) {
+  long e = (long)blockIdx.x * 256 + threadIdx.x;
+  if (e >= g.size()) return;
+  int z = e % g.z, y = (e / g.z) % g.y, x = e / ((long)g.y * g.z);
+  re[e] = top_input(g, qL, x, y, z, asis, P.nu);
+  im[e] = 0.0;
+}
+
+__global__ __launch_bounds__(256) void k_dft_axis(GridDims g, int axis, double sign,
+                                                  const double *__restrict__ ire,
+                                                  const double *__restrict__ iim,
+                                                  double *__restrict__ ore,
+                                                  double *__restrict__ oim,
+                                                  const double *__restrict__ tw,
+                                                  const double *__restrict__ mul) {
+  long e = (long)blockIdx.x * 256 + threadIdx.x;
+  if (e >= g.size()) return;
+  int M = axis == 0 ? g.x : axis == 1 ? g.y : g.z;
+  const double *c = tw + 2 * (axis == 0 ? 0 : axis == 1 ? g.x : g.x + g.y);
+  long stride = axis == 0 ? (long)g.y * g.z : axis == 1 ? g.z : 1;
+  int k = axis == 0 ? e / ((long)g.y * g.z) : axis == 1 ? (e / g.z) % g.y : e % g.z;
+  long base = e - k * stride;
+  double sr = 0, si = 0;
+  for (int m = 0; m < M; ++m) {
+    int km = (int)(((long)k * m) % M);
+    double cr = c[2 * km], ci = sign * c[2 * km + 1];
+    double xr = ire[base + m * stride], xi = iim[base + m * stride];
+    sr += xr * cr - xi * ci;
+    si += xr * ci + xi * cr;
+  }
+  if (mul) sr *= mul[e], si *= mul[e];
+  ore[e] = sr;
+  oim[e] = si;
+}
+
+__global__ __launch_bounds__(256) void k_top_out(long n, const double *__restrict__ re,
+                                                 double *__restrict__ eL) {
+  long e = (long)blockIdx.x * 256 + threadIdx.x;
+  if (e < n) eL[e] = re[e] / (double)n;
+}
+
+// ---------------------------------------------------------------- levels
+// e^l = P e^{l+1} + K_l * q^l.  The stencil pass is grid2grid
+// (forcefield0.c:898-931): e_m = sum_k khat_k q_{m+k-s/2}, periodic.
+// One wavefront computes a (LY x LZ*ZB) tile of one x-plane: for each of
+// the s_x input planes it stages a (LY+s_y-1) x (LZ*ZB+s_z-1) slab in LDS,
+// and every lane slides ZB outputs along z through register windows of
+// KC taps.  Prolongation (forcefield1.c:530-567) is gathered per output:
+// e_{2m+n} += J_n e^{l+1}_m.
+constexpr int kZB = 8;
+constexpr int kKC = 8;
+
+template <int NU>
+__device__ __forceinline__ double prolong_at(const double *J, GridDims gc, const double *ec,
+                                             int x, int y, int z) {
+  // fine index v receives J_n from coarse m with 2m + n == v (mod 2 M_c),
+  // i.e. n has the parity of v and m = (v - n)/2 mod M_c
+  const int H = NU / 2;
+  double acc = 0.0;
+  for (int nx = -H; nx <= H; ++nx) {
+    if (((x - nx) & 1) != 0) continue;
+    int mx = wrapi((x - nx) / 2, gc.x);
+    double sy = 0.0;
+    for (int ny = -H; ny <= H; ++ny) {
+      if (((y - ny) & 1) != 0) continue;
+      int my = wrapi((y - ny) / 2, gc.y);
+      const double *row = ec + ((long)mx * gc.y + my) * gc.z;
+      double sz = 0.0;
+      for (int nz = -H; nz <= H; ++nz) {
+        if (((z - nz) & 1) != 0) continue;
+        sz += J[nz + H] * row[wrapi((z - nz) / 2, gc.z)];
+      }
+      sy += J[ny + H] * sz;
+    }
+    acc += J[nx + H] * sy;
+  }
+  return acc;
+}
+
+template <int NU>
+__global__ __launch_bounds__(64) void k_level64(DevParams P, GridDims g, GridDims gc, GridDims sd,
+                                                int LZ, int LY, const double *__restrict__ khat,
+                                                const double *__restrict__ ql,
+                                                const double *__restrict__ ec,
+                                                double *__restrict__ el) {
+  extern __shared__ double slab[];
+  int tid = threadIdx.x;
+  int lz = tid % LZ, ly = tid / LZ;
+  int tz = LZ * kZB;
+  int ztiles = (g.z + tz - 1) / tz, ytiles = (g.y + LY - 1) / LY;
+  int b = blockIdx.x;
+  int zt = b % ztiles;
+  b /= ztiles;
+  int yt = b % ytiles;
+  int x = b / ytiles;
+  int z0 = zt * tz, y0 = yt * LY;
+  int rows = LY + sd.y - 1, pitch = tz + sd.z - 1 + kKC;  // +KC: window over-read pad
+  double acc[kZB];
+#pragma unroll
+  for (int t = 0; t < kZB; ++t) acc[t] = 0.0;
+  for (int kx = 0; kx < sd.x; ++kx) {
+    int xi = wrapi(x + kx - sd.x / 2, g.x);
+    const double *plane = ql + (long)xi * g.y * g.z;
+    for (int e = tid; e < rows * pitch; e += 64) {
+      int r = e / pitch, cc = e % pitch;
+      int yy = wrapi(y0 + r - sd.y / 2, g.y), zz = wrapi(z0 + cc - sd.z / 2, g.z);
+      slab[e] = plane[(long)yy * g.z + zz];
+    }
+    __syncthreads();
+    if (ly < LY) {
+      const double *kp = khat + (long)kx * sd.y * sd.z;
+      for (int ky = 0; ky < sd.y; ++ky) {
+        const double *row = slab + (ly + ky) * pitch + lz * kZB;
+        const double *kr = kp + ky * sd.z;
+        for (int k0 = 0; k0 < sd.z; k0 += kKC) {
+          double w[kZB + kKC - 1];
+#pragma unroll
+          for (int t = 0; t < kZB + kKC - 1; ++t) w[t] = row[k0 + t];
+#pragma unroll
+          for (int k = 0; k < kKC; ++k) {
+            double kv = (k0 + k < sd.z) ? kr[k0 + k] : 0.0;
+#pragma unroll
+            for (int t = 0; t < kZB; ++t) acc[t] += kv * w[t + k];
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+  if (ly < LY) {
+    int y = y0 + ly;
+    if (y < g.y) {
+#pragma unroll
+      for (int t = 0; t < kZB; ++t) {
+        int z = z0 + lz * kZB + t;
+        if (z < g.z) {
+          double pv = ec ? prolong_at<NU>(P.J, gc, ec, x, y, z) : 0.0;
+          el[((long)x * g.y + y) * g.z + z] = pv + acc[t];
+        }
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------- energy
+constexpr int kGridBS = 256;
+
+__global__ __launch_bounds__(kGridBS) void k_grid_energy(DevParams P, GridDims g,
+                                                         const double *__restrict__ q1,
+                                                         const double *__restrict__ e1,
+                                                         double *__restrict__ epart, int asis) {
+  long n = g.size();
+  double s = 0.0;
+  for (long e = (long)blockIdx.x * kGridBS + threadIdx.x; e < n; e += (long)gridDim.x * kGridBS) {
+    double qv;
+    if (!asis) {
+      qv = q1[e];
+    } else {
+      // reference HEAD at L = 1 (forcefield1.c:196-212): the padded array
+      // (dims M+nu-1, interior at nu/2-1, zero ghosts) read at flat index e
+      int nu = P.nu, o = nu / 2 - 1;
+      long py = g.y + nu - 1, pz = g.z + nu - 1;
+      long a = e / (py * pz), b = (e / pz) % py, c = e % pz;
+      bool in = a >= o && a < o + g.x && b >= o && b < o + g.y && c >= o && c < o + g.z;
+      qv = in ? q1[((a - o) * g.y + (b - o)) * g.z + (c - o)] : 0.0;
+    }
+    s += qv * e1[e];
+  }
+  double t = block_sum<kGridBS>(s);
+  if (threadIdx.x == 0) epart[blockIdx.x] = t;
+}
+
+// interpolate (forcefield1.c:570-607) + F += q E (:228-231), and the
+// potential phi_i = dE/dq_i: grid part plus the constant self terms.
+template <int NU>
+__global__ __launch_bounds__(256) void k_gather(DevParams P, GridDims g,
+                                                const double4 *__restrict__ spos,
+                                                const int *__restrict__ sidx,
+                                                const double *__restrict__ e1,
+                                                const double *__restrict__ scal,
+                                                double *__restrict__ F, double *__restrict__ phi) {
+  int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= P.N) return;
+  double4 a = spos[i];
+  double r[3] = {a.x, a.y, a.z};
+  int dims[3] = {g.x, g.y, g.z};
+  double t[3];
+  int m[3];
+#pragma unroll
+  for (int ax = 0; ax < 3; ++ax) {
+    double tt = dims[ax] * (r[ax] * P.ibox[ax]), mm = floor(tt);
+    t[ax] = tt - mm;
+    m[ax] = (int)mm;
+  }
+  double wz[NU], dwz[NU];
+  int iz[NU];
+  bspline<NU, true>(P.Q, t[2], wz, dwz);
+#pragma unroll
+  for (int k = 0; k < NU; ++k) iz[k] = wrapi(m[2] - k + NU / 2, g.z);
+  double ph = 0, ex = 0, ey = 0, ez = 0;
+#pragma unroll 1
+  for (int x = 0; x < NU; ++x) {
+    double dwx;
+    double wx = bspline_one<NU, true>(P.Q, x, t[0], &dwx);
+    long xo = (long)wrapi(m[0] - x + NU / 2, g.x) * g.y;
+    double px = 0, py = 0, pz = 0;
+#pragma unroll 1
+    for (int y = 0; y < NU; ++y) {
+      double dwy;
+      double wy = bspline_one<NU, true>(P.Q, y, t[1], &dwy);
+      const double *row = e1 + (xo + wrapi(m[1] - y + NU / 2, g.y)) * g.z;
+      double s0 = 0, s1 = 0;
+#pragma unroll
+      for (int z = 0; z < NU; ++z) {
+        double v = row[iz[z]];
+        s0 += wz[z] * v;
+        s1 += dwz[z] * v;
+      }
+      px += wy * s0;
+      py += dwy * s0;
+      pz += wy * s1;
+    }
+    ph += wx * px;
+    ex += dwx * px;
+    ey += wx * py;
+    ez += wx * pz;
+  }
+  // E = -A^{-T} (M * grad_t), orthorhombic A^{-T} = diag(1/box)
+  double Ex = -ex * g.x * P.ibox[0], Ey = -ey * g.y * P.ibox[1], Ez = -ez * g.z * P.ibox[2];
+  int oi = sidx[i];
+  if (F) {
+    F[3 * (long)oi] += a.w * Ex;
+    F[3 * (long)oi + 1] += a.w * Ey;
+    F[3 * (long)oi + 2] += a.w * Ez;
+  }
+  if (phi) phi[oi] += ph + a.w * P.coeff_self - scal[0] * P.coeff2;
+}
+
+__device__ __forceinline__ void neumaier(double &s, double &c, double v) {
+  double t = s + v;
+  if (fabs(s) >= fabs(v)) c += (s - t) + v;
+  else c += (v - t) + s;
+  s = t;
+}
+
+__global__ __launch_bounds__(64) void k_finalize(DevParams P, const double *__restrict__ ep, int np,
+                                                 const double *__restrict__ eg, int ng,
+                                                 const double *__restrict__ scal,
+                                                 double *__restrict__ out) {
+  if (threadIdx.x != 0) return;
+  double s = 0, c = 0;
+  for (int k = 0; k < np; ++k) neumaier(s, c, ep[k]);
+  double epair = s + c;
+  s = 0, c = 0;
+  for (int k = 0; k < ng; ++k) neumaier(s, c, eg[k]);
+  double egrid = 0.5 * (s + c);
+  double qs = scal[0], q2 = scal[1];
+  // constant part (forcefield1.c:114-130)
+  double econst = 0.5 * q2 * P.coeff1 - 0.5 * (qs * qs) * P.coeff2 - 0.5 * q2 * P.gamma0 * P.inv_a0;
+  s = 0, c = 0;
+  neumaier(s, c, econst);
+  neumaier(s, c, epair);
+  neumaier(s, c, egrid);
+  out[0] = s + c;
+  out[1] = econst;
+  out[2] = epair;
+  out[3] = egrid;
+}
+
+inline unsigned nblk(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+}  // namespace
+
+// ---------------------------------------------------------------- launchers
+void launch_fill_int(int *p, long n, int v, hipStream_t s) {
+  if (n > 0) k_fill_int<<<nblk(n, 256), 256, 0, s>>>(p, n, v);
+}
+void launch_fill_dbl(double *p, long n, double v, hipStream_t s) {
+  if (n > 0) k_fill_dbl<<<nblk(n, 256), 256, 0, s>>>(p, n, v);
+}
+
+int bin_blocks(int N) { return (int)nblk(N, kBinBS); }
+
+void launch_bin(const DevParams &P, const double *pos, const double *q, int *cell_of, int *cnt,
+                double4 *wpos, double *qpart, int *flags, hipStream_t s) {
+  k_bin<<<bin_blocks(P.N), kBinBS, 0, s>>>(P, pos, q, cell_of, cnt, wpos, qpart, flags);
+}
+
+void launch_scan(const DevParams &P, int *cnt, int *start, const double *qpart, int nqpart,
+                 double *scal, hipStream_t s) {
+  k_scan<<<1, kScanBS, 0, s>>>(P.ncell, cnt, start, qpart, nqpart, scal);
+}
+
+void launch_scatter(const DevParams &P, const int *cell_of, const double4 *wpos, const int *start,
+                    int *fill, double4 *spos, int *sidx, int *scell, hipStream_t s) {
+  k_scatter<<<nblk(P.N, 256), 256, 0, s>>>(P.N, cell_of, wpos, start, fill, spos, sidx, scell);
+}
+
+void launch_cell_sort(const DevParams &P, const int *start, double4 *spos, int *sidx,
+                      float4 *lpos, hipStream_t s) {
+  k_cell_sort<<<nblk(P.ncell, 256), 256, 0, s>>>(P, start, spos, sidx, lpos);
+}
+
+int pair_blocks(int N) { return (int)nblk(N, kPairBS); }
+
+void launch_pair_fp64(const DevParams &P, const double4 *spos, const int *sidx, const int *scell,
+                      const int *start, const int4 *sten, int nsten, double *F, double *phi,
+                      double *epart, hipStream_t s) {
+  k_pair64<<<pair_blocks(P.N), kPairBS, 0, s>>>(P, spos, sidx, scell, start, sten, nsten, F, phi,
+                                                epart);
+}
+
+void launch_pair_fp32(const DevParams &P, const float4 *lpos, const int *sidx, const int *scell,
+                      const int *start, const int4 *sten, int nsten, double *F, double *phi,
+                      double *epart, hipStream_t s) {
+  k_pair32<<<pair_blocks(P.N), kPairBS, 0, s>>>(P, lpos, sidx, scell, start, sten, nsten, F, phi,
+                                                epart);
+}
+
+void launch_pair_count(const DevParams &P, const double4 *spos, const int *scell, const int *start,
+                       const int4 *sten, int nsten, unsigned long long *count, hipStream_t s) {
+  k_pair_count<<<pair_blocks(P.N), kPairBS, 0, s>>>(P, spos, scell, start, sten, nsten, count);
+}
+
+#define MSM_NU_SWITCH(nu, CALL) \
+  switch (nu) {                 \
+    case 4: CALL(4); break;     \
+    case 6: CALL(6); break;     \
+    case 8: CALL(8); break;     \
+    default: CALL(10); break;   \
+  }
+
+void launch_spread(const DevParams &P, GridDims g, const double4 *spos, double *q1, hipStream_t s) {
+  launch_fill_dbl(q1, g.size(), 0.0, s);
+#define CALL(NU) k_spread<NU><<<nblk(P.N, 256), 256, 0, s>>>(P, g, spos, q1)
+  MSM_NU_SWITCH(P.nu, CALL)
+#undef CALL
+}
+
+void launch_restrict(const DevParams &P, GridDims fine, GridDims coarse, const double *qf,
+                     double *qc, hipStream_t s) {
+#define CALL(NU) k_restrict<NU><<<nblk(coarse.size(), 256), 256, 0, s>>>(P, fine, coarse, qf, qc)
+  MSM_NU_SWITCH(P.nu, CALL)
+#undef CALL
+}
+
+void launch_top(const DevParams &P, GridDims g, const double *qL, const double *khat,
+                const double *tw, double *work, double *eL, int asis, hipStream_t s) {
+  long n = g.size();
+  if (n <= kTopSmall) {
+    k_top_small<<<1, kTopBS, 0, s>>>(P, g, qL, khat, tw, eL, asis);
+    return;
+  }
+  double *r0 = work, *i0 = work + n, *r1 = work + 2 * n, *i1 = work + 3 * n;
+  unsigned nb = nblk(n, 256);
+  k_top_pack<<<nb, 256, 0, s>>>(P, g, qL, r0, i0, asis);
+  k_dft_axis<<<nb, 256, 0, s>>>(g, 2, -1.0, r0, i0, r1, i1, tw, nullptr);
+  k_dft_axis<<<nb, 256, 0, s>>>(g, 1, -1.0, r1, i1, r0, i0, tw, nullptr);
+  k_dft_axis<<<nb, 256, 0, s>>>(g, 0, -1.0, r0, i0, r1, i1, tw, khat);
+  k_dft_axis<<<nb, 256, 0, s>>>(g, 0, 1.0, r1, i1, r0, i0, tw, nullptr);
+  k_dft_axis<<<nb, 256, 0, s>>>(g, 1, 1.0, r0, i0, r1, i1, tw, nullptr);
+  k_dft_axis<<<nb, 256, 0, s>>>(g, 2, 1.0, r1, i1, r0, i0, tw, nullptr);
+  k_top_out<<<nb, 256, 0, s>>>(n, r0, eL);
+}
+
+void launch_level_fp64(const DevParams &P, GridDims g, GridDims gc, GridDims sd,
+                       const double *khat, const double *ql, const double *ec, double *el,
+                       hipStream_t s) {
+  int LZ = (g.z + kZB - 1) / kZB;
+  if (LZ > 16) LZ = 16;
+  int LY = 64 / LZ;
+  if (LY > g.y) LY = g.y;
+  int tz = LZ * kZB;
+  int ztiles = (g.z + tz - 1) / tz, ytiles = (g.y + LY - 1) / LY;
+  unsigned nb = (unsigned)(ztiles * ytiles * g.x);
+  size_t lds = sizeof(double) * (LY + sd.y - 1) * (tz + sd.z - 1 + kKC);
+#define CALL(NU) \
+  k_level64<NU><<<nb, 64, lds, s>>>(P, g, gc, sd, LZ, LY, khat, ql, ec, el)
+  MSM_NU_SWITCH(P.nu, CALL)
+#undef CALL
+}
+
+int grid_blocks(long n) {
+  long b = (n + kGridBS - 1) / kGridBS;
+  return (int)(b < 1024 ? b : 1024);
+}
+
+void launch_grid_energy(const DevParams &P, GridDims g, const double *q1, const double *e1,
+                        double *epart, int asis, hipStream_t s) {
+  k_grid_energy<<<grid_blocks(g.size()), kGridBS, 0, s>>>(P, g, q1, e1, epart, asis);
+}
+
+void launch_gather(const DevParams &P, GridDims g, const double4 *spos, const int *sidx,
+                   const double *e1, const double *scal, double *F, double *phi, hipStream_t s) {
+#define CALL(NU) k_gather<NU><<<nblk(P.N, 256), 256, 0, s>>>(P, g, spos, sidx, e1, scal, F, phi)
+  MSM_NU_SWITCH(P.nu, CALL)
+#undef CALL
+}
+
+void launch_finalize(const DevParams &P, const double *epair, int npair, const double *egrid,
+                     int ngrid, const double *scal, double *out, hipStream_t s) {
+  k_finalize<<<1, 64, 0, s>>>(P, epair, npair, egrid, ngrid, scal, out);
+}
+
+}  // namespace msm

@@ -1,0 +1,60 @@
+// msm_plan.h -- build-time state of one MSM evaluation plan (host, FP64).
+//
+// Holds what hkayabilisim/msmpb computes in FF_build / FF_rebuild
+// (code/forcefield0.c:271-680): the resolved parameters, the B-spline,
+// softening and two-scale tables, the anti-blurring weights and the level
+// stencils.  Grids are described in periodic (unpadded) form: level l has
+// dims M[l][0..2], x-major with z fastest.
+#pragma once
+
+#include <string>
+#include <vector>
+
+namespace msm {
+
+constexpr int kMaxLevel = 9;   // the reference's time_* arrays hold 10 levels
+
+struct Plan {
+  int N = 0;
+  double A[3][3] = {}, Ai[3][3] = {}, detA = 0;
+  bool ortho = true;           // box matrix diagonal
+  int nu = 0, L = 0;
+  int M[kMaxLevel + 1][3] = {};  // level grid dims, l = 1..L
+  int S[kMaxLevel + 1][3] = {};  // stencil dims, l = 1..L-1 (s_l = min(dmax, M_l))
+  double aCut[kMaxLevel + 1] = {};
+  double relCutoff = 0;
+  int nLim = 0;
+  double estErr = 1e-3;
+  double tau[10] = {};         // softening Taylor coefficients tau_0..tau_{nu-1}
+  double Q[50] = {};           // B-spline pieces: (nu/2) x nu, piece j in powers of (t - j)
+  double J[11] = {};           // two-scale stencil J[n + nu/2], n = -nu/2..nu/2
+  double gamma0 = 0;           // gamma(0) = tau(-1)
+  double coeff1 = 0, coeff2 = 0;
+  std::vector<double> omegap[kMaxLevel + 1][3];
+  std::vector<double> cL[3];
+  std::vector<double> khat[kMaxLevel + 1];  // l < L: S^3 real-space; l = L: M_L^3 Fourier
+};
+
+struct BuildRequest {
+  int N = 0;
+  const double *q = nullptr;   // for the a-priori error model (sum q^2)
+  double edges[3][3] = {};
+  int nu = -1, L = -1, Mtop[3] = {-1, -1, -1};
+  double a0 = -1, estErr = -1;
+};
+
+// Resolves parameters with the reference's rules and builds every table and
+// stencil.  Returns false with a message for inputs outside the domain the
+// reference computes correctly.
+bool build_plan(const BuildRequest &req, Plan &plan, std::string &err);
+
+// Recomputes the box-dependent parts (constants and stencils) for a new box
+// with the parameters of an existing plan (FF_rebuild).
+bool rebuild_plan(Plan &plan, const double edges[3][3], std::string &err);
+
+// A-priori relative force error estimates (forcefield0.c:521-587).
+double err_estimate(const Plan &plan, const double *q, int nu);
+double delta_f_estimate(const Plan &plan, const double *q);
+double invert3(const double a[3][3], double ai[3][3]);
+
+}  // namespace msm

@@ -22,7 +22,7 @@ _ip = ctypes.POINTER(ctypes.c_int)
 def build(force=False):
     """Compile liborc.so with make (gcc, plain C11)."""
     if force or not os.path.exists(_LIB_PATH):
-        subprocess.check_call(["make", "-s", "-B" if force else "-s", "-C", _HERE])
+        subprocess.check_call(["make", "-s"] + (["-B"] if force else []) + ["-C", _HERE])
     return _LIB_PATH
 
 
@@ -99,8 +99,11 @@ class Oracle:
 
     def __del__(self):
         h = getattr(self, "_h", None)
-        if h:
-            lib().orc_free(h)
+        if h and _lib is not None:
+            try:
+                _lib.orc_free(h)
+            except Exception:  # interpreter shutdown
+                pass
             self._h = None
 
     def energy(self, pos, q=None, want_phi=True):

@@ -1,0 +1,81 @@
+"""CPU checks of the drop-in boundary: the library loads, exports every
+symbol the headers declare, and the FF struct layout of include/forcefield.h
+is what the ctypes binding and C callers see."""
+import ctypes
+import os
+import re
+import subprocess
+
+import pytest
+
+from helpers import ROOT
+
+HEADERS = [os.path.join(ROOT, "include", h) for h in ("forcefield.h", "msmpb_hip.h")]
+
+
+def declared_functions():
+    names = set()
+    for h in HEADERS:
+        text = open(h).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        text = re.sub(r"#define[^\n]*(\\\n[^\n]*)*", "", text)
+        for m in re.finditer(r"([A-Za-z_][A-Za-z0-9_]*)\s*\(", text):
+            name = m.group(1)
+            prev = text[:m.start()].rstrip()
+            if name in ("if", "sizeof", "__typeof__", "double", "int", "void", "char", "bool") \
+                    or prev.endswith("("):
+                continue
+            if re.search(r"(typedef|struct|enum)\s*$", prev):
+                continue
+            names.add(name)
+    return sorted(names)
+
+
+def test_library_exports_every_declared_symbol():
+    from msmpb_amd import _lib
+    L = _lib.lib()
+    names = declared_functions()
+    assert len(names) >= 40
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+    assert set(names) <= set(_lib.EXPORTS)
+
+
+def test_ff_struct_layout_matches_c(tmp_path):
+    from msmpb_amd._lib import FFStruct
+    fields = [f[0] for f in FFStruct._fields_]
+    src = tmp_path / "off.c"
+    body = "\n".join(f'  printf("{f} %zu\\n", offsetof(FF, {f}));' for f in fields)
+    src.write_text('#include <stddef.h>\n#include <stdio.h>\n#include "forcefield.h"\n'
+                   f"int main(void) {{\n{body}\n  printf(\"sizeof %zu\\n\", sizeof(FF));\n}}\n")
+    exe = tmp_path / "off"
+    subprocess.check_call(["gcc", "-std=gnu11", "-I", os.path.join(ROOT, "include"), str(src),
+                           "-o", str(exe)])
+    out = dict(l.split() for l in subprocess.check_output([str(exe)], text=True).splitlines())
+    for f in fields:
+        assert int(out[f]) == getattr(FFStruct, f).offset, f
+    assert int(out["sizeof"]) == ctypes.sizeof(FFStruct)
+
+
+def test_reference_header_layout_offsets():
+    """Offsets fixed by code/forcefield.h:22-69 on LP64 (pointers 8 bytes,
+    the three FFTW members pointer-sized)."""
+    from msmpb_amd._lib import FFStruct
+    assert FFStruct.q.offset == 8
+    assert FFStruct.A.offset == 16 and FFStruct.Ai.offset == 88
+    assert FFStruct.detA.offset == 160
+    assert FFStruct.orderAcc.offset == 208 and FFStruct.topGridDim.offset == 216
+    assert FFStruct.nlist.offset == 232 and FFStruct.nlist_interaction_count.offset == 244
+    assert FFStruct.FFT.offset == 248 and FFStruct.fftw_in.offset == 256
+    assert FFStruct.tau.offset == 280 and FFStruct.nLim.offset == 328
+    assert FFStruct.cutoff.offset == 344 and FFStruct.coeff1.offset == 384
+    assert FFStruct.time_partcl2partcl.offset == 400 and FFStruct.time_padding.offset == 752
+    assert ctypes.sizeof(FFStruct) == 832
+
+
+def test_cli_built_and_links():
+    cli = os.path.join(ROOT, "msmpb_amd", "bin", "msmpb")
+    if not os.path.exists(cli):
+        pytest.skip("CLI not built")
+    r = subprocess.run([cli], capture_output=True, text=True)
+    assert r.returncode == 1 and "Usage: msmpb" in r.stderr

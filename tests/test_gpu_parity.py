@@ -1,0 +1,138 @@
+"""GPU parity: the HIP path (through the C ABI) against the CPU oracle.
+
+Plain FP64 must agree with the oracle to 1e-10 relative in energy and in
+forces (max |dF| / rms F), stage by stage on the level grids, for every
+layout and for the device-pointer and autograd entry points.
+"""
+import numpy as np
+import pytest
+
+from helpers import force_err, load_dataset, rel_err, rms_rel_force
+from msmpb_amd import inputs
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-10
+
+
+@pytest.fixture(scope="module")
+def torch_cuda():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return torch
+
+
+def _pair(sysm, oracle_mod, precision="fp64", layout="fixed", **over):
+    from msmpb_amd import MSM
+    p = dict(sysm.params)
+    p.update(over)
+    m = MSM(sysm.pos, sysm.q, sysm.edges, nu=p["nu"], levels=p["levels"], top_grid=p["top_grid"],
+            cutoff=p["cutoff"], precision=precision, layout=layout)
+    o = oracle_mod.Oracle(sysm.pos, sysm.q, sysm.edges, nu=p["nu"], L=p["levels"],
+                          Mtop=p["top_grid"], a0=p["cutoff"], compat=1 if layout == "asis" else 0)
+    return m, o
+
+
+def _dataset_system(name, nu, L, M, a0):
+    edges, q, pos, mass = load_dataset(name)
+    return inputs.System(name, pos, q, edges, mass, dict(nu=nu, levels=L, top_grid=M, cutoff=a0))
+
+
+CASES = {
+    "water300": inputs.cfg0_water300,
+    "crystal8192": inputs.cfg1_crystal8192,
+    "random3000": inputs.cfg2_random3000,
+    "spce300_L1_nu4": lambda: _dataset_system("spceN300", 4, 1, 24, 9.0),
+    "spce300_L2_nu6": lambda: _dataset_system("spceN300", 6, 2, 6, 9.0),
+    "spce300_L1_nu8_a11": lambda: _dataset_system("spceN300", 8, 1, 30, 11.0),
+    "nacl512_L3_nu6": lambda: _dataset_system("NaClN512", 6, 3, 4, 3.0),
+}
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_fp64_energy_forces_potentials(case, oracle_mod, torch_cuda):
+    print(f"[parity] fp64 {case}", flush=True)
+    s = CASES[case]()
+    m, o = _pair(s, oracle_mod)
+    E, F, phi = m.energy(s.pos, potential=True)
+    Eo, Fo, phio, to = o.energy(s.pos)
+    assert rel_err(E, Eo) < TOL, (E, Eo)
+    assert force_err(F, Fo) < TOL
+    assert np.max(np.abs(phi - phio)) / np.sqrt(np.mean(phio ** 2)) < TOL
+    t = m.terms()
+    for k in range(3):
+        assert abs(t[k] - to[k]) <= TOL * max(abs(Eo), abs(to[k]))
+    # the reference's neighbour statistic: pairs with a nearest image inside a0
+    assert m.pair_count == o.pair_count()
+
+
+@pytest.mark.parametrize("case", ["water300", "crystal8192", "random3000", "spce300_L2_nu6"])
+def test_fp64_stage_grids(case, oracle_mod, torch_cuda):
+    print(f"[parity] stages {case}", flush=True)
+    s = CASES[case]()
+    m, o = _pair(s, oracle_mod)
+    m.energy(s.pos)
+    o.energy(s.pos)
+    for l in range(1, m.levels + 1):
+        for which in (0, 1):
+            g, go = m.grid(l, which), o.grid(l, which)
+            assert g.shape == go.shape
+            scale = np.max(np.abs(go))
+            assert np.max(np.abs(g - go)) <= TOL * scale, (l, which)
+        st, sto = m.stencil(l), o.stencil(l)
+        assert np.max(np.abs(st - sto)) <= 1e-12 * np.max(np.abs(sto)), l
+
+
+@pytest.mark.parametrize("case", ["spce300_L1_nu4", "spce300_L2_nu6", "water300", "nacl512_L3_nu6"])
+def test_asis_layout(case, oracle_mod, torch_cuda):
+    print(f"[parity] asis {case}", flush=True)
+    s = CASES[case]()
+    m, o = _pair(s, oracle_mod, layout="asis")
+    E, F = m.energy(s.pos)
+    Eo, Fo, _, _ = o.energy(s.pos)
+    assert rel_err(E, Eo) < TOL
+    assert force_err(F, Fo) < TOL
+
+
+def test_device_path_matches_host(oracle_mod, torch_cuda):
+    torch = torch_cuda
+    from msmpb_amd.inputs import cfg2_random3000
+    s = cfg2_random3000()
+    m, o = _pair(s, oracle_mod)
+    E, F, phi = m.energy(s.pos, potential=True)
+    pos = torch.tensor(s.pos, device="cuda")
+    q = torch.tensor(s.q, device="cuda")
+    e, Fd, phid = m.eval_device(pos, q, forces=True, potential=True)
+    torch.cuda.synchronize()
+    assert abs(e[0].item() - E) <= 1e-13 * abs(E)
+    assert np.max(np.abs(Fd.cpu().numpy() - F)) <= 1e-13 * np.max(np.abs(F))
+    assert np.max(np.abs(phid.cpu().numpy() - phi)) <= 1e-13 * np.max(np.abs(phi))
+
+
+@pytest.mark.parametrize("recompute", [False, True])
+def test_autograd_adjoint(recompute, oracle_mod, torch_cuda):
+    torch = torch_cuda
+    from msmpb_amd import msm_energy
+    from msmpb_amd.inputs import cfg2_random3000
+    s = cfg2_random3000()
+    m, o = _pair(s, oracle_mod)
+    Eo, Fo, phio, _ = o.energy(s.pos)
+    pos = torch.tensor(s.pos, device="cuda", requires_grad=True)
+    q = torch.tensor(s.q, device="cuda", requires_grad=True)
+    E = msm_energy(m, pos, q, recompute=recompute)
+    E.backward()
+    assert rel_err(E.item(), Eo) < TOL
+    assert force_err(-pos.grad.cpu().numpy(), Fo) < TOL
+    assert np.max(np.abs(q.grad.cpu().numpy() - phio)) / np.sqrt(np.mean(phio ** 2)) < TOL
+
+
+def test_mixed_precision_accuracy(oracle_mod, torch_cuda):
+    for case in ("water300", "crystal8192", "random3000"):
+        print(f"[parity] mixed {case}", flush=True)
+        s = CASES[case]()
+        m, o = _pair(s, oracle_mod, precision="mixed")
+        E, F = m.energy(s.pos)
+        Eo, Fo, _, _ = o.energy(s.pos)
+        assert rms_rel_force(F, Fo) < 1e-5, case
+        assert rel_err(E, Eo) < 1e-5, case
