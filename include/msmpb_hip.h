@@ -79,6 +79,15 @@ int msm_forward_device(FF *ff, const double *d_pos, const double *d_q, double *d
 int msm_adjoint_device(FF *ff, const double *d_pos, const double *d_q, double *d_force,
                        double *d_phi, int recompute, void *stream);
 
+/* Per-stage GPU time of the device-path evaluations run since the last call
+ * while stage timing is on (FF_set_stage_timing): out[k] (k < n) receives
+ * the summed seconds of stage k, named by msm_stage_name(k) (8 stages:
+ * binning and sort, pairs, spreading, restriction, top level, level
+ * stencils with prolongation, energy reduction, interpolation).  Returns the
+ * number of evaluations covered (or a negative error code) and resets. */
+int msm_stage_times(FF *ff, double *out, int n);
+const char *msm_stage_name(int k);
+
 /* Diagnostics for the stage-by-stage parity tests: copy the last
  * evaluation's periodic level-l grid (which = 0: charges q^l, 1:
  * potentials e^l; x-major, z fastest) or the level-l stencil to host

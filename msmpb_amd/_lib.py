@@ -80,7 +80,7 @@ EXPORTS = [
     "FF_last_error", "FF_set_precision", "FF_get_precision", "FF_set_layout_compat",
     "FF_set_stage_timing", "FF_get_energy_terms", "FF_energy_potential", "msm_eval_device",
     "msm_forward_device", "msm_adjoint_device", "msm_grid_dims", "msm_stencil_dims",
-    "msm_copy_grid", "msm_copy_stencil", "msm_count_pairs",
+    "msm_copy_grid", "msm_copy_stencil", "msm_count_pairs", "msm_stage_times", "msm_stage_name",
 ]
 
 _lib = None
@@ -145,6 +145,10 @@ def lib():
     L.msm_copy_stencil.argtypes = [FFP, ctypes.c_int, _dp]
     L.msm_count_pairs.restype = ctypes.c_int
     L.msm_count_pairs.argtypes = [FFP, _dp, ctypes.POINTER(ctypes.c_longlong)]
+    L.msm_stage_times.restype = ctypes.c_int
+    L.msm_stage_times.argtypes = [FFP, _dp, ctypes.c_int]
+    L.msm_stage_name.restype = ctypes.c_char_p
+    L.msm_stage_name.argtypes = [ctypes.c_int]
     L.msm4g_tictocmanager.restype = ctypes.c_double
     L.msm4g_tictocmanager.argtypes = [ctypes.c_int]
     L.msm4g_toc.restype = ctypes.c_double

@@ -85,6 +85,9 @@ struct Engine {
   void *d_ksplit[msm::kMaxLevel + 1] = {};
   void *d_qsplit = nullptr;
   double *d_tw = nullptr, *d_topwork = nullptr;
+  // per-stage events of the evaluations since the last collection
+  Timing T;
+  int timed_evals = 0;
   // forward state kept for a following adjoint
   bool kept = false;
   const double *kept_pos = nullptr, *kept_q = nullptr;
@@ -92,6 +95,7 @@ struct Engine {
 
   ~Engine() { release(); }
   void release() {
+    T.clear();
     void *ptrs[] = {d_pos,  d_q,    d_F,    d_phi,   d_out,  d_qpart, d_scal,  d_epair,
                     d_egrid, d_cell_of, d_cnt, d_start, d_sidx, d_scell, d_flags, d_wpos,
                     d_spos, d_lpos, d_sten, d_count, d_qsplit, d_tw, d_topwork};
@@ -349,10 +353,12 @@ int eval_chain(FFX *x, const double *pos, const double *q, double *out, double *
   T.mark("partcl2partcl", s);
   stage_grids(E, x->precision, x->layout, T, s);
   stage_energy(E, x->layout, s);
+  T.mark("energy", s);
   if (F || phi) msm::launch_gather(E.P, E.gd[1], E.d_spos, E.d_sidx, E.d_el[1], E.d_scal, F, phi, s);
   T.mark("interpolation", s);
   if (out != E.d_out) HIP_TRY(hipMemcpyAsync(out, E.d_out, 4 * sizeof(double), hipMemcpyDeviceToDevice, s));
   HIP_TRY(hipGetLastError());
+  if (T.on) ++E.timed_evals;
   E.kept = false;
   return MSM_OK;
 }
@@ -380,6 +386,11 @@ void publish_timing(FFX *x, Timing &T) {
     else if (t == "interpolation") ff->time_interpolation = sec;
   }
 }
+
+// Stage names reported by msm_stage_times, in order.
+const char *kStageNames[] = {"nlist", "partcl2partcl", "anterpolation", "restriction",
+                             "grid2grid_top", "grid2grid", "energy", "interpolation"};
+constexpr int kNumStages = 8;
 
 void publish_plan(FFX *x) {
   FF *ff = &x->ff;
@@ -500,7 +511,9 @@ int energy_host(FF *ff, double (*force)[3], double (*position)[3], double *phi_o
   FFX *x = X(ff);
   Engine &E = *x->eng;
   if (!force || !position) return fail(MSM_ERR_ARG, "FF_energy: force and position must be non-NULL");
-  Timing T;
+  Timing &T = E.T;
+  T.clear();
+  E.timed_evals = 0;
   T.on = x->timing != 0;
   TRY(stage_host_inputs(x, position));
   int rc = eval_chain(x, E.d_pos, E.d_q, E.d_out, E.d_F, phi_out ? E.d_phi : nullptr, E.stream, T);
@@ -518,6 +531,7 @@ int energy_host(FF *ff, double (*force)[3], double (*position)[3], double *phi_o
   HIP_TRY(hipStreamSynchronize(E.stream));
   publish_timing(x, T);
   T.clear();
+  E.timed_evals = 0;
   if (flags) return fail(MSM_ERR_ARG, "FF_energy: non-finite position or charge");
   std::memcpy(E.last_terms, out + 1, 3 * sizeof(double));
   *E_out = out[0];
@@ -787,10 +801,32 @@ int msm_eval_device(FF *ff, const double *d_pos, const double *d_q, double *d_en
   if (!d_pos || !d_q || !d_energy) return fail(MSM_ERR_ARG, "msm_eval_device: NULL input");
   FFX *x = X(ff);
   hipStream_t s = stream ? (hipStream_t)stream : x->eng->stream;
-  Timing T;
+  Timing &T = x->eng->T;
+  T.on = x->timing != 0;
   TRY(eval_chain(x, d_pos, d_q, d_energy, d_force, d_phi, s, T));
   return MSM_OK;
 }
+
+int msm_stage_times(FF *ff, double *out, int n) {
+  TRY(check_built(ff));
+  Engine &E = *X(ff)->eng;
+  Timing &T = E.T;
+  for (int k = 0; k < n; ++k) out[k] = 0.0;
+  if (!T.ev.empty()) HIP_TRY(hipEventSynchronize(T.ev.back()));
+  for (size_t k = 1; k < T.ev.size(); ++k) {
+    if (T.tag[k] == "begin") continue;
+    float ms = 0;
+    HIP_TRY(hipEventElapsedTime(&ms, T.ev[k - 1], T.ev[k]));
+    for (int j = 0; j < kNumStages && j < n; ++j)
+      if (T.tag[k] == kStageNames[j]) out[j] += ms * 1e-3;
+  }
+  int evals = E.timed_evals;
+  T.clear();
+  E.timed_evals = 0;
+  return evals;
+}
+
+const char *msm_stage_name(int k) { return (k >= 0 && k < kNumStages) ? kStageNames[k] : ""; }
 
 int msm_forward_device(FF *ff, const double *d_pos, const double *d_q, double *d_energy, int keep,
                        void *stream) {

@@ -7,7 +7,7 @@ layout and for the device-pointer and autograd entry points.
 import numpy as np
 import pytest
 
-from helpers import force_err, load_dataset, rel_err, rms_rel_force
+from helpers import force_err, force_scale, load_dataset, rel_err, rms_rel_force
 from msmpb_amd import inputs
 
 pytestmark = pytest.mark.gpu
@@ -58,7 +58,7 @@ def test_fp64_energy_forces_potentials(case, oracle_mod, torch_cuda):
     E, F, phi = m.energy(s.pos, potential=True)
     Eo, Fo, phio, to = o.energy(s.pos)
     assert rel_err(E, Eo) < TOL, (E, Eo)
-    assert force_err(F, Fo) < TOL
+    assert force_err(F, Fo, force_scale(s.q, s.edges)) < TOL
     assert np.max(np.abs(phi - phio)) / np.sqrt(np.mean(phio ** 2)) < TOL
     t = m.terms()
     for k in range(3):
@@ -80,8 +80,9 @@ def test_fp64_stage_grids(case, oracle_mod, torch_cuda):
             assert g.shape == go.shape
             scale = np.max(np.abs(go))
             assert np.max(np.abs(g - go)) <= TOL * scale, (l, which)
+        # stencils are built on the host by independent FP64 code
         st, sto = m.stencil(l), o.stencil(l)
-        assert np.max(np.abs(st - sto)) <= 1e-12 * np.max(np.abs(sto)), l
+        assert np.max(np.abs(st - sto)) <= 1e-11 * np.max(np.abs(sto)), l
 
 
 @pytest.mark.parametrize("case", ["spce300_L1_nu4", "spce300_L2_nu6", "water300", "nacl512_L3_nu6"])
@@ -92,7 +93,7 @@ def test_asis_layout(case, oracle_mod, torch_cuda):
     E, F = m.energy(s.pos)
     Eo, Fo, _, _ = o.energy(s.pos)
     assert rel_err(E, Eo) < TOL
-    assert force_err(F, Fo) < TOL
+    assert force_err(F, Fo, force_scale(s.q, s.edges)) < TOL
 
 
 def test_device_path_matches_host(oracle_mod, torch_cuda):
