@@ -80,16 +80,17 @@ def run_gpu(args, sysm):
         for _ in range(args.warmup):
             step(m)
         torch.cuda.synchronize()
-        L.msm_stage_times(m._ff, (ctypes.c_double * 8)(), 8)  # drop warm-up events
+        L.msm_stage_times(m._ff, (ctypes.c_double * 16)(), 16)  # drop warm-up events
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for _ in range(args.steps):
             step(m)
         torch.cuda.synchronize()
         t1 = time.perf_counter()
-        st = (ctypes.c_double * 8)()
-        n = L.msm_stage_times(m._ff, st, 8)
-        stages = {L.msm_stage_name(k).decode(): st[k] / max(n, 1) for k in range(8)}
+        st = (ctypes.c_double * 16)()
+        n = L.msm_stage_times(m._ff, st, 16)
+        stages = {L.msm_stage_name(k).decode(): st[k] / max(n, 1) for k in range(16)
+                  if L.msm_stage_name(k)}
         step(m)
         torch.cuda.synchronize()
         result[tag] = dict(ms=(t1 - t0) * 1e3 / args.steps, stages=stages, E=out[0].item(),

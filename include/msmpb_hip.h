@@ -81,9 +81,9 @@ int msm_adjoint_device(FF *ff, const double *d_pos, const double *d_q, double *d
 
 /* Per-stage GPU time of the device-path evaluations run since the last call
  * while stage timing is on (FF_set_stage_timing): out[k] (k < n) receives
- * the summed seconds of stage k, named by msm_stage_name(k) (8 stages:
- * binning and sort, pairs, spreading, restriction, top level, level
- * stencils with prolongation, energy reduction, interpolation).  Returns the
+ * the summed seconds of stage k, named by msm_stage_name(k) (9 stages:
+ * binning and sort, pairs, spreading, restriction, level stencils, top
+ * level, prolongation, energy reduction, interpolation).  Returns the
  * number of evaluations covered (or a negative error code) and resets. */
 int msm_stage_times(FF *ff, double *out, int n);
 const char *msm_stage_name(int k);

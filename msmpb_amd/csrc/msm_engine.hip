@@ -44,22 +44,36 @@ int fail(int code, const std::string &msg) {
       return fail(MSM_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_));     \
   } while (0)
 
+// Named GPU intervals of one or more evaluations (start/stop events on the
+// stream that runs the stage, so concurrent stages are timed separately).
 struct Timing {
   bool on = false;
-  std::vector<hipEvent_t> ev;
-  std::vector<std::string> tag;
-  void mark(const char *t, hipStream_t s) {
-    if (!on) return;
+  struct Iv {
+    std::string tag;
+    hipEvent_t a, b;
+  };
+  std::vector<Iv> iv;
+  std::vector<hipEvent_t> open_ev;
+  int open(hipStream_t s) {
+    if (!on) return -1;
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return -1;
+    hipEventRecord(e, s);
+    open_ev.push_back(e);
+    return (int)open_ev.size() - 1;
+  }
+  void close(int k, const char *tag, hipStream_t s) {
+    if (!on || k < 0) return;
     hipEvent_t e;
     if (hipEventCreate(&e) != hipSuccess) return;
     hipEventRecord(e, s);
-    ev.push_back(e);
-    tag.push_back(t);
+    iv.push_back({tag, open_ev[k], e});
   }
   void clear() {
-    for (auto e : ev) hipEventDestroy(e);
-    ev.clear();
-    tag.clear();
+    for (auto &v : iv) hipEventDestroy(v.b);
+    for (auto e : open_ev) hipEventDestroy(e);
+    iv.clear();
+    open_ev.clear();
   }
 };
 
@@ -69,6 +83,8 @@ struct Engine {
   msm::GridDims gd[msm::kMaxLevel + 1]{}, sd[msm::kMaxLevel + 1]{};
   int device = 0;
   hipStream_t stream = nullptr;
+  hipStream_t stream2 = nullptr;   // short-range pairs run here, beside the grid chain
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   int nsten = 0;
   // device buffers
   double *d_pos = nullptr, *d_q = nullptr, *d_F = nullptr, *d_phi = nullptr;
@@ -85,6 +101,11 @@ struct Engine {
   void *d_ksplit[msm::kMaxLevel + 1] = {};
   void *d_qsplit = nullptr;
   double *d_tw = nullptr, *d_topwork = nullptr;
+  // level stencils on MFMA, per precision (0: FP64, 1: BF16 hi/lo)
+  msm::G2GArgs g2g[2]{};
+  size_t g2g_lds[2] = {0, 0};
+  double *d_part[2][msm::kMaxLevel + 1] = {};
+  void *d_bfrag[2][msm::kMaxLevel + 1] = {};
   // per-stage events of the evaluations since the last collection
   Timing T;
   int timed_evals = 0;
@@ -101,6 +122,13 @@ struct Engine {
                     d_spos, d_lpos, d_sten, d_count, d_qsplit, d_tw, d_topwork};
     for (void *p : ptrs)
       if (p) hipFree(p);
+    for (int pr = 0; pr < 2; ++pr)
+      for (int l = 0; l <= msm::kMaxLevel; ++l) {
+        if (d_part[pr][l]) hipFree(d_part[pr][l]);
+        if (d_bfrag[pr][l]) hipFree(d_bfrag[pr][l]);
+        d_part[pr][l] = nullptr;
+        d_bfrag[pr][l] = nullptr;
+      }
     for (int l = 0; l <= msm::kMaxLevel; ++l) {
       if (d_ql[l]) hipFree(d_ql[l]);
       if (d_el[l]) hipFree(d_el[l]);
@@ -117,8 +145,12 @@ struct Engine {
     d_count = nullptr;
     d_qsplit = nullptr;
     d_tw = d_topwork = nullptr;
+    if (ev_fork) hipEventDestroy(ev_fork);
+    if (ev_join) hipEventDestroy(ev_join);
+    ev_fork = ev_join = nullptr;
+    if (stream2) hipStreamDestroy(stream2);
     if (stream) hipStreamDestroy(stream);
-    stream = nullptr;
+    stream = stream2 = nullptr;
   }
 };
 
@@ -212,9 +244,13 @@ void fill_params(const msm::Plan &p, int N, msm::DevParams &P) {
 // Uploads the box-dependent device data (stencils, twiddles, parameters).
 int upload_box(Engine &E) {
   msm::Plan &p = E.plan;
+  int ncell_alloc = E.P.ncell;
   fill_params(p, p.N, E.P);
   std::vector<int4> sten;
   pair_geometry(p, E.P, sten);
+  if (ncell_alloc > 0 && E.P.ncell > ncell_alloc)
+    return fail(MSM_ERR_ARG, "box change enlarges the pair-search cell grid beyond its buffers; "
+                             "rebuild the FF with FF_build");
   E.nsten = (int)sten.size();
   if (E.d_sten) hipFree(E.d_sten), E.d_sten = nullptr;
   TRY(dalloc(&E.d_sten, sten.size()));
@@ -236,6 +272,51 @@ int upload_box(Engine &E) {
   if (E.d_tw) hipFree(E.d_tw), E.d_tw = nullptr;
   TRY(dalloc(&E.d_tw, tw.size()));
   HIP_TRY(hipMemcpy(E.d_tw, tw.data(), tw.size() * sizeof(double), hipMemcpyHostToDevice));
+  // Toeplitz B operands of the level stencils in MFMA lane order
+  for (int pr = 0; pr < 2; ++pr)
+    for (int l = 1; l < p.L; ++l) {
+      msm::G2GLevel &lv = E.g2g[pr].lv[l - 1];
+      size_t bytes = msm::g2g_bfrag_bytes(pr, lv);
+      std::vector<unsigned char> host(bytes);
+      msm::g2g_bfrag_fill(pr, lv, p.khat[l].data(), host.data());
+      if (E.d_bfrag[pr][l]) hipFree(E.d_bfrag[pr][l]), E.d_bfrag[pr][l] = nullptr;
+      TRY(dalloc((unsigned char **)&E.d_bfrag[pr][l], bytes));
+      HIP_TRY(hipMemcpy(E.d_bfrag[pr][l], host.data(), bytes, hipMemcpyHostToDevice));
+      lv.bfrag = E.d_bfrag[pr][l];
+    }
+  return MSM_OK;
+}
+
+// Tiles, kx groups and partial buffers of the MFMA level-stencil launch.
+int plan_levels(Engine &E) {
+  const msm::Plan &p = E.plan;
+  static const int target[2] = {4096, 2048};
+  for (int pr = 0; pr < 2; ++pr) {
+    msm::G2GArgs &A = E.g2g[pr];
+    A.nlev = 0;
+    A.ntasks = 0;
+    E.g2g_lds[pr] = 0;
+    for (int l = 1; l < p.L; ++l) {
+      msm::G2GLevel &lv = A.lv[l - 1];
+      msm::plan_g2g_level(pr, E.gd[l], E.sd[l], target[pr], lv);
+      lv.task0 = A.ntasks;
+      A.ntasks += (long)lv.g.x * lv.ty * lv.tz * lv.G;
+      lv.q = E.d_ql[l];
+      TRY(dalloc(&E.d_part[pr][l], (size_t)lv.G * E.gd[l].size()));
+      lv.part = E.d_part[pr][l];
+      A.nlev = l;
+    }
+    // one launch covers every level, so they share the K-chunk count (extra
+    // chunks of a smaller stencil hold zero Toeplitz columns)
+    int nch = 0;
+    for (int k = 0; k < A.nlev; ++k) nch = std::max(nch, A.lv[k].nch);
+    if (nch > msm::g2g_max_nch(pr))
+      return fail(MSM_ERR_ARG, "FF_build: level stencil too wide for the MFMA tiles");
+    for (int k = 0; k < A.nlev; ++k) {
+      A.lv[k].nch = nch;
+      E.g2g_lds[pr] = std::max(E.g2g_lds[pr], msm::g2g_lds_bytes(pr, A.lv[k]));
+    }
+  }
   return MSM_OK;
 }
 
@@ -245,13 +326,23 @@ int create_engine(FFX *x, const msm::Plan &plan) {
   x->eng = E;
   HIP_TRY(hipGetDevice(&E->device));
   HIP_TRY(hipStreamCreateWithFlags(&E->stream, hipStreamNonBlocking));
+  HIP_TRY(hipStreamCreateWithFlags(&E->stream2, hipStreamNonBlocking));
+  HIP_TRY(hipEventCreateWithFlags(&E->ev_fork, hipEventDisableTiming));
+  HIP_TRY(hipEventCreateWithFlags(&E->ev_join, hipEventDisableTiming));
   const msm::Plan &p = E->plan;
   for (int l = 1; l <= p.L; ++l) {
     E->gd[l] = {p.M[l][0], p.M[l][1], p.M[l][2]};
     E->sd[l] = {p.S[l][0], p.S[l][1], p.S[l][2]};
   }
   E->sd[p.L] = E->gd[p.L];
-  TRY(upload_box(*E));
+  // the pair-search cell grid sizes the binning buffers below, so it must be
+  // known before anything is allocated (upload_box recomputes the same values)
+  {
+    std::vector<int4> sten;
+    fill_params(p, p.N, E->P);
+    pair_geometry(p, E->P, sten);
+  }
+  if (E->P.ncell <= 0) return fail(MSM_ERR_ARG, "FF_build: empty pair-search cell grid");
   long N = p.N;
   int nb = msm::bin_blocks((int)N), np = msm::pair_blocks((int)N);
   TRY(dalloc(&E->d_pos, 3 * N));
@@ -261,7 +352,7 @@ int create_engine(FFX *x, const msm::Plan &plan) {
   TRY(dalloc(&E->d_out, 8));
   TRY(dalloc(&E->d_qpart, 2 * nb));
   TRY(dalloc(&E->d_scal, 4));
-  TRY(dalloc(&E->d_epair, np));
+  TRY(dalloc(&E->d_epair, std::max(np, E->P.ncell)));
   TRY(dalloc(&E->d_egrid, msm::grid_blocks(E->gd[1].size())));
   TRY(dalloc(&E->d_cell_of, N));
   TRY(dalloc(&E->d_cnt, E->P.ncell));
@@ -278,6 +369,8 @@ int create_engine(FFX *x, const msm::Plan &plan) {
     TRY(dalloc(&E->d_el[l], E->gd[l].size()));
   }
   TRY(dalloc(&E->d_topwork, 4 * E->gd[p.L].size()));
+  TRY(plan_levels(*E));
+  TRY(upload_box(*E));
   return MSM_OK;
 }
 
@@ -308,27 +401,37 @@ void stage_pairs(Engine &E, int precision, double *F, double *phi, hipStream_t s
 void stage_grids(Engine &E, int precision, int layout, Timing &T, hipStream_t s) {
   const msm::Plan &p = E.plan;
   int L = p.L;
-  msm::launch_spread(E.P, E.gd[1], E.d_spos, E.d_ql[1], s);
-  T.mark("anterpolation", s);
+  int pr = precision == MSM_PREC_MIXED ? msm::MSM_G2G_BF16X3 : msm::MSM_G2G_FP64;
+  int t = T.open(s);
+  msm::launch_spread(E.P, E.gd[1], E.d_spos, E.d_start, E.d_ql[1], s);
+  T.close(t, "anterpolation", s);
   for (int l = 2; l <= L; ++l) {
+    t = T.open(s);
     msm::launch_restrict(E.P, E.gd[l - 1], E.gd[l], E.d_ql[l - 1], E.d_ql[l], s);
-    T.mark("restriction", s);
+    T.close(t, "restriction", s);
   }
+  if (L >= 2) {
+    // stencil correlations of every level l < L in one launch
+    t = T.open(s);
+    msm::launch_g2g_tiles(pr, E.g2g[pr], E.g2g_lds[pr], s);
+    T.close(t, "grid2grid", s);
+  }
+  t = T.open(s);
   msm::launch_top(E.P, E.gd[L], E.d_ql[L], E.d_khat[L], E.d_tw, E.d_topwork, E.d_el[L],
                   layout == MSM_LAYOUT_ASIS && L >= 2, s);
-  T.mark("grid2grid_top", s);
+  T.close(t, "grid2grid_top", s);
   for (int l = L - 1; l >= 1; --l) {
-    (void)precision;
-    msm::launch_level_fp64(E.P, E.gd[l], E.gd[l + 1], E.sd[l], E.d_khat[l], E.d_ql[l],
-                           E.d_el[l + 1], E.d_el[l], s);
-    T.mark("grid2grid", s);
+    t = T.open(s);
+    msm::launch_prolong_sum(E.P, E.gd[l], E.gd[l + 1], E.g2g[pr].lv[l - 1].G, E.d_part[pr][l],
+                            E.d_el[l + 1], E.d_el[l], s);
+    T.close(t, "prolongation", s);
   }
 }
 
 void stage_energy(Engine &E, int layout, hipStream_t s) {
   msm::launch_grid_energy(E.P, E.gd[1], E.d_ql[1], E.d_el[1], E.d_egrid,
                           layout == MSM_LAYOUT_ASIS && E.plan.L == 1, s);
-  msm::launch_finalize(E.P, E.d_epair, msm::pair_blocks(E.P.N), E.d_egrid,
+  msm::launch_finalize(E.P, E.d_epair, E.P.ncell, E.d_egrid,
                        msm::grid_blocks(E.gd[1].size()), E.d_scal, E.d_out, s);
 }
 
@@ -342,20 +445,35 @@ int check_built(FF *ff) {
   return MSM_OK;
 }
 
+// Pairs on the second stream, concurrently with the grid chain; the join
+// precedes the energy reduction and the interpolation, which add to the
+// pair results.
+int fork_pairs(Engine &E, int precision, double *F, double *phi, hipStream_t s, Timing &T) {
+  HIP_TRY(hipEventRecord(E.ev_fork, s));
+  HIP_TRY(hipStreamWaitEvent(E.stream2, E.ev_fork, 0));
+  int t = T.open(E.stream2);
+  stage_pairs(E, precision, F, phi, E.stream2);
+  T.close(t, "partcl2partcl", E.stream2);
+  HIP_TRY(hipEventRecord(E.ev_join, E.stream2));
+  return MSM_OK;
+}
+
 // Full evaluation on device pointers; out receives {E, const, pair, grid}.
 int eval_chain(FFX *x, const double *pos, const double *q, double *out, double *F, double *phi,
                hipStream_t s, Timing &T) {
   Engine &E = *x->eng;
-  T.mark("begin", s);
+  int t = T.open(s);
   stage_prepare(E, pos, q, s);
-  T.mark("nlist", s);
-  stage_pairs(E, x->precision, F, phi, s);
-  T.mark("partcl2partcl", s);
+  T.close(t, "nlist", s);
+  TRY(fork_pairs(E, x->precision, F, phi, s, T));
   stage_grids(E, x->precision, x->layout, T, s);
+  HIP_TRY(hipStreamWaitEvent(s, E.ev_join, 0));
+  t = T.open(s);
   stage_energy(E, x->layout, s);
-  T.mark("energy", s);
+  T.close(t, "energy", s);
+  t = T.open(s);
   if (F || phi) msm::launch_gather(E.P, E.gd[1], E.d_spos, E.d_sidx, E.d_el[1], E.d_scal, F, phi, s);
-  T.mark("interpolation", s);
+  T.close(t, "interpolation", s);
   if (out != E.d_out) HIP_TRY(hipMemcpyAsync(out, E.d_out, 4 * sizeof(double), hipMemcpyDeviceToDevice, s));
   HIP_TRY(hipGetLastError());
   if (T.on) ++E.timed_evals;
@@ -364,7 +482,7 @@ int eval_chain(FFX *x, const double *pos, const double *q, double *out, double *
 }
 
 void publish_timing(FFX *x, Timing &T) {
-  if (!T.on || T.ev.size() < 2) return;
+  if (!T.on || T.iv.empty()) return;
   FF *ff = &x->ff;
   ff->time_partcl2partcl = ff->time_anterpolation = ff->time_interpolation = 0;
   ff->time_nlist = 0;
@@ -372,25 +490,27 @@ void publish_timing(FFX *x, Timing &T) {
     ff->time_grid2grid[l] = ff->time_restriction[l] = ff->time_prolongation[l] =
         ff->time_padding[l] = 0;
   int L = x->eng->plan.L, lr = 2, lg = L - 1;
-  for (size_t k = 1; k < T.ev.size(); ++k) {
+  for (auto &v : T.iv) {
     float ms = 0;
-    hipEventElapsedTime(&ms, T.ev[k - 1], T.ev[k]);
+    hipEventElapsedTime(&ms, v.a, v.b);
     double sec = ms * 1e-3;
-    const std::string &t = T.tag[k];
+    const std::string &t = v.tag;
     if (t == "nlist") ff->time_nlist = sec;
     else if (t == "partcl2partcl") ff->time_partcl2partcl = sec;
     else if (t == "anterpolation") ff->time_anterpolation = sec;
     else if (t == "restriction") { if (lr <= 9) ff->time_restriction[lr] = sec; ++lr; }
     else if (t == "grid2grid_top") { if (L <= 9) ff->time_grid2grid[L] = sec; }
-    else if (t == "grid2grid") { if (lg >= 1 && lg <= 9) ff->time_grid2grid[lg] = sec; --lg; }
+    else if (t == "grid2grid") ff->time_grid2grid[1] = sec;  // all levels l < L, one launch
+    else if (t == "prolongation") { if (lg >= 1 && lg <= 9) ff->time_prolongation[lg] = sec; --lg; }
     else if (t == "interpolation") ff->time_interpolation = sec;
   }
 }
 
 // Stage names reported by msm_stage_times, in order.
-const char *kStageNames[] = {"nlist", "partcl2partcl", "anterpolation", "restriction",
-                             "grid2grid_top", "grid2grid", "energy", "interpolation"};
-constexpr int kNumStages = 8;
+const char *kStageNames[] = {"nlist",         "partcl2partcl", "anterpolation",
+                             "restriction",   "grid2grid",     "grid2grid_top",
+                             "prolongation",  "energy",        "interpolation"};
+constexpr int kNumStages = 9;
 
 void publish_plan(FFX *x) {
   FF *ff = &x->ff;
@@ -494,13 +614,18 @@ int build_impl(FF *ff, int N, double (*position)[3], double edges[3][3]) {
       std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   TRY(create_engine(x, plan));
   publish_plan(x);
-  x->built = true;
   for (int l = 1; l <= plan.L && l < 10; ++l) ff->time_stencil[l] = tstencil / plan.L;
-  // neighbour statistics for the reference's JSON fields
-  TRY(stage_host_inputs(x, position));
+  // neighbour statistics for the reference's JSON fields; the FF counts as
+  // built only once this first GPU pass has succeeded
+  x->built = true;
   long long cnt = 0;
   auto t1 = std::chrono::steady_clock::now();
-  TRY(count_pairs_dev(x, x->eng->d_pos, x->eng->d_q, &cnt));
+  int rc = stage_host_inputs(x, position);
+  if (rc == MSM_OK) rc = count_pairs_dev(x, x->eng->d_pos, x->eng->d_q, &cnt);
+  if (rc != MSM_OK) {
+    x->built = false;
+    return rc;
+  }
   ff->time_nlist = std::chrono::duration<double>(std::chrono::steady_clock::now() - t1).count();
   ff->nlist_interaction_count = (int)cnt;
   return MSM_OK;
@@ -812,13 +937,13 @@ int msm_stage_times(FF *ff, double *out, int n) {
   Engine &E = *X(ff)->eng;
   Timing &T = E.T;
   for (int k = 0; k < n; ++k) out[k] = 0.0;
-  if (!T.ev.empty()) HIP_TRY(hipEventSynchronize(T.ev.back()));
-  for (size_t k = 1; k < T.ev.size(); ++k) {
-    if (T.tag[k] == "begin") continue;
+  HIP_TRY(hipStreamSynchronize(E.stream2));
+  for (auto &v : T.iv) {
+    HIP_TRY(hipEventSynchronize(v.b));
     float ms = 0;
-    HIP_TRY(hipEventElapsedTime(&ms, T.ev[k - 1], T.ev[k]));
+    HIP_TRY(hipEventElapsedTime(&ms, v.a, v.b));
     for (int j = 0; j < kNumStages && j < n; ++j)
-      if (T.tag[k] == kStageNames[j]) out[j] += ms * 1e-3;
+      if (v.tag == kStageNames[j]) out[j] += ms * 1e-3;
   }
   int evals = E.timed_evals;
   T.clear();
@@ -837,8 +962,9 @@ int msm_forward_device(FF *ff, const double *d_pos, const double *d_q, double *d
   hipStream_t s = stream ? (hipStream_t)stream : E.stream;
   Timing T;
   stage_prepare(E, d_pos, d_q, s);
-  stage_pairs(E, x->precision, nullptr, nullptr, s);
+  TRY(fork_pairs(E, x->precision, nullptr, nullptr, s, T));
   stage_grids(E, x->precision, x->layout, T, s);
+  HIP_TRY(hipStreamWaitEvent(s, E.ev_join, 0));
   stage_energy(E, x->layout, s);
   HIP_TRY(hipMemcpyAsync(d_energy, E.d_out, 4 * sizeof(double), hipMemcpyDeviceToDevice, s));
   HIP_TRY(hipGetLastError());
@@ -861,7 +987,7 @@ int msm_adjoint_device(FF *ff, const double *d_pos, const double *d_q, double *d
     stage_prepare(E, d_pos, d_q, s);
     stage_grids(E, x->precision, x->layout, T, s);
   }
-  stage_pairs(E, x->precision, d_force, d_phi, s);
+  stage_pairs(E, x->precision, d_force, d_phi, s);  // the pair kernel writes, gather adds
   if (d_force || d_phi)
     msm::launch_gather(E.P, E.gd[1], E.d_spos, E.d_sidx, E.d_el[1], E.d_scal, d_force, d_phi, s);
   HIP_TRY(hipGetLastError());

@@ -51,9 +51,9 @@ void launch_scatter(const DevParams &P, const int *cell_of, const double4 *wpos,
 void launch_cell_sort(const DevParams &P, const int *start, double4 *spos, int *sidx,
                       float4 *lpos, hipStream_t s);
 
-// Short-range pairs, full neighbour shell, one thread per atom.  Writes
-// force (if F) and potential (if phi) at the original atom index, and one
-// energy partial per block into epart.
+// Short-range pairs, full neighbour shell, one wavefront per home cell.
+// Writes force (if F) and potential (if phi) at the original atom index,
+// and one energy partial per cell into epart (ncell doubles).
 int pair_blocks(int N);
 void launch_pair_fp64(const DevParams &P, const double4 *spos, const int *sidx, const int *scell,
                       const int *start, const int4 *sten, int nsten, double *F, double *phi,
@@ -65,8 +65,10 @@ void launch_pair_fp32(const DevParams &P, const float4 *lpos, const int *sidx, c
 void launch_pair_count(const DevParams &P, const double4 *spos, const int *scell, const int *start,
                        const int4 *sten, int nsten, unsigned long long *count, hipStream_t s);
 
-// B-spline spreading of charges onto the level-1 grid (grid zeroed here).
-void launch_spread(const DevParams &P, GridDims g, const double4 *spos, double *q1, hipStream_t s);
+// B-spline spreading of charges onto the level-1 grid, one workgroup per
+// grid brick (every grid point written exactly once).
+void launch_spread(const DevParams &P, GridDims g, const double4 *spos, const int *start,
+                   double *q1, hipStream_t s);
 // q^l = R q^{l-1} (full weighting with the two-scale stencil).
 void launch_restrict(const DevParams &P, GridDims fine, GridDims coarse, const double *qf,
                      double *qc, hipStream_t s);
@@ -79,14 +81,37 @@ void launch_top(const DevParams &P, GridDims g, const double *qL, const double *
 void launch_level_fp64(const DevParams &P, GridDims g, GridDims gc, GridDims sd,
                        const double *khat, const double *ql, const double *ec, double *el,
                        hipStream_t s);
-// Same with the stencil correlation as BF16 hi/lo split MFMA products
-// accumulated in FP32 (khat_split: prepared by prepare_split_stencil).
-void launch_level_mfma(const DevParams &P, GridDims g, GridDims gc, GridDims sd,
-                       const void *khat_split, const double *ql, const double *ec, double *el,
-                       void *qsplit, hipStream_t s);
-size_t split_stencil_bytes(GridDims sd);
-size_t split_grid_bytes(GridDims g, GridDims sd);
-void prepare_split_stencil(GridDims sd, const double *khat, void *host_out);
+
+// ---- level stencils on the matrix cores (msm_mfma.hip) ----
+constexpr int MSM_G2G_FP64 = 0;    // v_mfma_f64_16x16x4_f64
+constexpr int MSM_G2G_BF16X3 = 1;  // v_mfma_f32_16x16x32_bf16, hi/lo split, FP32 accumulation
+
+struct G2GLevel {
+  GridDims g, s;          // level grid and stencil dims
+  int nch;                // K chunks per (kx, ky)
+  int G;                  // kx groups (one partial grid each)
+  int ty, tz;             // 16-wide tiles along y and z
+  long task0;             // first wavefront of this level in the launch
+  const double *q;        // q^l
+  double *part;           // G x M^3 partial correlations
+  const void *bfrag;      // Toeplitz B operands in per-lane layout
+};
+struct G2GArgs {
+  G2GLevel lv[9];
+  int nlev;
+  long ntasks;
+};
+// Chooses tiles, K chunks and kx groups for about target_waves wavefronts.
+void plan_g2g_level(int prec, GridDims g, GridDims s, int target_waves, G2GLevel &L);
+size_t g2g_bfrag_bytes(int prec, const G2GLevel &L);
+void g2g_bfrag_fill(int prec, const G2GLevel &L, const double *khat, void *host_out);
+size_t g2g_lds_bytes(int prec, const G2GLevel &L);
+int g2g_max_nch(int prec);
+// All levels' stencil correlations in one launch (one wavefront per task).
+void launch_g2g_tiles(int prec, G2GArgs args, size_t lds, hipStream_t s);
+// e^l = sum of the G partials + prolongation of e^{l+1}.
+void launch_prolong_sum(const DevParams &P, GridDims g, GridDims gc, int G, const double *part,
+                        const double *ec, double *el, hipStream_t s);
 
 // Grid energy partials: sum q1*e1 per block (asis != 0: the reference
 // HEAD's L = 1 index map).

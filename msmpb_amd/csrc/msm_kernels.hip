@@ -188,30 +188,46 @@ __global__ __launch_bounds__(256) void k_scatter(int N, const int *__restrict__ 
   scell[slot] = c;
 }
 
+// One wavefront per cell: ranks the cell's atoms by original index (so the
+// order, and every sum built on it, is deterministic), moves them in place
+// and writes the cell-local FP32 coordinates used by the mixed pair kernel.
 __global__ __launch_bounds__(256) void k_cell_sort(DevParams P, const int *__restrict__ start,
                                                    double4 *__restrict__ spos,
                                                    int *__restrict__ sidx,
                                                    float4 *__restrict__ lpos) {
-  int c = blockIdx.x * 256 + threadIdx.x;
+  int c = blockIdx.x * 4 + (threadIdx.x >> 6);
+  int lane = threadIdx.x & 63;
   if (c >= P.ncell) return;
-  int b = start[c], e = start[c + 1];
-  for (int k = b + 1; k < e; ++k) {  // insertion sort by original index
-    int key = sidx[k];
-    double4 v = spos[k];
-    int m = k - 1;
-    while (m >= b && sidx[m] > key) {
-      sidx[m + 1] = sidx[m];
-      spos[m + 1] = spos[m];
-      --m;
-    }
-    sidx[m + 1] = key;
-    spos[m + 1] = v;
-  }
+  int b = start[c], n = start[c + 1] - b;
   int cz = c % P.nc[2], cy = (c / P.nc[2]) % P.nc[1], cx = c / (P.nc[1] * P.nc[2]);
   double ox = cx * P.cw[0], oy = cy * P.cw[1], oz = cz * P.cw[2];
-  for (int k = b; k < e; ++k) {
-    double4 v = spos[k];
-    lpos[k] = make_float4((float)(v.x - ox), (float)(v.y - oy), (float)(v.z - oz), (float)v.w);
+  if (n <= 64) {
+    int key = lane < n ? sidx[b + lane] : 0x7fffffff;
+    double4 v = lane < n ? spos[b + lane] : make_double4(0, 0, 0, 0);
+    int rank = 0;
+    for (int j = 0; j < n; ++j) rank += __shfl(key, j, 64) < key;
+    if (lane < n) {
+      sidx[b + rank] = key;
+      spos[b + rank] = v;
+      lpos[b + rank] = make_float4((float)(v.x - ox), (float)(v.y - oy), (float)(v.z - oz), (float)v.w);
+    }
+  } else if (lane == 0) {  // rare crowded cell: insertion sort by one lane
+    for (int k = b + 1; k < b + n; ++k) {
+      int key = sidx[k];
+      double4 v = spos[k];
+      int m = k - 1;
+      while (m >= b && sidx[m] > key) {
+        sidx[m + 1] = sidx[m];
+        spos[m + 1] = spos[m];
+        --m;
+      }
+      sidx[m + 1] = key;
+      spos[m + 1] = v;
+    }
+    for (int k = b; k < b + n; ++k) {
+      double4 v = spos[k];
+      lpos[k] = make_float4((float)(v.x - ox), (float)(v.y - oy), (float)(v.z - oz), (float)v.w);
+    }
   }
 }
 
@@ -221,123 +237,175 @@ __global__ __launch_bounds__(256) void k_cell_sort(DevParams P, const int *__res
 // the cutoff, with g(r) = 1/r - tau(r^2/a0^2 - 1)/a0.  Neighbour cells are
 // visited through a stencil of cell offsets; an offset that leaves the box
 // carries the image shift, so small boxes get every image exactly once.
+//
+// One wavefront per home cell: the cell's atoms (up to 64 per pass) occupy
+// lane groups of S lanes (S = 4, 2 or 1 as the cell holds <= 16, <= 32 or
+// more atoms), and for every stencil offset -- the same for the whole
+// wavefront -- the S lanes of a group stride over that neighbour cell's
+// atoms.  All lanes run the same loop, a j atom is fetched once for all
+// groups, and a group's partial sums are combined with shuffles; no
+// atomics, so the result is deterministic.
 constexpr int kPairBS = 256;
 
-__global__ __launch_bounds__(kPairBS) void k_pair64(
-    DevParams P, const double4 *__restrict__ spos, const int *__restrict__ sidx,
-    const int *__restrict__ scell, const int *__restrict__ start, const int4 *__restrict__ sten,
-    int nsten, double *__restrict__ F, double *__restrict__ phi, double *__restrict__ epart) {
-  int i = blockIdx.x * kPairBS + threadIdx.x;
-  double acc = 0.0, fx = 0.0, fy = 0.0, fz = 0.0, qi = 0.0;
-  if (i < P.N) {
-    double4 ai = spos[i];
-    qi = ai.w;
-    int c = scell[i];
-    int cz = c % P.nc[2], cy = (c / P.nc[2]) % P.nc[1], cx = c / (P.nc[1] * P.nc[2]);
-    const int nu = P.nu;
-    for (int o = 0; o < nsten; ++o) {
-      int4 d = sten[o];
-      int ox = cx + d.x, oy = cy + d.y, oz = cz + d.z;
-      int kx = floordiv(ox, P.nc[0]), ky = floordiv(oy, P.nc[1]), kz = floordiv(oz, P.nc[2]);
-      ox -= kx * P.nc[0], oy -= ky * P.nc[1], oz -= kz * P.nc[2];
-      double bx = kx * P.box[0] - ai.x, by = ky * P.box[1] - ai.y, bz = kz * P.box[2] - ai.z;
-      int cn = (ox * P.nc[1] + oy) * P.nc[2] + oz;
-      int jb = start[cn], je = start[cn + 1];
-      for (int j = jb; j < je; ++j) {
-        double4 aj = spos[j];
-        double dx = aj.x + bx, dy = aj.y + by, dz = aj.z + bz;
-        double r2 = dx * dx + dy * dy + dz * dz;
-        if (r2 < P.a02 && j != i) {
-          double s = r2 * P.inv_a02 - 1.0;
-          double ts = P.tau[nu - 1], t1 = 0.0;
-          for (int k = nu - 2; k >= 0; --k) {
-            t1 = ts + s * t1;
-            ts = P.tau[k] + s * ts;
-          }
-          double rinv = 1.0 / sqrt(r2);
-          double e = rinv - ts * P.inv_a0;
-          double cf = -rinv * rinv * rinv - t1 * P.two_inv_a03;
-          acc += aj.w * e;
-          double qc = aj.w * cf;
-          fx += qc * dx, fy += qc * dy, fz += qc * dz;
-        }
-      }
-    }
-    int oi = sidx[i];
-    if (F) {
-      F[3 * (long)oi] = qi * fx;
-      F[3 * (long)oi + 1] = qi * fy;
-      F[3 * (long)oi + 2] = qi * fz;
-    }
-    if (phi) phi[oi] = acc;
-  }
-  double t = block_sum<kPairBS>(0.5 * qi * acc);
-  if (threadIdx.x == 0) epart[blockIdx.x] = t;
-}
+// Neighbour atoms are staged through LDS in chunks: the 64 lanes take 64
+// stencil offsets at a time, a wavefront scan of their cell populations
+// decides how many cells fit the chunk, and the lanes copy those atoms --
+// already shifted into the home cell's frame -- cooperatively.  The
+// interaction loop then reads only LDS.
+template <bool MIXED>
+struct PairStage;
+template <>
+struct PairStage<true> {
+  using T = float4;
+  static constexpr int CAP = 512;
+};
+template <>
+struct PairStage<false> {
+  using T = double4;
+  static constexpr int CAP = 256;
+};
 
-// Mixed precision: the same sum with FP32 pair arithmetic on cell-local
-// coordinates (|x| < cell width, so differences lose no bits to the box
-// size) and FP64 accumulation per atom.
-__global__ __launch_bounds__(kPairBS) void k_pair32(
-    DevParams P, const float4 *__restrict__ lpos, const int *__restrict__ sidx,
-    const int *__restrict__ scell, const int *__restrict__ start, const int4 *__restrict__ sten,
+template <bool MIXED, int NU>
+__global__ __launch_bounds__(64) void k_pair_cell(
+    DevParams P, const double4 *__restrict__ spos, const float4 *__restrict__ lpos,
+    const int *__restrict__ sidx, const int *__restrict__ start, const int4 *__restrict__ sten,
     int nsten, double *__restrict__ F, double *__restrict__ phi, double *__restrict__ epart) {
-  int i = blockIdx.x * kPairBS + threadIdx.x;
-  double acc = 0.0, fx = 0.0, fy = 0.0, fz = 0.0, qi = 0.0;
-  if (i < P.N) {
-    float4 ai = lpos[i];
-    qi = ai.w;
-    int c = scell[i];
-    int cz = c % P.nc[2], cy = (c / P.nc[2]) % P.nc[1], cx = c / (P.nc[1] * P.nc[2]);
-    const int nu = P.nu;
-    const float a02 = (float)P.a02, ia02 = (float)P.inv_a02, ia0 = (float)P.inv_a0;
-    const float c3 = (float)P.two_inv_a03;
-    float tau[kMaxNu];
-#pragma unroll
-    for (int k = 0; k < kMaxNu; ++k) tau[k] = (float)P.tau[k];
-    const float cwx = (float)P.cw[0], cwy = (float)P.cw[1], cwz = (float)P.cw[2];
-    for (int o = 0; o < nsten; ++o) {
-      int4 d = sten[o];
-      int ox = cx + d.x, oy = cy + d.y, oz = cz + d.z;
-      int kx = floordiv(ox, P.nc[0]), ky = floordiv(oy, P.nc[1]), kz = floordiv(oz, P.nc[2]);
-      ox -= kx * P.nc[0], oy -= ky * P.nc[1], oz -= kz * P.nc[2];
-      // neighbour-cell origin relative to the home-cell origin
-      float bx = d.x * cwx - ai.x, by = d.y * cwy - ai.y, bz = d.z * cwz - ai.z;
-      int cn = (ox * P.nc[1] + oy) * P.nc[2] + oz;
-      int jb = start[cn], je = start[cn + 1];
-      for (int j = jb; j < je; ++j) {
-        float4 aj = lpos[j];
-        float dx = aj.x + bx, dy = aj.y + by, dz = aj.z + bz;
-        float r2 = dx * dx + dy * dy + dz * dz;
-        if (r2 < a02 && j != i) {
-          float s = r2 * ia02 - 1.0f;
-          float ts = tau[kMaxNu - 1], t1 = 0.0f;
-          // tau holds zeros above nu-1, so the fixed-length Horner is exact
-#pragma unroll
-          for (int k = kMaxNu - 2; k >= 0; --k) {
-            t1 = ts + s * t1;
-            ts = tau[k] + s * ts;
-          }
-          float rinv = rsqrtf(r2);
-          float e = rinv - ts * ia0;
-          float cf = -rinv * rinv * rinv - t1 * c3;
-          float qc = aj.w * cf;
-          acc += (double)(aj.w * e);
-          fx += (double)(qc * dx), fy += (double)(qc * dy), fz += (double)(qc * dz);
-        }
+  using T = typename PairStage<MIXED>::T;
+  constexpr int CAP = PairStage<MIXED>::CAP;
+  __shared__ T jbuf[CAP];
+  __shared__ int jidx[CAP];
+  __shared__ int seg_off[65], seg_beg[64];
+  __shared__ T seg_shift[64];
+  const int c = blockIdx.x, lane = threadIdx.x;
+  const int b0 = start[c], n = start[c + 1] - b0;
+  const int cz = c % P.nc[2], cy = (c / P.nc[2]) % P.nc[1], cx = c / (P.nc[1] * P.nc[2]);
+  double esum = 0.0;
+  for (int i0 = 0; i0 < n;) {
+    const int rem = n - i0;
+    const int S = rem <= 16 ? 4 : (rem <= 32 ? 2 : 1);
+    const int a = lane / S, bs = lane - a * S;
+    const bool act = a < rem;
+    const int ii = b0 + i0 + a;
+    T ai;
+    if constexpr (MIXED) ai = act ? lpos[ii] : make_float4(0.f, 0.f, 0.f, 0.f);
+    else ai = act ? spos[ii] : make_double4(0.0, 0.0, 0.0, 0.0);
+    double acc = 0.0, fx = 0.0, fy = 0.0, fz = 0.0;
+    for (int o0 = 0; o0 < nsten;) {
+      // ---- stage: lane k takes stencil offset o0 + k
+      const int o = o0 + lane;
+      int cnt = 0, jb = 0;
+      T sh;
+      if (o < nsten) {
+        int4 d = sten[o];
+        int ox = cx + d.x, oy = cy + d.y, oz = cz + d.z;
+        int kx = floordiv(ox, P.nc[0]), ky = floordiv(oy, P.nc[1]), kz = floordiv(oz, P.nc[2]);
+        ox -= kx * P.nc[0], oy -= ky * P.nc[1], oz -= kz * P.nc[2];
+        int cn = (ox * P.nc[1] + oy) * P.nc[2] + oz;
+        jb = start[cn];
+        cnt = start[cn + 1] - jb;
+        if constexpr (MIXED)  // neighbour-cell origin relative to the home-cell origin
+          sh = make_float4(d.x * (float)P.cw[0], d.y * (float)P.cw[1], d.z * (float)P.cw[2], 0.f);
+        else  // periodic image of the neighbour cell
+          sh = make_double4(kx * P.box[0], ky * P.box[1], kz * P.box[2], 0.0);
       }
+      int incl = cnt;
+      for (int off = 1; off < 64; off <<= 1) {
+        int t = __shfl_up(incl, off, 64);
+        if (lane >= off) incl += t;
+      }
+      unsigned long long fit = __ballot(o < nsten && incl <= CAP);
+      int m = __popcll(fit);  // leading lanes whose cells fit (counts are >= 0)
+      if (m == 0) m = 1;      // one cell larger than the chunk: read it in pieces below
+      const int total = __shfl(incl, m - 1, 64);
+      if (lane < m) {
+        seg_off[lane] = incl - cnt;
+        seg_beg[lane] = jb;
+        seg_shift[lane] = sh;
+      }
+      if (lane == 0) seg_off[m] = total;
+      __syncthreads();
+      for (int p0 = 0; p0 < total; p0 += CAP) {
+        const int np = min(CAP, total - p0);
+        for (int e = lane; e < np; e += 64) {
+          int g = p0 + e, lo = 0, hi = m;  // seg_off[lo] <= g < seg_off[hi]
+          while (hi - lo > 1) {
+            int mid = (lo + hi) >> 1;
+            if (seg_off[mid] <= g) lo = mid; else hi = mid;
+          }
+          int j = seg_beg[lo] + (g - seg_off[lo]);
+          T v, s2 = seg_shift[lo];
+          if constexpr (MIXED) v = lpos[j];
+          else v = spos[j];
+          v.x += s2.x, v.y += s2.y, v.z += s2.z;
+          jbuf[e] = v;
+          jidx[e] = j;
+        }
+        __syncthreads();
+        // ---- interactions of the staged atoms with this lane group's atom
+        for (int e = bs; e < np; e += S) {
+          T aj = jbuf[e];
+          if constexpr (MIXED) {
+            float dx = aj.x - ai.x, dy = aj.y - ai.y, dz = aj.z - ai.z;
+            float r2 = dx * dx + dy * dy + dz * dz;
+            if (r2 < (float)P.a02 && jidx[e] != ii && act) {
+              float s = r2 * (float)P.inv_a02 - 1.0f;
+              float ts = (float)P.tau[NU - 1], t1 = 0.0f;
+#pragma unroll
+              for (int k = NU - 2; k >= 0; --k) {
+                t1 = ts + s * t1;
+                ts = (float)P.tau[k] + s * ts;
+              }
+              float rinv = rsqrtf(r2);
+              float en = rinv - ts * (float)P.inv_a0;
+              float qc = aj.w * (-rinv * rinv * rinv - t1 * (float)P.two_inv_a03);
+              acc += (double)(aj.w * en);
+              fx += (double)(qc * dx), fy += (double)(qc * dy), fz += (double)(qc * dz);
+            }
+          } else {
+            double dx = aj.x - ai.x, dy = aj.y - ai.y, dz = aj.z - ai.z;
+            double r2 = dx * dx + dy * dy + dz * dz;
+            if (r2 < P.a02 && jidx[e] != ii && act) {
+              double s = r2 * P.inv_a02 - 1.0;
+              double ts = P.tau[NU - 1], t1 = 0.0;
+#pragma unroll
+              for (int k = NU - 2; k >= 0; --k) {
+                t1 = ts + s * t1;
+                ts = P.tau[k] + s * ts;
+              }
+              double rinv = 1.0 / sqrt(r2);
+              double en = rinv - ts * P.inv_a0;
+              double qc = aj.w * (-rinv * rinv * rinv - t1 * P.two_inv_a03);
+              acc += aj.w * en;
+              fx += qc * dx, fy += qc * dy, fz += qc * dz;
+            }
+          }
+        }
+        __syncthreads();
+      }
+      o0 += m;
     }
-    (void)nu;
-    int oi = sidx[i];
-    if (F) {
-      F[3 * (long)oi] = qi * fx;
-      F[3 * (long)oi + 1] = qi * fy;
-      F[3 * (long)oi + 2] = qi * fz;
+    // combine the S lanes of each group (consecutive lanes)
+    for (int off = 1; off < S; off <<= 1) {
+      acc += __shfl_xor(acc, off, 64);
+      fx += __shfl_xor(fx, off, 64);
+      fy += __shfl_xor(fy, off, 64);
+      fz += __shfl_xor(fz, off, 64);
     }
-    if (phi) phi[oi] = acc;
+    if (act && bs == 0) {
+      double qi = ai.w;
+      int oi = sidx[ii];
+      if (F) {
+        F[3 * (long)oi] = qi * fx;
+        F[3 * (long)oi + 1] = qi * fy;
+        F[3 * (long)oi + 2] = qi * fz;
+      }
+      if (phi) phi[oi] = acc;
+      esum += 0.5 * qi * acc;
+    }
+    i0 += 64 / S;
   }
-  double t = block_sum<kPairBS>(0.5 * qi * acc);
-  if (threadIdx.x == 0) epart[blockIdx.x] = t;
+  for (int off = 32; off > 0; off >>= 1) esum += __shfl_down(esum, off, 64);
+  if (lane == 0) epart[c] = esum;
 }
 
 __global__ __launch_bounds__(kPairBS) void k_pair_count(
@@ -374,75 +442,146 @@ __global__ __launch_bounds__(kPairBS) void k_pair_count(
 }
 
 // ---------------------------------------------------------------- spread
-// anterpolate (forcefield1.c:324-368): q^1_{m-i+NU/2} += q Q(i+t) per axis.
-// Atoms arrive sorted by cell, so a wavefront's atomics hit a compact
-// region of the grid that stays resident in L2.
+// anterpolate (forcefield1.c:324-368): q^1_{m-i+NU/2} += q Q(i+t) per axis,
+// restated as a gather by grid bricks.  One workgroup owns a brick of the
+// level-1 grid; it lists the pair-search cell columns whose atoms can reach
+// the brick (atoms are sorted by cell, so each column is one or two
+// contiguous index ranges), accumulates every reaching atom's clipped
+// NU^3 stencil in LDS, and stores the brick once.  No global atomics.
+constexpr int kBX = 8, kBY = 8, kBZ = 4;
+constexpr int kMaxRanges = 1024;
+
+__device__ __forceinline__ void cell_span(double g0, int B, int half, int M, int nc, int &lo,
+                                          int &len) {
+  // cells holding atoms whose grid index m lies in [g0 - half, g0 + B + half - 2];
+  // one cell of margin each side absorbs rounding in the binning
+  lo = (int)floor(nc * (g0 - half) / M) - 1;
+  int hi = (int)floor(nc * (g0 + B + half - 1) / M) + 1;
+  len = hi - lo + 1;
+  if (len >= nc) lo = 0, len = nc;
+}
+
 template <int NU>
-__global__ __launch_bounds__(256) void k_spread(DevParams P, GridDims g,
-                                                const double4 *__restrict__ spos,
-                                                double *__restrict__ q1) {
-  int i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= P.N) return;
-  double4 a = spos[i];
-  double r[3] = {a.x, a.y, a.z};
-  int dims[3] = {g.x, g.y, g.z};
-  double t[3];
-  int m[3];
-#pragma unroll
-  for (int ax = 0; ax < 3; ++ax) {
-    double s = r[ax] * P.ibox[ax];
-    s = s - floor(s);
-    double tt = dims[ax] * s, mm = floor(tt);
-    t[ax] = tt - mm;
-    m[ax] = (int)mm;
-  }
-  double wz[NU];
-  int iz[NU];
-  bspline<NU, false>(P.Q, t[2], wz, nullptr);
-#pragma unroll
-  for (int k = 0; k < NU; ++k) iz[k] = wrapi(m[2] - k + NU / 2, g.z);
-#pragma unroll 1
-  for (int x = 0; x < NU; ++x) {
-    double wx = a.w * bspline_one<NU, false>(P.Q, x, t[0], nullptr);
-    long xo = (long)wrapi(m[0] - x + NU / 2, g.x) * g.y;
-#pragma unroll 1
-    for (int y = 0; y < NU; ++y) {
-      double wxy = wx * bspline_one<NU, false>(P.Q, y, t[1], nullptr);
-      double *row = q1 + (xo + wrapi(m[1] - y + NU / 2, g.y)) * g.z;
-#pragma unroll
-      for (int z = 0; z < NU; ++z) unsafeAtomicAdd(row + iz[z], wxy * wz[z]);
+__global__ __launch_bounds__(256) void k_spread_brick(DevParams P, GridDims g,
+                                                      const double4 *__restrict__ spos,
+                                                      const int *__restrict__ start,
+                                                      double *__restrict__ q1) {
+  __shared__ double acc[kBX * kBY * kBZ];
+  __shared__ int rbeg[kMaxRanges], roff[kMaxRanges + 1];
+  const int tid = threadIdx.x;
+  const int nbz = (g.z + kBZ - 1) / kBZ, nby = (g.y + kBY - 1) / kBY;
+  int b = blockIdx.x;
+  const int gz0 = (b % nbz) * kBZ;
+  b /= nbz;
+  const int gy0 = (b % nby) * kBY, gx0 = (b / nby) * kBX;
+  for (int k = tid; k < kBX * kBY * kBZ; k += 256) acc[k] = 0.0;
+  int lox, lenx, loy, leny, loz, lenz;
+  cell_span(gx0, kBX, NU / 2, g.x, P.nc[0], lox, lenx);
+  cell_span(gy0, kBY, NU / 2, g.y, P.nc[1], loy, leny);
+  cell_span(gz0, kBZ, NU / 2, g.z, P.nc[2], loz, lenz);
+  const int ncol = lenx * leny;
+  const int dims[3] = {g.x, g.y, g.z}, org[3] = {gx0, gy0, gz0}, ext[3] = {kBX, kBY, kBZ};
+  // columns in batches small enough for the range table; each column's z
+  // cells [loz, loz + lenz) wrap at most once, giving two index ranges
+  for (int c0 = 0; c0 < ncol; c0 += kMaxRanges / 2) {
+    const int nc_b = min(ncol - c0, kMaxRanges / 2);
+    __syncthreads();  // previous batch done with the table
+    for (int cc = tid; cc < nc_b; cc += 256) {
+      int col = c0 + cc;
+      int x = wrapi(lox + col / leny, P.nc[0]), y = wrapi(loy + col % leny, P.nc[1]);
+      int base = (x * P.nc[1] + y) * P.nc[2];
+      int z0 = wrapi(loz, P.nc[2]), z1 = z0 + lenz;
+      int e1 = min(z1, P.nc[2]), e2 = z1 > P.nc[2] ? z1 - P.nc[2] : 0;
+      rbeg[2 * cc] = start[base + z0];
+      roff[2 * cc + 1] = start[base + e1] - start[base + z0];
+      rbeg[2 * cc + 1] = start[base];
+      roff[2 * cc + 2] = start[base + e2] - start[base];
     }
+    __syncthreads();
+    const int nr = 2 * nc_b;
+    if (tid == 0) {  // prefix of the range lengths
+      roff[0] = 0;
+      for (int k = 0; k < nr; ++k) roff[k + 1] += roff[k];
+    }
+    __syncthreads();
+    const int total = roff[nr];
+    for (int k = tid; k < total; k += 256) {
+      int lo = 0, hi = nr;  // roff[lo] <= k < roff[hi]
+      while (hi - lo > 1) {
+        int mid = (lo + hi) >> 1;
+        if (roff[mid] <= k) lo = mid; else hi = mid;
+      }
+      double4 at = spos[rbeg[lo] + (k - roff[lo])];
+      double r[3] = {at.x, at.y, at.z};
+      int loc[3][NU];
+      double w[3][NU];
+      bool any = true;
+#pragma unroll
+      for (int ax = 0; ax < 3; ++ax) {
+        double sv = r[ax] * P.ibox[ax];
+        sv = sv - floor(sv);
+        double t = dims[ax] * sv, m = floor(t);
+        t -= m;
+        int mi = (int)m, cnt = 0;
+#pragma unroll
+        for (int i = 0; i < NU; ++i) {
+          int l = wrapi(mi - i + NU / 2 - org[ax], dims[ax]);
+          loc[ax][i] = l < ext[ax] ? l : -1;
+          cnt += l < ext[ax];
+        }
+        any = any && cnt > 0;
+        if (any) bspline<NU, false>(P.Q, t, w[ax], nullptr);
+      }
+      if (!any) continue;
+#pragma unroll
+      for (int x = 0; x < NU; ++x) {
+        if (loc[0][x] < 0) continue;
+        double wx = at.w * w[0][x];
+#pragma unroll
+        for (int y = 0; y < NU; ++y) {
+          if (loc[1][y] < 0) continue;
+          double wxy = wx * w[1][y];
+          int row = (loc[0][x] * kBY + loc[1][y]) * kBZ;
+#pragma unroll
+          for (int z = 0; z < NU; ++z)
+            if (loc[2][z] >= 0) atomicAdd(&acc[row + loc[2][z]], wxy * w[2][z]);
+        }
+      }
+    }
+  }
+  __syncthreads();
+  for (int k = tid; k < kBX * kBY * kBZ; k += 256) {
+    int z = gz0 + k % kBZ, y = gy0 + (k / kBZ) % kBY, x = gx0 + k / (kBZ * kBY);
+    if (x < g.x && y < g.y && z < g.z) q1[((long)x * g.y + y) * g.z + z] = acc[k];
   }
 }
 
 // ---------------------------------------------------------------- restrict
 // restrict_ (forcefield1.c:401-477): q^l_m = sum_n J_nx J_ny J_nz
 // q^{l-1}_{2m+n}, n in [-NU/2, NU/2]^3, periodic.
+// One wavefront per coarse point: lane k takes tap pairs (nx, ny) = k, k+64,
+// ... of the (NU+1)^2 and sums its NU+1 z taps; a wavefront reduction adds
+// them (plenty of parallelism even for the small coarse grids).
 template <int NU>
 __global__ __launch_bounds__(256) void k_restrict(DevParams P, GridDims f, GridDims c,
                                                   const double *__restrict__ qf,
                                                   double *__restrict__ qc) {
-  long id = (long)blockIdx.x * 256 + threadIdx.x;
+  long id = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  int lane = threadIdx.x & 63;
   if (id >= c.size()) return;
   int z = id % c.z, y = (id / c.z) % c.y, x = id / ((long)c.z * c.y);
-  const int H = NU / 2;
-  int iz[NU + 1];
-#pragma unroll
-  for (int n = 0; n <= NU; ++n) iz[n] = wrapi(2 * z + n - H, f.z);
+  const int H = NU / 2, T = NU + 1;
   double acc = 0.0;
-  for (int nx = 0; nx <= NU; ++nx) {
-    long xo = (long)wrapi(2 * x + nx - H, f.x) * f.y;
-    double sy = 0.0;
-    for (int ny = 0; ny <= NU; ++ny) {
-      const double *row = qf + (xo + wrapi(2 * y + ny - H, f.y)) * f.z;
-      double sz = 0.0;
+  for (int k = lane; k < T * T; k += 64) {
+    int nx = k / T, ny = k - nx * T;
+    const double *row = qf + ((long)wrapi(2 * x + nx - H, f.x) * f.y + wrapi(2 * y + ny - H, f.y)) * f.z;
+    double sz = 0.0;
 #pragma unroll
-      for (int nz = 0; nz <= NU; ++nz) sz += P.J[nz] * row[iz[nz]];
-      sy += P.J[ny] * sz;
-    }
-    acc += P.J[nx] * sy;
+    for (int nz = 0; nz <= NU; ++nz) sz += P.J[nz] * row[wrapi(2 * z + nz - H, f.z)];
+    acc += P.J[nx] * P.J[ny] * sz;
   }
-  qc[id] = acc;
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+  if (lane == 0) qc[id] = acc;
 }
 
 // ---------------------------------------------------------------- top level
@@ -762,17 +901,42 @@ __device__ __forceinline__ void neumaier(double &s, double &c, double v) {
   s = t;
 }
 
-__global__ __launch_bounds__(64) void k_finalize(DevParams P, const double *__restrict__ ep, int np,
-                                                 const double *__restrict__ eg, int ng,
-                                                 const double *__restrict__ scal,
-                                                 double *__restrict__ out) {
-  if (threadIdx.x != 0) return;
+// Adds two compensated partial sums (s, c): error-free sum of the heads,
+// carries added.
+__device__ __forceinline__ void csum_add(double &s, double &c, double s2, double c2) {
+  double t = s + s2, bb = t - s;
+  double err = (s - (t - bb)) + (s2 - bb);
+  s = t;
+  c += c2 + err;
+}
+
+// Compensated sums of the per-cell pair partials and per-block grid partials:
+// each thread sums a fixed strided subset, then a fixed binary tree combines
+// the 256 (sum, carry) pairs, so the result does not depend on scheduling.
+__global__ __launch_bounds__(256) void k_finalize(DevParams P, const double *__restrict__ ep, int np,
+                                                  const double *__restrict__ eg, int ng,
+                                                  const double *__restrict__ scal,
+                                                  double *__restrict__ out) {
+  __shared__ double ss[2][256], cc[2][256];
+  int t = threadIdx.x;
   double s = 0, c = 0;
-  for (int k = 0; k < np; ++k) neumaier(s, c, ep[k]);
-  double epair = s + c;
+  for (int k = t; k < np; k += 256) neumaier(s, c, ep[k]);
+  ss[0][t] = s, cc[0][t] = c;
   s = 0, c = 0;
-  for (int k = 0; k < ng; ++k) neumaier(s, c, eg[k]);
-  double egrid = 0.5 * (s + c);
+  for (int k = t; k < ng; k += 256) neumaier(s, c, eg[k]);
+  ss[1][t] = s, cc[1][t] = c;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (t < h)
+      for (int w = 0; w < 2; ++w) {
+        double a = ss[w][t], b = cc[w][t];
+        csum_add(a, b, ss[w][t + h], cc[w][t + h]);
+        ss[w][t] = a, cc[w][t] = b;
+      }
+    __syncthreads();
+  }
+  if (t != 0) return;
+  double epair = ss[0][0] + cc[0][0], egrid = 0.5 * (ss[1][0] + cc[1][0]);
   double qs = scal[0], q2 = scal[1];
   // constant part (forcefield1.c:114-130)
   double econst = 0.5 * q2 * P.coeff1 - 0.5 * (qs * qs) * P.coeff2 - 0.5 * q2 * P.gamma0 * P.inv_a0;
@@ -817,29 +981,10 @@ void launch_scatter(const DevParams &P, const int *cell_of, const double4 *wpos,
 
 void launch_cell_sort(const DevParams &P, const int *start, double4 *spos, int *sidx,
                       float4 *lpos, hipStream_t s) {
-  k_cell_sort<<<nblk(P.ncell, 256), 256, 0, s>>>(P, start, spos, sidx, lpos);
+  k_cell_sort<<<nblk(P.ncell, 4), 256, 0, s>>>(P, start, spos, sidx, lpos);
 }
 
 int pair_blocks(int N) { return (int)nblk(N, kPairBS); }
-
-void launch_pair_fp64(const DevParams &P, const double4 *spos, const int *sidx, const int *scell,
-                      const int *start, const int4 *sten, int nsten, double *F, double *phi,
-                      double *epart, hipStream_t s) {
-  k_pair64<<<pair_blocks(P.N), kPairBS, 0, s>>>(P, spos, sidx, scell, start, sten, nsten, F, phi,
-                                                epart);
-}
-
-void launch_pair_fp32(const DevParams &P, const float4 *lpos, const int *sidx, const int *scell,
-                      const int *start, const int4 *sten, int nsten, double *F, double *phi,
-                      double *epart, hipStream_t s) {
-  k_pair32<<<pair_blocks(P.N), kPairBS, 0, s>>>(P, lpos, sidx, scell, start, sten, nsten, F, phi,
-                                                epart);
-}
-
-void launch_pair_count(const DevParams &P, const double4 *spos, const int *scell, const int *start,
-                       const int4 *sten, int nsten, unsigned long long *count, hipStream_t s) {
-  k_pair_count<<<pair_blocks(P.N), kPairBS, 0, s>>>(P, spos, scell, start, sten, nsten, count);
-}
 
 #define MSM_NU_SWITCH(nu, CALL) \
   switch (nu) {                 \
@@ -849,16 +994,45 @@ void launch_pair_count(const DevParams &P, const double4 *spos, const int *scell
     default: CALL(10); break;   \
   }
 
-void launch_spread(const DevParams &P, GridDims g, const double4 *spos, double *q1, hipStream_t s) {
-  launch_fill_dbl(q1, g.size(), 0.0, s);
-#define CALL(NU) k_spread<NU><<<nblk(P.N, 256), 256, 0, s>>>(P, g, spos, q1)
+
+void launch_pair_fp64(const DevParams &P, const double4 *spos, const int *sidx, const int *scell,
+                      const int *start, const int4 *sten, int nsten, double *F, double *phi,
+                      double *epart, hipStream_t s) {
+  (void)scell;
+#define CALL(NU)                                                                             \
+  k_pair_cell<false, NU><<<P.ncell, 64, 0, s>>>(P, spos, nullptr, sidx, start, sten, nsten, F, \
+                                                 phi, epart)
+  MSM_NU_SWITCH(P.nu, CALL)
+#undef CALL
+}
+
+void launch_pair_fp32(const DevParams &P, const float4 *lpos, const int *sidx, const int *scell,
+                      const int *start, const int4 *sten, int nsten, double *F, double *phi,
+                      double *epart, hipStream_t s) {
+  (void)scell;
+#define CALL(NU)                                                                             \
+  k_pair_cell<true, NU><<<P.ncell, 64, 0, s>>>(P, nullptr, lpos, sidx, start, sten, nsten, F,  \
+                                                phi, epart)
+  MSM_NU_SWITCH(P.nu, CALL)
+#undef CALL
+}
+
+void launch_pair_count(const DevParams &P, const double4 *spos, const int *scell, const int *start,
+                       const int4 *sten, int nsten, unsigned long long *count, hipStream_t s) {
+  k_pair_count<<<pair_blocks(P.N), kPairBS, 0, s>>>(P, spos, scell, start, sten, nsten, count);
+}
+
+void launch_spread(const DevParams &P, GridDims g, const double4 *spos, const int *start,
+                   double *q1, hipStream_t s) {
+  unsigned nb = (unsigned)(((g.x + kBX - 1) / kBX) * ((g.y + kBY - 1) / kBY) * ((g.z + kBZ - 1) / kBZ));
+#define CALL(NU) k_spread_brick<NU><<<nb, 256, 0, s>>>(P, g, spos, start, q1)
   MSM_NU_SWITCH(P.nu, CALL)
 #undef CALL
 }
 
 void launch_restrict(const DevParams &P, GridDims fine, GridDims coarse, const double *qf,
                      double *qc, hipStream_t s) {
-#define CALL(NU) k_restrict<NU><<<nblk(coarse.size(), 256), 256, 0, s>>>(P, fine, coarse, qf, qc)
+#define CALL(NU) k_restrict<NU><<<nblk(coarse.size(), 4), 256, 0, s>>>(P, fine, coarse, qf, qc)
   MSM_NU_SWITCH(P.nu, CALL)
 #undef CALL
 }
@@ -918,7 +1092,7 @@ void launch_gather(const DevParams &P, GridDims g, const double4 *spos, const in
 
 void launch_finalize(const DevParams &P, const double *epair, int npair, const double *egrid,
                      int ngrid, const double *scal, double *out, hipStream_t s) {
-  k_finalize<<<1, 64, 0, s>>>(P, epair, npair, egrid, ngrid, scal, out);
+  k_finalize<<<1, 256, 0, s>>>(P, epair, npair, egrid, ngrid, scal, out);
 }
 
 }  // namespace msm

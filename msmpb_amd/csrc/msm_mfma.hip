@@ -1,0 +1,347 @@
+// msm_mfma.hip -- level stencil correlations as implicit GEMMs on the matrix
+// cores of gfx950, and the prolongation pass that combines the levels.
+//
+// For level l < L the stencil pass of grid2grid (code/forcefield0.c:898-931
+// of hkayabilisim/msmpb) is
+//     c_l[x][y][z] = sum_{kx,ky,kz} K[kx][ky][kz] q[x+kx-cx][y+ky-cy][z+kz-cz]
+// (periodic, c = s/2).  Along z it is a banded Toeplitz product: for one
+// output plane x, input plane xi = x+kx-cx and row shift ky,
+//     C(y, z) += sum_k A(y, k) B(k, z),   A(y, k) = q[xi][y+ky-cy][z0-cz+k],
+//                                         B(k, z) = K[kx][ky][k-z] (0 <= k-z < s_z).
+// One wavefront owns a 16 x 16 (y, z) output tile of one plane and a range
+// of kx; it stages each input slab (16+s_y-1 rows x K columns) in LDS and
+// issues one 16x16xK MFMA per (ky, K-chunk).  B depends only on
+// (kx, ky, chunk), so it is prepared once per box on the host in the exact
+// per-lane operand layout and streamed from L2.
+//   FP64:  v_mfma_f64_16x16x4_f64, K chunks of 4.
+//   mixed: v_mfma_f32_16x16x32_bf16, K chunks of 32, q and K split into
+//          bf16 hi + lo, three products (hi*hi + hi*lo + lo*hi) accumulated
+//          in FP32.
+// Every level's tiles go into a single launch (the stencil passes of
+// different levels are independent); each kx-group writes its own partial
+// grid, and k_prolong_sum adds the partials and the prolongated coarser
+// potential (forcefield1.c:530-567) level by level, deterministically.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+#include "msm_kernels.h"
+
+namespace msm {
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ int wrapm(int v, int m) {
+  int r = v % m;
+  return r < 0 ? r + m : r;
+}
+
+// float -> bf16 bits, round to nearest even (inputs are finite)
+__host__ __device__ inline uint16_t bf16_bits(float f) {
+  uint32_t u;
+  std::memcpy(&u, &f, 4);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+__host__ __device__ inline float bf16_float(uint16_t h) {
+  uint32_t u = (uint32_t)h << 16;
+  float f;
+  std::memcpy(&f, &u, 4);
+  return f;
+}
+
+template <int PREC>
+struct Cfg;
+template <>
+struct Cfg<MSM_G2G_FP64> {
+  static constexpr int KC = 4;     // K per MFMA
+  static constexpr int KPL = 1;    // K elements per lane
+  static constexpr int PAD = 1;    // LDS row padding (elements)
+};
+template <>
+struct Cfg<MSM_G2G_BF16X3> {
+  static constexpr int KC = 32;
+  static constexpr int KPL = 8;
+  static constexpr int PAD = 8;
+};
+
+// B operands of one (kx, ky) row: NCH chunks (x2 for the bf16 hi/lo pair).
+template <int PREC, int NCH>
+struct BRow;
+template <int NCH>
+struct BRow<MSM_G2G_FP64, NCH> {
+  double b[NCH];
+  __device__ void load(const void *base, long f0, int lane) {
+    const double *p = reinterpret_cast<const double *>(base) + f0 * 64 + lane;
+#pragma unroll
+    for (int c = 0; c < NCH; ++c) b[c] = p[c * 64];
+  }
+};
+template <int NCH>
+struct BRow<MSM_G2G_BF16X3, NCH> {
+  u32x4 hi[NCH], lo[NCH];
+  __device__ void load(const void *base, long f0, int lane) {
+    const u32x4 *p = reinterpret_cast<const u32x4 *>(base) + f0 * 128 + lane;
+#pragma unroll
+    for (int c = 0; c < NCH; ++c) {
+      hi[c] = p[c * 128];
+      lo[c] = p[c * 128 + 64];
+    }
+  }
+};
+
+template <int PREC, int NCH>
+__global__ __launch_bounds__(64) void k_g2g_tiles(G2GArgs args) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  constexpr int KC = Cfg<PREC>::KC, KPL = Cfg<PREC>::KPL, PAD = Cfg<PREC>::PAD;
+  long task = blockIdx.x;
+  int li = 0;
+  while (li + 1 < args.nlev && task >= args.lv[li + 1].task0) ++li;
+  const G2GLevel &L = args.lv[li];
+  long t = task - L.task0;
+  int grp = t % L.G;
+  t /= L.G;
+  int zt = t % L.tz;
+  t /= L.tz;
+  int yt = t % L.ty;
+  int x = (int)(t / L.ty);
+  const GridDims g = L.g, s = L.s;
+  int per = (s.x + L.G - 1) / L.G;
+  int kx0 = grp * per, kx1 = min(s.x, kx0 + per);
+  int y0 = yt * 16, z0 = zt * 16;
+  const int R = 16 + s.y - 1, W = NCH * KC, Wp = W + PAD;
+  const int lane = threadIdx.x;
+  const int row = lane & 15, kg = lane >> 4;
+  f32x4 acc32 = {0.f, 0.f, 0.f, 0.f};
+  f64x4 acc64 = {0.0, 0.0, 0.0, 0.0};
+  for (int kx = kx0; kx < kx1; ++kx) {
+    int xi = wrapm(x + kx - s.x / 2, g.x);
+    const double *plane = L.q + (long)xi * g.y * g.z;
+    BRow<PREC, NCH> bnext;
+    bnext.load(L.bfrag, (long)(kx * s.y) * NCH, lane);  // row ky = 0, in flight during the slab load
+    __syncthreads();  // previous slab fully consumed
+    // slab: lane = column (periodic z index computed once), rows uniform
+    for (int cb = 0; cb < W; cb += 64) {
+      int cc = cb + lane;
+      if (cc < W) {
+        int zz = wrapm(z0 - s.z / 2 + cc, g.z);
+        int yy = wrapm(y0 - s.y / 2, g.y);
+        for (int r = 0; r < R; ++r) {
+          double v = plane[(long)yy * g.z + zz];
+          if (++yy == g.y) yy = 0;
+          if constexpr (PREC == MSM_G2G_FP64) {
+            reinterpret_cast<double *>(lds_raw)[r * Wp + cc] = v;
+          } else {
+            float f = (float)v;
+            uint16_t hi = bf16_bits(f);
+            uint16_t lo = bf16_bits(f - bf16_float(hi));
+            reinterpret_cast<uint16_t *>(lds_raw)[r * Wp + cc] = hi;
+            reinterpret_cast<uint16_t *>(lds_raw)[R * Wp + r * Wp + cc] = lo;
+          }
+        }
+      }
+    }
+    __syncthreads();
+    for (int ky = 0; ky < s.y; ++ky) {
+      BRow<PREC, NCH> bcur = bnext;
+      if (ky + 1 < s.y) bnext.load(L.bfrag, (long)(kx * s.y + ky + 1) * NCH, lane);
+#pragma unroll
+      for (int ch = 0; ch < NCH; ++ch) {
+        if constexpr (PREC == MSM_G2G_FP64) {
+          const double *slab = reinterpret_cast<const double *>(lds_raw);
+          double av = slab[(row + ky) * Wp + ch * KC + kg];
+          acc64 = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bcur.b[ch], acc64, 0, 0, 0);
+        } else {
+          const uint16_t *sh = reinterpret_cast<const uint16_t *>(lds_raw);
+          int off = (row + ky) * Wp + ch * KC + kg * KPL;
+          bf16x8 ahi = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u32x4 *>(sh + off));
+          bf16x8 alo = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u32x4 *>(sh + R * Wp + off));
+          bf16x8 bhi = __builtin_bit_cast(bf16x8, bcur.hi[ch]);
+          bf16x8 blo = __builtin_bit_cast(bf16x8, bcur.lo[ch]);
+          acc32 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alo, bhi, acc32, 0, 0, 0);
+          acc32 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi, blo, acc32, 0, 0, 0);
+          acc32 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi, bhi, acc32, 0, 0, 0);
+        }
+      }
+    }
+  }
+  // C layout, column lane%16 in both cases: the f32 16x16x32 result holds
+  // rows 4*(lane/16) + i in register i; the f64 16x16x4 result holds rows
+  // 4*i + lane/16 (measured on gfx950 with one-hot operands).
+  double *out = L.part + (long)grp * g.size() + (long)x * g.y * g.z;
+  int z = z0 + row;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    int y = y0 + ((PREC == MSM_G2G_FP64) ? 4 * i + kg : 4 * kg + i);
+    if (y < g.y && z < g.z) {
+      double v = (PREC == MSM_G2G_FP64) ? acc64[i] : (double)acc32[i];
+      out[(long)y * g.z + z] = v;
+    }
+  }
+}
+
+// e^l = sum_g part[g] + P e^{l+1}.  Fine index v receives J_n from coarse m
+// with 2m + n == v (mod 2 M_c): n has the parity of v, m = (v - n)/2 mod M_c.
+// Per-axis contributors of prolongation: fine index v receives J_n e_m from
+// coarse m with 2m + n == v (mod 2 M_c), i.e. n of the parity of v and
+// m = (v - n)/2 mod M_c; at most NU/2 + 1 of them.
+template <int NU>
+__device__ __forceinline__ int prolong_axis(const double *J, int v, int Mc, int *m, double *w) {
+  const int H = NU / 2;
+  int k = 0;
+#pragma unroll
+  for (int n = -H; n <= H; ++n)
+    if (((v - n) & 1) == 0) {
+      m[k] = wrapm((v - n) / 2, Mc);
+      w[k] = J[n + H];
+      ++k;
+    }
+  return k;
+}
+
+template <int NU>
+__global__ __launch_bounds__(256) void k_prolong_sum(DevParams P, GridDims g, GridDims gc, int G,
+                                                     const double *__restrict__ part,
+                                                     const double *__restrict__ ec,
+                                                     double *__restrict__ el) {
+  long id = (long)blockIdx.x * 256 + threadIdx.x;
+  long n = g.size();
+  if (id >= n) return;
+  int z = id % g.z, y = (id / g.z) % g.y, x = id / ((long)g.z * g.y);
+  constexpr int K = NU / 2 + 1;
+  int mx[K], my[K], mz[K];
+  double wx[K], wy[K], wz[K];
+  int nx = prolong_axis<NU>(P.J, x, gc.x, mx, wx);
+  int ny = prolong_axis<NU>(P.J, y, gc.y, my, wy);
+  int nz = prolong_axis<NU>(P.J, z, gc.z, mz, wz);
+  double p = 0.0;
+  for (int a = 0; a < nx; ++a) {
+    double sy = 0.0;
+    for (int b = 0; b < ny; ++b) {
+      const double *rowp = ec + ((long)mx[a] * gc.y + my[b]) * gc.z;
+      double sz = 0.0;
+#pragma unroll
+      for (int c = 0; c < K; ++c)
+        if (c < nz) sz += wz[c] * rowp[mz[c]];
+      sy += wy[b] * sz;
+    }
+    p += wx[a] * sy;
+  }
+  double c = 0.0;
+  for (int k = 0; k < G; ++k) c += part[(long)k * n + id];
+  el[id] = p + c;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- host side
+int g2g_chunk_k(int prec) { return prec == MSM_G2G_FP64 ? Cfg<MSM_G2G_FP64>::KC : Cfg<MSM_G2G_BF16X3>::KC; }
+
+void plan_g2g_level(int prec, GridDims g, GridDims s, int target_waves, G2GLevel &L) {
+  int KC = g2g_chunk_k(prec);
+  L.g = g;
+  L.s = s;
+  L.nch = (16 + s.z - 1 + KC - 1) / KC;
+  L.ty = (g.y + 15) / 16;
+  L.tz = (g.z + 15) / 16;
+  long tiles = (long)g.x * L.ty * L.tz;
+  long G = (target_waves + tiles - 1) / tiles;
+  if (G < 1) G = 1;
+  if (G > s.x) G = s.x;
+  int per = (s.x + (int)G - 1) / (int)G;
+  L.G = (s.x + per - 1) / per;  // no empty groups
+}
+
+size_t g2g_bfrag_elems(int prec, const G2GLevel &L) {
+  size_t nfrag = (size_t)L.s.x * L.s.y * L.nch;
+  return prec == MSM_G2G_FP64 ? nfrag * 64 : nfrag * 2 * 64 * 8;  // doubles / bf16 values
+}
+
+size_t g2g_bfrag_bytes(int prec, const G2GLevel &L) {
+  return g2g_bfrag_elems(prec, L) * (prec == MSM_G2G_FP64 ? 8 : 2);
+}
+
+// B(k, j) = K[kx][ky][k - j] for 0 <= k - j < s_z, in per-lane operand order:
+// lane holds column j = lane % 16 and K rows (lane / 16) * KPL + e of the chunk.
+void g2g_bfrag_fill(int prec, const G2GLevel &L, const double *khat, void *host_out) {
+  const int KC = g2g_chunk_k(prec), KPL = prec == MSM_G2G_FP64 ? 1 : 8;
+  const GridDims s = L.s;
+  auto kval = [&](int kx, int ky, int k, int j) {
+    int kz = k - j;
+    return (kz >= 0 && kz < s.z) ? khat[((long)kx * s.y + ky) * s.z + kz] : 0.0;
+  };
+  for (int kx = 0; kx < s.x; ++kx)
+    for (int ky = 0; ky < s.y; ++ky)
+      for (int ch = 0; ch < L.nch; ++ch) {
+        long f = ((long)kx * s.y + ky) * L.nch + ch;
+        for (int lane = 0; lane < 64; ++lane) {
+          int j = lane % 16;
+          for (int e = 0; e < KPL; ++e) {
+            int k = ch * KC + (lane / 16) * KPL + e;
+            double v = kval(kx, ky, k, j);
+            if (prec == MSM_G2G_FP64) {
+              static_cast<double *>(host_out)[f * 64 + lane] = v;
+            } else {
+              uint16_t *o = static_cast<uint16_t *>(host_out);
+              float fv = (float)v;
+              uint16_t hi = bf16_bits(fv), lo = bf16_bits(fv - bf16_float(hi));
+              o[(f * 128 + lane) * 8 + e] = hi;
+              o[(f * 128 + 64 + lane) * 8 + e] = lo;
+            }
+          }
+        }
+      }
+}
+
+size_t g2g_lds_bytes(int prec, const G2GLevel &L) {
+  int R = 16 + L.s.y - 1;
+  if (prec == MSM_G2G_FP64) return sizeof(double) * R * (L.nch * 4 + Cfg<MSM_G2G_FP64>::PAD);
+  return 2 * sizeof(uint16_t) * R * (L.nch * 32 + Cfg<MSM_G2G_BF16X3>::PAD);
+}
+
+template <int PREC, int NCH>
+static void g2g_launch(const G2GArgs &args, size_t lds, hipStream_t s) {
+  k_g2g_tiles<PREC, NCH><<<(unsigned)args.ntasks, 64, lds, s>>>(args);
+}
+
+int g2g_max_nch(int prec) { return prec == MSM_G2G_FP64 ? 16 : 4; }
+
+void launch_g2g_tiles(int prec, G2GArgs args, size_t lds, hipStream_t s) {
+  if (args.ntasks <= 0) return;
+  // all levels of one launch share the chunk count (the plan equalises it)
+  int nch = args.lv[0].nch;
+  if (prec == MSM_G2G_FP64) {
+    switch (nch) {
+#define C(N) case N: g2g_launch<MSM_G2G_FP64, N>(args, lds, s); break;
+      C(1) C(2) C(3) C(4) C(5) C(6) C(7) C(8) C(9) C(10) C(11) C(12) C(13) C(14) C(15) C(16)
+#undef C
+    }
+  } else {
+    switch (nch) {
+      case 1: g2g_launch<MSM_G2G_BF16X3, 1>(args, lds, s); break;
+      case 2: g2g_launch<MSM_G2G_BF16X3, 2>(args, lds, s); break;
+      case 3: g2g_launch<MSM_G2G_BF16X3, 3>(args, lds, s); break;
+      case 4: g2g_launch<MSM_G2G_BF16X3, 4>(args, lds, s); break;
+    }
+  }
+}
+
+void launch_prolong_sum(const DevParams &P, GridDims g, GridDims gc, int G, const double *part,
+                        const double *ec, double *el, hipStream_t s) {
+  unsigned nb = (unsigned)((g.size() + 255) / 256);
+  switch (P.nu) {
+    case 4: k_prolong_sum<4><<<nb, 256, 0, s>>>(P, g, gc, G, part, ec, el); break;
+    case 6: k_prolong_sum<6><<<nb, 256, 0, s>>>(P, g, gc, G, part, ec, el); break;
+    case 8: k_prolong_sum<8><<<nb, 256, 0, s>>>(P, g, gc, G, part, ec, el); break;
+    default: k_prolong_sum<10><<<nb, 256, 0, s>>>(P, g, gc, G, part, ec, el); break;
+  }
+}
+
+}  // namespace msm

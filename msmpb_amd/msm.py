@@ -83,7 +83,10 @@ class MSM:
 
     def close(self):
         if getattr(self, "_ff", None):
-            _lib.lib().FF_delete(self._ff)
+            try:
+                _lib.lib().FF_delete(self._ff)
+            except Exception:  # interpreter shutdown
+                pass
             self._ff = None
 
     def __del__(self):

@@ -137,3 +137,19 @@ def test_mixed_precision_accuracy(oracle_mod, torch_cuda):
         Eo, Fo, _, _ = o.energy(s.pos)
         assert rms_rel_force(F, Fo) < 1e-5, case
         assert rel_err(E, Eo) < 1e-5, case
+
+
+def test_water96k_fp64_and_mixed(oracle_mod, torch_cuda):
+    """The benchmark workload (96,000 atoms, 6,859 pair-search cells): FP64 to
+    the oracle at 1e-10 and the mixed path's rms relative force error."""
+    print("[parity] water96k", flush=True)
+    s = inputs.cfg3_water96k(6)
+    m, o = _pair(s, oracle_mod)
+    E, F = m.energy(s.pos)
+    Eo, Fo, _, _ = o.energy(s.pos, want_phi=False)
+    assert rel_err(E, Eo) < TOL
+    assert force_err(F, Fo) < TOL
+    m.set_precision("mixed")
+    Em, Fm = m.energy(s.pos)
+    assert rms_rel_force(Fm, Fo) < 1e-5
+    assert rel_err(Em, Eo) < 1e-6
