@@ -14,6 +14,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -82,7 +83,7 @@ struct Engine {
   msm::DevParams P{};
   msm::GridDims gd[msm::kMaxLevel + 1]{}, sd[msm::kMaxLevel + 1]{};
   int device = 0;
-  hipStream_t stream = nullptr;
+  hipStream_t stream = nullptr;    // the library's streams of this device (shared, not owned)
   hipStream_t stream2 = nullptr;   // short-range pairs run here, beside the grid chain
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   int nsten = 0;
@@ -148,8 +149,6 @@ struct Engine {
     if (ev_fork) hipEventDestroy(ev_fork);
     if (ev_join) hipEventDestroy(ev_join);
     ev_fork = ev_join = nullptr;
-    if (stream2) hipStreamDestroy(stream2);
-    if (stream) hipStreamDestroy(stream);
     stream = stream2 = nullptr;
   }
 };
@@ -214,6 +213,10 @@ void pair_geometry(const msm::Plan &p, msm::DevParams &P, std::vector<int4> &ste
         }
         if (m2 < a0 * a0) sten.push_back(make_int4(dx, dy, dz, 0));
       }
+  P.self_wrap = 0;
+  for (const int4 &d : sten)
+    if ((d.x || d.y || d.z) && d.x % P.nc[0] == 0 && d.y % P.nc[1] == 0 && d.z % P.nc[2] == 0)
+      P.self_wrap = 1;
 }
 
 void fill_params(const msm::Plan &p, int N, msm::DevParams &P) {
@@ -288,6 +291,31 @@ int upload_box(Engine &E) {
 }
 
 // Tiles, kx groups and partial buffers of the MFMA level-stencil launch.
+// Two streams per device, created once and shared by every FF of the
+// process: the evaluation chain and the concurrent pair kernel.  HIP maps
+// streams onto a few hardware queues; creating two per FF would make later
+// FFs share a queue between the pair stream and the chain, which serialises
+// them.  FFs on one device are thereby ordered with each other.
+int device_streams(int dev, hipStream_t &main, hipStream_t &side) {
+  static std::mutex mu;
+  static std::vector<std::pair<hipStream_t, hipStream_t>> tab;
+  std::lock_guard<std::mutex> lk(mu);
+  if ((int)tab.size() <= dev) tab.resize(dev + 1, {nullptr, nullptr});
+  auto &t = tab[dev];
+  if (!t.first) {
+    hipStream_t a = nullptr, b = nullptr;
+    HIP_TRY(hipStreamCreateWithFlags(&a, hipStreamNonBlocking));
+    if (hipStreamCreateWithFlags(&b, hipStreamNonBlocking) != hipSuccess) {
+      hipStreamDestroy(a);
+      return fail(MSM_ERR_HIP, "hipStreamCreateWithFlags failed");
+    }
+    t = {a, b};
+  }
+  main = t.first;
+  side = t.second;
+  return MSM_OK;
+}
+
 int plan_levels(Engine &E) {
   const msm::Plan &p = E.plan;
   static const int target[2] = {4096, 2048};
@@ -325,8 +353,7 @@ int create_engine(FFX *x, const msm::Plan &plan) {
   E->plan = plan;
   x->eng = E;
   HIP_TRY(hipGetDevice(&E->device));
-  HIP_TRY(hipStreamCreateWithFlags(&E->stream, hipStreamNonBlocking));
-  HIP_TRY(hipStreamCreateWithFlags(&E->stream2, hipStreamNonBlocking));
+  TRY(device_streams(E->device, E->stream, E->stream2));
   HIP_TRY(hipEventCreateWithFlags(&E->ev_fork, hipEventDisableTiming));
   HIP_TRY(hipEventCreateWithFlags(&E->ev_join, hipEventDisableTiming));
   const msm::Plan &p = E->plan;

@@ -15,6 +15,8 @@ struct DevParams {
   double ibox[3];      // 1 / edge lengths (diagonal of Ai)
   int nc[3];           // pair-search cells per axis
   int ncell;
+  int self_wrap;       // some stencil offset != 0 wraps onto the home cell (narrow
+                       // cell grid): periodic self images must be excluded by index
   double cw[3];        // cell widths
   double a0, a02, inv_a0, inv_a02, two_inv_a03;
   int nu;

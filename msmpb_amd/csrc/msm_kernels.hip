@@ -265,13 +265,99 @@ struct PairStage<false> {
   static constexpr int CAP = 256;
 };
 
+// Interactions of one home atom (position ai, sorted index ii) with the
+// staged atoms e = bs, bs + S, ... < npad (npad a multiple of 16 S; the tail
+// holds far-away padding).  Self pairs have r2 == 0; with CHK, periodic self
+// images (narrow cell grids only) are excluded by index as well.
+template <bool CHK, int NU>
+__device__ __forceinline__ void pair_run_fp32(const DevParams &P, const float4 *jbuf,
+                                              const int *jidx, int npad, int bs, int S, float4 ai,
+                                              int ii, double &acc, double &fx, double &fy,
+                                              double &fz) {
+  const float a02 = (float)P.a02, ia02 = (float)P.inv_a02, ia0 = (float)P.inv_a0,
+              t3 = (float)P.two_inv_a03;
+  float tau[NU], dtau[NU - 1];  // tau(s) and tau'(s) coefficients
+#pragma unroll
+  for (int k = 0; k < NU; ++k) tau[k] = (float)P.tau[k];
+#pragma unroll
+  for (int k = 0; k + 1 < NU; ++k) dtau[k] = (float)((k + 1) * P.tau[k + 1]);
+  for (int e0 = bs; e0 < npad; e0 += 16 * S) {
+    // FP32 partial sums over 16 neighbours, then added in FP64
+    float sa = 0.f, sx = 0.f, sy = 0.f, sz = 0.f;
+#pragma unroll 8
+    for (int u = 0; u < 16; ++u) {
+      // branch-free: some lane of the wavefront nearly always accepts
+      const int e = e0 + u * S;
+      const float4 aj = jbuf[e];
+      const float dx = aj.x - ai.x, dy = aj.y - ai.y, dz = aj.z - ai.z;
+      const float r2 = dx * dx + dy * dy + dz * dz;
+      bool ok = r2 < a02 && r2 > 0.f;
+      if constexpr (CHK) ok = ok && jidx[e] != ii;
+      const float r2s = ok ? r2 : 1.0f;
+      const float qj = ok ? aj.w : 0.0f;
+      const float sv = r2s * ia02 - 1.0f;
+      float ts = tau[NU - 1], t1 = dtau[NU - 2];
+#pragma unroll
+      for (int k = NU - 2; k >= 0; --k) ts = tau[k] + sv * ts;
+#pragma unroll
+      for (int k = NU - 3; k >= 0; --k) t1 = dtau[k] + sv * t1;
+      const float rinv = __builtin_amdgcn_rsqf(r2s);  // r2 >= 1e-2 for physical input
+      sa += qj * (rinv - ts * ia0);
+      const float qc = qj * (-rinv * rinv * rinv - t1 * t3);
+      sx += qc * dx, sy += qc * dy, sz += qc * dz;
+    }
+    acc += (double)sa;
+    fx += (double)sx, fy += (double)sy, fz += (double)sz;
+  }
+}
+
+// 1/sqrt(r2) in FP64: hardware estimate refined by two Newton steps.
+__device__ __forceinline__ double rsqrt_f64(double r2) {
+  double y = __builtin_amdgcn_rsq(r2);
+  const double h = 0.5 * r2;
+  y = y * fma(-h * y, y, 1.5);
+  y = y * fma(-h * y, y, 1.5);
+  return y;
+}
+
+template <bool CHK, int NU>
+__device__ __forceinline__ void pair_run_fp64(const DevParams &P, const double4 *jbuf,
+                                              const int *jidx, int npad, int bs, int S, double4 ai,
+                                              int ii, double &acc, double &fx, double &fy,
+                                              double &fz) {
+  const double a02 = P.a02;
+  double dtau[NU - 1];
+#pragma unroll
+  for (int k = 0; k + 1 < NU; ++k) dtau[k] = (k + 1) * P.tau[k + 1];
+  for (int e = bs; e < npad; e += S) {
+    const double4 aj = jbuf[e];
+    const double dx = aj.x - ai.x, dy = aj.y - ai.y, dz = aj.z - ai.z;
+    const double r2 = dx * dx + dy * dy + dz * dz;
+    bool ok = r2 < a02 && r2 > 0.0;
+    if constexpr (CHK) ok = ok && jidx[e] != ii;
+    if (ok) {
+      const double sv = r2 * P.inv_a02 - 1.0;
+      double ts = P.tau[NU - 1], t1 = dtau[NU - 2];
+#pragma unroll
+      for (int k = NU - 2; k >= 0; --k) ts = P.tau[k] + sv * ts;
+#pragma unroll
+      for (int k = NU - 3; k >= 0; --k) t1 = dtau[k] + sv * t1;
+      const double rinv = rsqrt_f64(r2);
+      acc += aj.w * (rinv - ts * P.inv_a0);
+      const double qc = aj.w * (-rinv * rinv * rinv - t1 * P.two_inv_a03);
+      fx += qc * dx, fy += qc * dy, fz += qc * dz;
+    }
+  }
+}
+
 template <bool MIXED, int NU>
 __global__ __launch_bounds__(64) void k_pair_cell(
     DevParams P, const double4 *__restrict__ spos, const float4 *__restrict__ lpos,
     const int *__restrict__ sidx, const int *__restrict__ start, const int4 *__restrict__ sten,
     int nsten, double *__restrict__ F, double *__restrict__ phi, double *__restrict__ epart) {
   using T = typename PairStage<MIXED>::T;
-  constexpr int CAP = PairStage<MIXED>::CAP;
+  using S1 = decltype(T().x);
+  constexpr int CAP = PairStage<MIXED>::CAP;  // multiple of 64
   __shared__ T jbuf[CAP];
   __shared__ int jidx[CAP];
   __shared__ int seg_off[65], seg_beg[64];
@@ -279,6 +365,29 @@ __global__ __launch_bounds__(64) void k_pair_cell(
   const int c = blockIdx.x, lane = threadIdx.x;
   const int b0 = start[c], n = start[c + 1] - b0;
   const int cz = c % P.nc[2], cy = (c / P.nc[2]) % P.nc[1], cx = c / (P.nc[1] * P.nc[2]);
+  // bounding box of the home atoms: neighbour atoms farther than the cutoff
+  // from it interact with none of them and are not staged
+  S1 lo[3] = {(S1)1e30, (S1)1e30, (S1)1e30}, hi[3] = {(S1)-1e30, (S1)-1e30, (S1)-1e30};
+  for (int k = lane; k < n; k += 64) {
+    T v;
+    if constexpr (MIXED) v = lpos[b0 + k];
+    else v = spos[b0 + k];
+    lo[0] = fmin(lo[0], v.x), lo[1] = fmin(lo[1], v.y), lo[2] = fmin(lo[2], v.z);
+    hi[0] = fmax(hi[0], v.x), hi[1] = fmax(hi[1], v.y), hi[2] = fmax(hi[2], v.z);
+  }
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1)
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      lo[d] = fmin(lo[d], __shfl_xor(lo[d], off, 64));
+      hi[d] = fmax(hi[d], __shfl_xor(hi[d], off, 64));
+    }
+  // slightly enlarged cutoff: the box test must never drop a pair that the
+  // interaction test (different rounding) would keep
+  const S1 cut2 = (S1)P.a02 * (MIXED ? (S1)1.0001 : (S1)(1.0 + 1e-12));
+  const unsigned long long lanes_below = (1ull << lane) - 1ull;
+  // padding atoms and idle home slots sit far apart and far from the box
+  const S1 far = (S1)1e18;
   double esum = 0.0;
   for (int i0 = 0; i0 < n;) {
     const int rem = n - i0;
@@ -287,8 +396,8 @@ __global__ __launch_bounds__(64) void k_pair_cell(
     const bool act = a < rem;
     const int ii = b0 + i0 + a;
     T ai;
-    if constexpr (MIXED) ai = act ? lpos[ii] : make_float4(0.f, 0.f, 0.f, 0.f);
-    else ai = act ? spos[ii] : make_double4(0.0, 0.0, 0.0, 0.0);
+    if constexpr (MIXED) ai = act ? lpos[ii] : make_float4(-far, -far, -far, 0.f);
+    else ai = act ? spos[ii] : make_double4(-far, -far, -far, 0.0);
     double acc = 0.0, fx = 0.0, fy = 0.0, fz = 0.0;
     for (int o0 = 0; o0 < nsten;) {
       // ---- stage: lane k takes stencil offset o0 + k
@@ -325,60 +434,64 @@ __global__ __launch_bounds__(64) void k_pair_cell(
       if (lane == 0) seg_off[m] = total;
       __syncthreads();
       for (int p0 = 0; p0 < total; p0 += CAP) {
-        const int np = min(CAP, total - p0);
-        for (int e = lane; e < np; e += 64) {
-          int g = p0 + e, lo = 0, hi = m;  // seg_off[lo] <= g < seg_off[hi]
-          while (hi - lo > 1) {
-            int mid = (lo + hi) >> 1;
-            if (seg_off[mid] <= g) lo = mid; else hi = mid;
+        const int ng = min(CAP, total - p0);
+        int np = 0;  // staged atoms that pass the bounding-box test
+        // the chunk's global loads are issued four at a time before any of
+        // them is used; the segment of element e only moves forward as e grows
+        constexpr int IT = CAP / 64, LB = 4;
+        int sl = 0;
+#pragma unroll
+        for (int it0 = 0; it0 < IT; it0 += LB) {
+          T v[LB];
+          int jj[LB];
+#pragma unroll
+          for (int u = 0; u < LB; ++u) {
+            const int e = 64 * (it0 + u) + lane;
+            jj[u] = -1;
+            if (e < ng) {
+              const int g = p0 + e;
+              while (sl + 1 < m && seg_off[sl + 1] <= g) ++sl;
+              jj[u] = seg_beg[sl] + (g - seg_off[sl]);
+              if constexpr (MIXED) v[u] = lpos[jj[u]];
+              else v[u] = spos[jj[u]];
+              const T s2 = seg_shift[sl];
+              v[u].x += s2.x, v[u].y += s2.y, v[u].z += s2.z;
+            }
           }
-          int j = seg_beg[lo] + (g - seg_off[lo]);
-          T v, s2 = seg_shift[lo];
-          if constexpr (MIXED) v = lpos[j];
-          else v = spos[j];
-          v.x += s2.x, v.y += s2.y, v.z += s2.z;
-          jbuf[e] = v;
-          jidx[e] = j;
+#pragma unroll
+          for (int u = 0; u < LB; ++u) {
+            bool keep = false;
+            if (jj[u] >= 0) {
+              S1 bx = fmax(fmax(lo[0] - v[u].x, v[u].x - hi[0]), (S1)0);
+              S1 by = fmax(fmax(lo[1] - v[u].y, v[u].y - hi[1]), (S1)0);
+              S1 bz = fmax(fmax(lo[2] - v[u].z, v[u].z - hi[2]), (S1)0);
+              keep = bx * bx + by * by + bz * bz < cut2;
+            }
+            unsigned long long bm = __ballot(keep);
+            if (keep) {
+              int slot = np + __popcll(bm & lanes_below);
+              jbuf[slot] = v[u];
+              jidx[slot] = jj[u];
+            }
+            np += __popcll(bm);
+          }
+        }
+        // pad to a multiple of 64 (np <= CAP, CAP a multiple of 64)
+        const int npad = (np + 63) & ~63;
+        if (np + lane < npad) {
+          T v;
+          v.x = far, v.y = far, v.z = far, v.w = 0;
+          jbuf[np + lane] = v;
+          jidx[np + lane] = -1;
         }
         __syncthreads();
         // ---- interactions of the staged atoms with this lane group's atom
-        for (int e = bs; e < np; e += S) {
-          T aj = jbuf[e];
-          if constexpr (MIXED) {
-            float dx = aj.x - ai.x, dy = aj.y - ai.y, dz = aj.z - ai.z;
-            float r2 = dx * dx + dy * dy + dz * dz;
-            if (r2 < (float)P.a02 && jidx[e] != ii && act) {
-              float s = r2 * (float)P.inv_a02 - 1.0f;
-              float ts = (float)P.tau[NU - 1], t1 = 0.0f;
-#pragma unroll
-              for (int k = NU - 2; k >= 0; --k) {
-                t1 = ts + s * t1;
-                ts = (float)P.tau[k] + s * ts;
-              }
-              float rinv = rsqrtf(r2);
-              float en = rinv - ts * (float)P.inv_a0;
-              float qc = aj.w * (-rinv * rinv * rinv - t1 * (float)P.two_inv_a03);
-              acc += (double)(aj.w * en);
-              fx += (double)(qc * dx), fy += (double)(qc * dy), fz += (double)(qc * dz);
-            }
-          } else {
-            double dx = aj.x - ai.x, dy = aj.y - ai.y, dz = aj.z - ai.z;
-            double r2 = dx * dx + dy * dy + dz * dz;
-            if (r2 < P.a02 && jidx[e] != ii && act) {
-              double s = r2 * P.inv_a02 - 1.0;
-              double ts = P.tau[NU - 1], t1 = 0.0;
-#pragma unroll
-              for (int k = NU - 2; k >= 0; --k) {
-                t1 = ts + s * t1;
-                ts = P.tau[k] + s * ts;
-              }
-              double rinv = 1.0 / sqrt(r2);
-              double en = rinv - ts * P.inv_a0;
-              double qc = aj.w * (-rinv * rinv * rinv - t1 * P.two_inv_a03);
-              acc += aj.w * en;
-              fx += qc * dx, fy += qc * dy, fz += qc * dz;
-            }
-          }
+        if constexpr (MIXED) {
+          if (P.self_wrap) pair_run_fp32<true, NU>(P, jbuf, jidx, npad, bs, S, ai, ii, acc, fx, fy, fz);
+          else pair_run_fp32<false, NU>(P, jbuf, jidx, npad, bs, S, ai, ii, acc, fx, fy, fz);
+        } else {
+          if (P.self_wrap) pair_run_fp64<true, NU>(P, jbuf, jidx, npad, bs, S, ai, ii, acc, fx, fy, fz);
+          else pair_run_fp64<false, NU>(P, jbuf, jidx, npad, bs, S, ai, ii, acc, fx, fy, fz);
         }
         __syncthreads();
       }
@@ -443,48 +556,72 @@ __global__ __launch_bounds__(kPairBS) void k_pair_count(
 
 // ---------------------------------------------------------------- spread
 // anterpolate (forcefield1.c:324-368): q^1_{m-i+NU/2} += q Q(i+t) per axis,
-// restated as a gather by grid bricks.  One workgroup owns a brick of the
-// level-1 grid; it lists the pair-search cell columns whose atoms can reach
-// the brick (atoms are sorted by cell, so each column is one or two
-// contiguous index ranges), accumulates every reaching atom's clipped
-// NU^3 stencil in LDS, and stores the brick once.  No global atomics.
-constexpr int kBX = 8, kBY = 8, kBZ = 4;
-constexpr int kMaxRanges = 1024;
+// restated per grid brick as a matrix product on the matrix cores.  A brick
+// is 16 x 4 x 4 points; with wx, wy, wz an atom's B-spline weights folded
+// onto the brick's x, y and z indices,
+//     Q[x][(y, z)] = sum_atoms wx[x] * (q wy[y] wz[z]),
+// a 16 x 16 output tile accumulated by v_mfma_f64_16x16x4_f64 over the atoms
+// four at a time.  The workgroup lists the pair-search cell columns whose
+// atoms can reach the brick (atoms are sorted by cell, so a column is one or
+// two contiguous index ranges), stages the folded weights of the reaching
+// atoms in LDS 256 candidates at a time, and its four wavefronts split the
+// atoms of each round; their tiles are summed in a fixed order and every grid
+// point is stored once.  No atomics: the result is deterministic.
+constexpr int kSX = 16, kSY = 4, kSZ = 4;
+constexpr int kSRanges = 512;       // index ranges per batch of cell columns
+constexpr int kWXs = 17, kWYs = 5;  // padded record strides (doubles)
 
-__device__ __forceinline__ void cell_span(double g0, int B, int half, int M, int nc, int &lo,
+typedef double sp_f64x4 __attribute__((ext_vector_type(4)));
+
+// Cells holding atoms whose grid index m lies in [g0 - half, g0 + B + half - 2],
+// i.e. fractional coordinate in [(g0 - half) / M, (g0 + B + half - 1) / M); one
+// grid spacing of margin on each side absorbs rounding in the binning.
+__device__ __forceinline__ void cell_span(int g0, int B, int half, int M, int nc, int &lo,
                                           int &len) {
-  // cells holding atoms whose grid index m lies in [g0 - half, g0 + B + half - 2];
-  // one cell of margin each side absorbs rounding in the binning
-  lo = (int)floor(nc * (g0 - half) / M) - 1;
-  int hi = (int)floor(nc * (g0 + B + half - 1) / M) + 1;
+  lo = (int)floor((double)nc * (g0 - half - 1) / M);
+  int hi = (int)floor((double)nc * (g0 + B + half) / M);
   len = hi - lo + 1;
   if (len >= nc) lo = 0, len = nc;
 }
 
+// Locates candidate k in the range table (roff[lo] <= k < roff[lo + 1]),
+// advancing from a previous position lo (k only grows) and loads its atom.
+__device__ __forceinline__ double4 spread_fetch(const int *roff, const int *rbeg, int nr, int k,
+                                                int total, int &lo,
+                                                const double4 *__restrict__ spos) {
+  if (k >= total) return make_double4(0.0, 0.0, 0.0, 0.0);
+  while (lo + 1 < nr && roff[lo + 1] <= k) ++lo;
+  return spos[rbeg[lo] + (k - roff[lo])];
+}
+
 template <int NU>
-__global__ __launch_bounds__(256) void k_spread_brick(DevParams P, GridDims g,
-                                                      const double4 *__restrict__ spos,
-                                                      const int *__restrict__ start,
-                                                      double *__restrict__ q1) {
-  __shared__ double acc[kBX * kBY * kBZ];
-  __shared__ int rbeg[kMaxRanges], roff[kMaxRanges + 1];
-  const int tid = threadIdx.x;
-  const int nbz = (g.z + kBZ - 1) / kBZ, nby = (g.y + kBY - 1) / kBY;
+__global__ __launch_bounds__(256) void k_spread_mfma(DevParams P, GridDims g,
+                                                     const double4 *__restrict__ spos,
+                                                     const int *__restrict__ start,
+                                                     double *__restrict__ q1) {
+  // per-wavefront records of 64 staged atoms
+  __shared__ double WX[256 * kWXs];
+  __shared__ double WY[256 * kWYs], WZ[256 * kWYs];
+  __shared__ int rbeg[kSRanges], roff[kSRanges + 1];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int nbz = (g.z + kSZ - 1) / kSZ, nby = (g.y + kSY - 1) / kSY;
   int b = blockIdx.x;
-  const int gz0 = (b % nbz) * kBZ;
+  const int gz0 = (b % nbz) * kSZ;
   b /= nbz;
-  const int gy0 = (b % nby) * kBY, gx0 = (b / nby) * kBX;
-  for (int k = tid; k < kBX * kBY * kBZ; k += 256) acc[k] = 0.0;
+  const int gy0 = (b % nby) * kSY, gx0 = (b / nby) * kSX;
   int lox, lenx, loy, leny, loz, lenz;
-  cell_span(gx0, kBX, NU / 2, g.x, P.nc[0], lox, lenx);
-  cell_span(gy0, kBY, NU / 2, g.y, P.nc[1], loy, leny);
-  cell_span(gz0, kBZ, NU / 2, g.z, P.nc[2], loz, lenz);
+  cell_span(gx0, kSX, NU / 2, g.x, P.nc[0], lox, lenx);
+  cell_span(gy0, kSY, NU / 2, g.y, P.nc[1], loy, leny);
+  cell_span(gz0, kSZ, NU / 2, g.z, P.nc[2], loz, lenz);
   const int ncol = lenx * leny;
-  const int dims[3] = {g.x, g.y, g.z}, org[3] = {gx0, gy0, gz0}, ext[3] = {kBX, kBY, kBZ};
-  // columns in batches small enough for the range table; each column's z
-  // cells [loz, loz + lenz) wrap at most once, giving two index ranges
-  for (int c0 = 0; c0 < ncol; c0 += kMaxRanges / 2) {
-    const int nc_b = min(ncol - c0, kMaxRanges / 2);
+  const int dims[3] = {g.x, g.y, g.z}, org[3] = {gx0, gy0, gz0}, ext[3] = {kSX, kSY, kSZ};
+  const unsigned long long below = (1ull << lane) - 1ull;
+  double *const RX = WX + wv * 64 * kWXs, *const RY = WY + wv * 64 * kWYs,
+               *const RZ = WZ + wv * 64 * kWYs;
+  const int row = lane & 15, kq = lane >> 4;
+  sp_f64x4 acc = {0.0, 0.0, 0.0, 0.0};
+  for (int c0 = 0; c0 < ncol; c0 += kSRanges / 2) {
+    const int nc_b = min(ncol - c0, kSRanges / 2);
     __syncthreads();  // previous batch done with the table
     for (int cc = tid; cc < nc_b; cc += 256) {
       int col = c0 + cc;
@@ -499,60 +636,111 @@ __global__ __launch_bounds__(256) void k_spread_brick(DevParams P, GridDims g,
     }
     __syncthreads();
     const int nr = 2 * nc_b;
-    if (tid == 0) {  // prefix of the range lengths
-      roff[0] = 0;
-      for (int k = 0; k < nr; ++k) roff[k + 1] += roff[k];
+    if (wv == 0) {  // inclusive prefix of roff[1..nr]: 8 entries per lane
+      int loc[8], run = 0;
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        int k = 8 * lane + u;
+        run += k < nr ? roff[k + 1] : 0;
+        loc[u] = run;
+      }
+      int incl = run;
+      for (int off = 1; off < 64; off <<= 1) {
+        int t = __shfl_up(incl, off, 64);
+        if (lane >= off) incl += t;
+      }
+      const int excl = incl - run;
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        int k = 8 * lane + u;
+        if (k < nr) roff[k + 1] = excl + loc[u];
+      }
+      if (lane == 0) roff[0] = 0;
     }
     __syncthreads();
     const int total = roff[nr];
-    for (int k = tid; k < total; k += 256) {
-      int lo = 0, hi = nr;  // roff[lo] <= k < roff[hi]
-      while (hi - lo > 1) {
-        int mid = (lo + hi) >> 1;
-        if (roff[mid] <= k) lo = mid; else hi = mid;
-      }
-      double4 at = spos[rbeg[lo] + (k - roff[lo])];
-      double r[3] = {at.x, at.y, at.z};
-      int loc[3][NU];
-      double w[3][NU];
-      bool any = true;
+    // wavefront wv takes candidates 64 (4 r + wv) + lane, r = 0, 1, ...; the
+    // next candidate's atom is fetched while the current one is processed
+    int k = wv * 64 + lane, lo = 0;
+    double4 nxt = spread_fetch(roff, rbeg, nr, k, total, lo, spos);
+    for (; k - lane < total; k += 256) {
+      const double4 at = nxt;
+      const bool valid = k < total;
+      nxt = spread_fetch(roff, rbeg, nr, k + 256, total, lo, spos);
+      // ---- which atoms reach the brick, and where their support starts
+      bool reach = valid;
+      double tt[3];
+      int rlo[3];
+      const double r[3] = {at.x, at.y, at.z};
 #pragma unroll
       for (int ax = 0; ax < 3; ++ax) {
         double sv = r[ax] * P.ibox[ax];
         sv = sv - floor(sv);
         double t = dims[ax] * sv, m = floor(t);
-        t -= m;
-        int mi = (int)m, cnt = 0;
-#pragma unroll
-        for (int i = 0; i < NU; ++i) {
-          int l = wrapi(mi - i + NU / 2 - org[ax], dims[ax]);
-          loc[ax][i] = l < ext[ax] ? l : -1;
-          cnt += l < ext[ax];
-        }
-        any = any && cnt > 0;
-        if (any) bspline<NU, false>(P.Q, t, w[ax], nullptr);
+        tt[ax] = t - m;
+        // brick-relative index of the lowest support point m - NU/2 + 1
+        rlo[ax] = wrapi((int)m - NU / 2 + 1 - org[ax], dims[ax]);
+        reach = reach && (rlo[ax] < ext[ax] || rlo[ax] > dims[ax] - NU);
       }
-      if (!any) continue;
+      const unsigned long long bm = __ballot(reach);
+      const int nrel = __popcll(bm), npad = (nrel + 3) & ~3;
+      if (nrel == 0) continue;  // wavefront-uniform
+      const int slot = reach ? __popcll(bm & below) : -1;
+      // clear this lane's record and, independently, the padding record with
+      // this lane's index (slots [nrel, npad) hold no atom)
 #pragma unroll
-      for (int x = 0; x < NU; ++x) {
-        if (loc[0][x] < 0) continue;
-        double wx = at.w * w[0][x];
+      for (int pass = 0; pass < 2; ++pass) {
+        const int sl = pass == 0 ? slot : ((lane >= nrel && lane < npad) ? lane : -1);
+        if (sl < 0) continue;
 #pragma unroll
-        for (int y = 0; y < NU; ++y) {
-          if (loc[1][y] < 0) continue;
-          double wxy = wx * w[1][y];
-          int row = (loc[0][x] * kBY + loc[1][y]) * kBZ;
+        for (int u = 0; u < kSX; ++u) RX[sl * kWXs + u] = 0.0;
 #pragma unroll
-          for (int z = 0; z < NU; ++z)
-            if (loc[2][z] >= 0) atomicAdd(&acc[row + loc[2][z]], wxy * w[2][z]);
+        for (int u = 0; u < kSY; ++u) RY[sl * kWYs + u] = 0.0, RZ[sl * kWYs + u] = 0.0;
+      }
+      if (reach) {
+        double w[NU];
+        double *dst[3] = {RX + slot * kWXs, RY + slot * kWYs, RZ + slot * kWYs};
+        const double scale[3] = {1.0, at.w, 1.0};
+#pragma unroll
+        for (int ax = 0; ax < 3; ++ax) {
+          bspline<NU, false>(P.Q, tt[ax], w, nullptr);
+#pragma unroll
+          for (int i = 0; i < NU; ++i) {  // weight i sits at point m + NU/2 - i
+            int rr = rlo[ax] + (NU - 1 - i);
+            while (rr >= dims[ax]) rr -= dims[ax];
+            if (rr < ext[ax]) dst[ax][rr] += scale[ax] * w[i];
+          }
         }
       }
+      // LDS operations of one wavefront complete in order; the barrier keeps
+      // the compiler from moving the reads above the writes
+      __builtin_amdgcn_wave_barrier();
+      // ---- Q[x][(y, z)] += WX[a][x] * WY[a][y] * WZ[a][z], four atoms per MFMA
+      for (int a0 = 0; a0 < npad; a0 += 4) {
+        const int a = a0 + kq;
+        const double av = RX[a * kWXs + row];
+        const double bv = RY[a * kWYs + (row >> 2)] * RZ[a * kWYs + (row & 3)];
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
+      }
+      __builtin_amdgcn_wave_barrier();
     }
   }
+  // ---- sum the four wavefronts' tiles in a fixed order and store
   __syncthreads();
-  for (int k = tid; k < kBX * kBY * kBZ; k += 256) {
-    int z = gz0 + k % kBZ, y = gy0 + (k / kBZ) % kBY, x = gx0 + k / (kBZ * kBY);
-    if (x < g.x && y < g.y && z < g.z) q1[((long)x * g.y + y) * g.z + z] = acc[k];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) WX[(wv * 4 + i) * 64 + lane] = acc[i];
+  __syncthreads();
+  if (wv == 0) {
+    // f64 16x16x4 result layout: register i of lane l holds row 4 i + l / 16,
+    // column l % 16
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      double v = WX[i * 64 + lane];
+      for (int w = 1; w < 4; ++w) v += WX[(w * 4 + i) * 64 + lane];
+      const int x = gx0 + 4 * i + (lane >> 4), col = lane & 15;
+      const int y = gy0 + (col >> 2), z = gz0 + (col & 3);
+      if (x < g.x && y < g.y && z < g.z) q1[((long)x * g.y + y) * g.z + z] = v;
+    }
   }
 }
 
@@ -1024,8 +1212,8 @@ void launch_pair_count(const DevParams &P, const double4 *spos, const int *scell
 
 void launch_spread(const DevParams &P, GridDims g, const double4 *spos, const int *start,
                    double *q1, hipStream_t s) {
-  unsigned nb = (unsigned)(((g.x + kBX - 1) / kBX) * ((g.y + kBY - 1) / kBY) * ((g.z + kBZ - 1) / kBZ));
-#define CALL(NU) k_spread_brick<NU><<<nb, 256, 0, s>>>(P, g, spos, start, q1)
+  unsigned nb = (unsigned)(((g.x + kSX - 1) / kSX) * ((g.y + kSY - 1) / kSY) * ((g.z + kSZ - 1) / kSZ));
+#define CALL(NU) k_spread_mfma<NU><<<nb, 256, 0, s>>>(P, g, spos, start, q1)
   MSM_NU_SWITCH(P.nu, CALL)
 #undef CALL
 }

@@ -100,6 +100,19 @@ class MSM:
         _check(_lib.lib().FF_set_precision(self._ff, PRECISIONS[precision]), "precision")
         self.precision = precision
 
+    def set_stage_timing(self, enable):
+        """Record per-stage GPU intervals (read with stage_times())."""
+        _check(_lib.lib().FF_set_stage_timing(self._ff, 1 if enable else 0), "stage timing")
+
+    def stage_times(self):
+        """Mean seconds per evaluation of each stage since the last call."""
+        L = _lib.lib()
+        st = (ctypes.c_double * 16)()
+        n = L.msm_stage_times(self._ff, st, 16)
+        _check(min(n, 0), "stage times")
+        return {L.msm_stage_name(k).decode(): st[k] / max(n, 1) for k in range(16)
+                if L.msm_stage_name(k)}
+
     def set_charges(self, charges):
         self._q[:] = np.asarray(charges, dtype=np.float64)
 
