@@ -3,12 +3,14 @@ MI355X, mixed precision against plain FP64, with the rms relative force
 error of the mixed path.
 
 One step is one full evaluation (binning, short-range pairs, spreading,
-restriction, top-level DFT, level stencils, interpolation; energy and
-forces) of the water96k workload: 32,000 rigid 3-site waters in a cubic box
-of 98.6 A, order-6 B-splines, cutoff 10 A, 4 levels (grids 48/24/12/6),
-positions and charges resident in HBM.  Prints one JSON line.
+restriction, level stencils, top-level DFT, prolongation, interpolation;
+energy and forces) of the water96k workload: 32,000 rigid 3-site waters in a
+cubic box of 98.6 A, order-6 B-splines, cutoff 10 A, 4 levels (grids
+48/24/12/6), positions and charges resident in HBM.  Prints one JSON line.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--nu 6] [--no-cpu]
+    python bench.py --workload water1m      # 1M-atom run, its own JSON line
+    python bench.py --copy-calibration MIB  # plain copy for counter scaling
 """
 import argparse
 import ctypes
@@ -24,6 +26,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "MSM energy+force evals/s at 96k atoms, mixed vs FP64 GPU; rms rel force err"
 PEAK = {"hbm": 8000.0, "fp64": 78.6, "fp32": 157.3, "bf16": 2500.0}  # GB/s, TFLOP/s
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "water96k_hbm_traffic.txt")
 
 
 def parse():
@@ -33,88 +36,129 @@ def parse():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--nu", type=int, default=6)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--workload", default="water96k", choices=["water96k", "water1m"])
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB five times (the reference "
+                         "kernel for scaling FETCH_SIZE / WRITE_SIZE counters) and exit")
     return ap.parse_args()
 
 
+def copy_calibration(mib):
+    """Plain copy kernel of known size: reads and writes mib MiB per launch."""
+    import torch
+    n = mib * (1 << 20) // 4
+    x = torch.ones(n, dtype=torch.float32, device="cuda:0")
+    y = torch.empty_like(x)
+    for _ in range(5):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes_per_launch": n * 4, "launches": 5}))
+
+
 def algorithmic(stage, m, P, nu):
-    """Algorithmic work of one launch-group of a stage: (amount, unit, bound)."""
+    """Algorithmic work of one launch of a stage's kernel: (amount, unit, kernel)."""
     N = m.N
-    if stage == "partcl2partcl":  # P within-cutoff pairs, 35 + 4(nu-1) FLOP each
-        return P * (35 + 4 * (nu - 1)), "flop", "valu"
+    mixed = "true" if m.precision == "mixed" else "false"
+    if stage == "partcl2partcl":  # P within-cutoff pairs i < j, 35 + 4(nu-1) FLOP each
+        return P * (35 + 4 * (nu - 1)), "flop", f"k_pair_cell<{mixed}, {nu}>"
     if stage == "grid2grid":  # 2 * sum_l prod(M_l) prod(s_l), l < L
-        f = 0
-        for l in range(1, m.levels):
-            f += 2 * np.prod(m.grid_dims(l)) * np.prod(m.stencil_dims(l))
-        return f, "flop", "mfma"
-    if stage == "anterpolation":  # read atoms (x,y,z,q), write level-1 grid
-        return N * 32 + 8 * np.prod(m.grid_dims(1)), "byte", "hbm"
-    if stage == "interpolation":  # read atoms + grid, read-modify-write F
-        return N * (32 + 4 + 48) + 8 * np.prod(m.grid_dims(1)), "byte", "hbm"
+        f = sum(2 * np.prod(m.grid_dims(l)) * np.prod(m.stencil_dims(l))
+                for l in range(1, m.levels))
+        return f, "flop", "k_g2g_tiles<" + ("1" if mixed == "true" else "0")
+    if stage == "anterpolation":  # N (2 nu^3 + 6 nu^2) FLOP
+        return N * (2 * nu ** 3 + 6 * nu ** 2), "flop", f"k_spread_mfma<{nu}>"
+    if stage == "interpolation":  # N (8 nu^3 + 6 nu^2) FLOP
+        return N * (8 * nu ** 3 + 6 * nu ** 2), "flop", f"k_gather<{nu}>"
     return None
 
 
-def run_gpu(args, sysm):
+def traffic_of(kernel):
+    """Calibrated HBM bytes per dispatch of `kernel` from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (None if absent)."""
+    if not os.path.exists(TRAFFIC_FILE):
+        return None
+    for line in open(TRAFFIC_FILE):
+        if line.startswith("#") or not line.strip():
+            continue
+        name, _, _, _, b = line.rsplit(None, 4)
+        if name.strip().startswith(kernel):
+            return float(b)
+    return None
+
+
+def device_free():
+    import torch
+    torch.cuda.synchronize()
+    return torch.cuda.mem_get_info(0)[0]
+
+
+def run_precisions(args, sysm, precisions=("fp64", "mixed"), steps=None, warmup=None):
+    """Timed evaluations per precision through msm_eval_device: a plain timed
+    region of K steps, then K more steps with per-stage HIP events on the
+    streams the kernels run on (kept out of the plain timed region)."""
     import torch
     from msmpb_amd import MSM, _lib
     L = _lib.lib()
+    steps = args.steps if steps is None else steps
+    warmup = args.warmup if warmup is None else warmup
     dev = torch.device("cuda:0")
     p = dict(sysm.params)
     kw = dict(nu=p["nu"], levels=p["levels"], top_grid=p["top_grid"], cutoff=p["cutoff"])
-    m64 = MSM(sysm.pos, sysm.q, sysm.edges, precision="fp64", **kw)
-    mmx = MSM(sysm.pos, sysm.q, sysm.edges, precision="mixed", **kw)
     pos = torch.tensor(sysm.pos, dtype=torch.float64, device=dev)
     q = torch.tensor(sysm.q, dtype=torch.float64, device=dev)
     out = torch.empty(4, dtype=torch.float64, device=dev)
     F = torch.empty((sysm.N, 3), dtype=torch.float64, device=dev)
-    stream = torch.cuda.current_stream(dev)
-    sp = ctypes.c_void_p(stream.cuda_stream)
+    sp = ctypes.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
     vp = lambda t: ctypes.c_void_p(t.data_ptr())  # noqa: E731
+    res = {}
+    for tag in precisions:
+        free0 = device_free()
+        m = MSM(sysm.pos, sysm.q, sysm.edges, precision=tag, **kw)
+        footprint = free0 - device_free()
 
-    def step(m):
-        rc = L.msm_eval_device(m._ff, vp(pos), vp(q), vp(out), vp(F), None, sp)
-        if rc != 0:
-            raise RuntimeError(_lib.last_error())
+        def step():
+            rc = L.msm_eval_device(m._ff, vp(pos), vp(q), vp(out), vp(F), None, sp)
+            if rc != 0:
+                raise RuntimeError(_lib.last_error())
 
-    result = {}
-    for tag, m in (("fp64", m64), ("mixed", mmx)):
-        for _ in range(args.warmup):
-            step(m)
+        for _ in range(warmup):
+            step()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        for _ in range(args.steps):
-            step(m)
+        for _ in range(steps):
+            step()
         torch.cuda.synchronize()
         t1 = time.perf_counter()
-        # stage breakdown in a separate, shorter loop: the interval events
-        # are kept out of the timed region
         m.set_stage_timing(True)
-        for _ in range(max(3, args.steps // 3)):
-            step(m)
+        for _ in range(steps):
+            step()
         torch.cuda.synchronize()
         stages = m.stage_times()
         m.set_stage_timing(False)
-        step(m)
+        step()
         torch.cuda.synchronize()
-        result[tag] = dict(ms=(t1 - t0) * 1e3 / args.steps, stages=stages, E=out[0].item(),
-                           F=F.cpu().numpy().copy(), m=m)
-    # forward + adjoint (autograd) path, mixed precision
+        res[tag] = dict(ms=(t1 - t0) * 1e3 / steps, stages=stages, E=out[0].item(),
+                        F=F.cpu().numpy().copy(), m=m, footprint=footprint)
+    res["pos"], res["q"] = pos, q
+    return res
+
+
+def autograd_ms(m, pos, q, steps, warmup, recompute=None):
+    """Forward energy + adjoint (dE/dr, dE/dq) through torch.autograd."""
+    import torch
     from msmpb_amd import msm_energy
     pg = pos.clone().requires_grad_(True)
-    for _ in range(args.warmup):
-        e = msm_energy(mmx, pg, q)
-        e.backward()
-        pg.grad = None
+    qg = q.clone().requires_grad_(True)
+    for _ in range(warmup):
+        msm_energy(m, pg, qg, recompute=recompute).backward()
+        pg.grad = qg.grad = None
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for _ in range(args.steps):
-        e = msm_energy(mmx, pg, q)
-        e.backward()
-        pg.grad = None
+    for _ in range(steps):
+        msm_energy(m, pg, qg, recompute=recompute).backward()
+        pg.grad = qg.grad = None
     torch.cuda.synchronize()
-    result["mixed_autograd_ms"] = (time.perf_counter() - t0) * 1e3 / args.steps
-    result["pairs"] = m64.pair_count
-    result["models"] = (m64, mmx)
-    return result
+    return (time.perf_counter() - t0) * 1e3 / steps
 
 
 def cpu_baseline(sysm):
@@ -130,40 +174,41 @@ def cpu_baseline(sysm):
     t = time.process_time() - t0
     return dict(value=1.0 / t, unit="evals/s", cores=1, kind="port",
                 sample=f"1 full evaluation of the same {sysm.N}-atom workload (energy+forces, "
-                       f"pair-list build excluded), {t:.2f} s CPU"), E, F
+                       f"pair-list build excluded), {t:.2f} s CPU, one core"), E, F
 
 
-def main():
-    args = parse()
-    if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
-              "(the multi-GPU decomposition is design-only)", file=sys.stderr)
-        sys.exit(2)
+def rms_rel(Fa, Fb):
+    return float(np.sqrt(np.sum((Fa - Fb) ** 2) / np.sum(Fb ** 2)))
+
+
+def roofline(stages, m, pairs, nu):
+    """Dominant kernel of the evaluation: achieved rate from its HIP-event
+    duration against the gfx950 peak, with its calibrated HBM traffic."""
+    dom = max(stages, key=stages.get)
+    alg = algorithmic(dom, m, pairs, nu)
+    if alg is None or stages[dom] <= 0:
+        return None
+    amount, unit, kernel = alg
+    ach = amount / stages[dom] / 1e12
+    peak = {"partcl2partcl": PEAK["fp32"] if m.precision == "mixed" else PEAK["fp64"],
+            "grid2grid": PEAK["bf16"] if m.precision == "mixed" else PEAK["fp64"]}.get(
+                dom, PEAK["fp64"])
+    return dict(bound="mfma", achieved=ach, peak=peak, unit="TFLOP/s", frac=ach / peak,
+                traffic=traffic_of(kernel), kernel=kernel, stage=dom,
+                work_per_launch=float(amount), launch_ms=stages[dom] * 1e3)
+
+
+def main_water96k(args):
     from msmpb_amd import inputs
     sysm = inputs.cfg3_water96k(args.nu)
-    r = run_gpu(args, sysm)
+    r = run_precisions(args, sysm)
     F64, Fmx = r["fp64"]["F"], r["mixed"]["F"]
-    rms_rel = float(np.sqrt(np.sum((Fmx - F64) ** 2) / np.sum(F64 ** 2)))
-    relE = abs(r["mixed"]["E"] - r["fp64"]["E"]) / abs(r["fp64"]["E"])
+    m64, mmx = r["fp64"]["m"], r["mixed"]["m"]
+    pairs = m64.pair_count
     ms_mx, ms_64 = r["mixed"]["ms"], r["fp64"]["ms"]
-    m64, mmx = r["models"]
-    # roofline of the dominant kernel of the mixed path
+    ad_mx = autograd_ms(mmx, r["pos"], r["q"], args.steps, args.warmup)
+    ad_64 = autograd_ms(m64, r["pos"], r["q"], args.steps, args.warmup)
     stages = r["mixed"]["stages"]
-    dom = max(stages, key=stages.get)
-    roof = None
-    alg = algorithmic(dom, mmx, r["pairs"], mmx.nu)
-    if alg is not None and stages[dom] > 0:
-        amount, unit, bound = alg
-        if unit == "flop":
-            ach = amount / stages[dom] / 1e12
-            peak = PEAK["bf16"] if bound == "mfma" else PEAK["fp32"]
-            roof = dict(bound="mfma", achieved=ach, peak=peak, unit="TFLOP/s", frac=ach / peak,
-                        traffic=None, kernel=dom, work_per_launch=float(amount))
-        else:
-            ach = amount / stages[dom] / 1e9
-            roof = dict(bound="hbm", achieved=ach, peak=PEAK["hbm"], unit="GB/s",
-                        frac=ach / PEAK["hbm"], traffic=None, kernel=dom,
-                        work_per_launch=float(amount))
     line = {
         "metric": METRIC, "value": 1e3 / ms_mx, "unit": "evals/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_mx,
@@ -175,21 +220,79 @@ def main():
                    "precision": "mixed", "global_batch": sysm.N},
         "fp64_evals_per_s": 1e3 / ms_64, "fp64_ms_per_step": ms_64,
         "speedup_mixed_vs_fp64": ms_64 / ms_mx,
-        "rms_rel_force_err": rms_rel, "rel_energy_err": relE,
-        "mixed_forward_adjoint_ms": r["mixed_autograd_ms"],
+        "rms_rel_force_err": rms_rel(Fmx, F64),
+        "rel_energy_err": abs(r["mixed"]["E"] - r["fp64"]["E"]) / abs(r["fp64"]["E"]),
+        "mixed_forward_adjoint_ms": ad_mx, "fp64_forward_adjoint_ms": ad_64,
+        "speedup_adjoint_mixed_vs_fp64": ad_64 / ad_mx,
         "stage_ms_mixed": {k: v * 1e3 for k, v in stages.items()},
         "stage_ms_fp64": {k: v * 1e3 for k, v in r["fp64"]["stages"].items()},
-        "pairs_within_cutoff": r["pairs"],
-        "roofline": roof,
+        "hbm_footprint_bytes": {"fp64": r["fp64"]["footprint"], "mixed": r["mixed"]["footprint"]},
+        "pairs_within_cutoff": pairs,
+        "roofline": roofline(stages, mmx, pairs, mmx.nu),
     }
+    # the same workload at the other B-spline order (nu = 4)
+    if args.nu == 6:
+        s4 = inputs.cfg3_water96k(4)
+        r4 = run_precisions(args, s4)
+        line["nu4"] = {"mixed_evals_per_s": 1e3 / r4["mixed"]["ms"],
+                       "fp64_evals_per_s": 1e3 / r4["fp64"]["ms"],
+                       "rms_rel_force_err": rms_rel(r4["mixed"]["F"], r4["fp64"]["F"]),
+                       "rel_energy_err": abs(r4["mixed"]["E"] - r4["fp64"]["E"])
+                       / abs(r4["fp64"]["E"])}
+        for t in ("fp64", "mixed"):
+            r4[t]["m"].close()
     if not args.no_cpu:
         cb, Ec, Fc = cpu_baseline(sysm)
         line["cpu_baseline"] = cb
         line["fp64_vs_cpu_rel_energy"] = abs(r["fp64"]["E"] - Ec) / abs(Ec)
-        line["rms_rel_force_err_vs_oracle"] = float(np.sqrt(np.sum((Fmx - Fc) ** 2) / np.sum(Fc ** 2)))
+        line["fp64_vs_cpu_rms_rel_force"] = rms_rel(F64, Fc)
+        line["rms_rel_force_err_vs_oracle"] = rms_rel(Fmx, Fc)
     else:
         line["cpu_baseline"] = None
     print(json.dumps(line))
+
+
+def main_water1m(args):
+    """1M-atom box: evals/s of both precisions, the adjoint with recomputed
+    grid levels, the device-memory footprint of each FF, and the mixed
+    path's force error against FP64."""
+    from msmpb_amd import inputs
+    sysm = inputs.cfg4_water1m()
+    steps, warmup = max(2, args.steps // 10), 1
+    r = run_precisions(args, sysm, steps=steps, warmup=warmup)
+    m64, mmx = r["fp64"]["m"], r["mixed"]["m"]
+    ad = autograd_ms(mmx, r["pos"], r["q"], steps, warmup, recompute=True)
+    line = {
+        "metric": "MSM energy+force evals/s at 1M atoms (water1m)",
+        "value": 1e3 / r["mixed"]["ms"], "unit": "evals/s", "n_gpus": 1, "steps": steps,
+        "warmup": warmup, "ms_per_step": r["mixed"]["ms"], "higher_is_better": True,
+        "data": "synthetic",
+        "config": {"workload": f"water1m: 333,333 rigid 3-site waters ({sysm.N} atoms), cubic "
+                               f"box 215 A, nu=6, cutoff 10 A, {mmx.levels} levels, top grid "
+                               f"{mmx.top_grid[0]} (finest {mmx.grid_dims(1)[0]})"},
+        "fp64_evals_per_s": 1e3 / r["fp64"]["ms"],
+        "rms_rel_force_err": rms_rel(r["mixed"]["F"], r["fp64"]["F"]),
+        "mixed_forward_adjoint_recompute_ms": ad,
+        "hbm_footprint_bytes": {"fp64": r["fp64"]["footprint"], "mixed": r["mixed"]["footprint"]},
+        "stage_ms_mixed": {k: v * 1e3 for k, v in r["mixed"]["stages"].items()},
+        "pairs_within_cutoff": m64.pair_count,
+    }
+    print(json.dumps(line))
+
+
+def main():
+    args = parse()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
+              "(the multi-GPU decomposition is design-only)", file=sys.stderr)
+        sys.exit(2)
+    if args.workload == "water1m":
+        main_water1m(args)
+    else:
+        main_water96k(args)
 
 
 if __name__ == "__main__":
