@@ -5,9 +5,12 @@
  * keep positions and charges in GPU memory (the PyTorch autograd binding in
  * msmpb_amd/) and callers that choose the precision or the reference HEAD
  * index layout.  All pointers named d_* are device pointers (HBM) of the
- * current HIP device; `stream` is a hipStream_t (NULL = the FF's own
- * stream).  Every int-returning call returns 0 on success and a negative
- * code on failure, with a message available from FF_last_error().
+ * current HIP device; `stream` is a hipStream_t on which the call is
+ * ordered (NULL = the default stream, as everywhere in HIP; PyTorch's
+ * default stream is that stream).  Internally the short-range pairs run on
+ * a second library stream forked from and joined back into `stream`.
+ * Every int-returning call returns 0 on success and a negative code on
+ * failure, with a message available from FF_last_error().
  */
 #ifndef MSMPB_HIP_EXT_H
 #define MSMPB_HIP_EXT_H

@@ -92,6 +92,10 @@ struct Engine {
   double *d_out = nullptr, *d_qpart = nullptr, *d_scal = nullptr;
   double *d_epair = nullptr, *d_egrid = nullptr;
   int *d_cell_of = nullptr, *d_cnt = nullptr, *d_start = nullptr, *d_sidx = nullptr;
+  int *d_rank = nullptr;
+  double *d_Fp = nullptr, *d_phip = nullptr;  // short-range parts of F and phi
+  double *d_ep2 = nullptr;                    // per-workgroup pair-energy partials
+  unsigned *d_done = nullptr;  // workgroup counter of the interpolation's energy reduction
   int *d_scell = nullptr, *d_flags = nullptr;
   double4 *d_wpos = nullptr, *d_spos = nullptr;
   float4 *d_lpos = nullptr;
@@ -120,6 +124,7 @@ struct Engine {
     T.clear();
     void *ptrs[] = {d_pos,  d_q,    d_F,    d_phi,   d_out,  d_qpart, d_scal,  d_epair,
                     d_egrid, d_cell_of, d_cnt, d_start, d_sidx, d_scell, d_flags, d_wpos,
+                    d_rank, d_done, d_Fp, d_phip, d_ep2,
                     d_spos, d_lpos, d_sten, d_count, d_qsplit, d_tw, d_topwork};
     for (void *p : ptrs)
       if (p) hipFree(p);
@@ -139,7 +144,9 @@ struct Engine {
       d_ksplit[l] = nullptr;
     }
     d_pos = d_q = d_F = d_phi = d_out = d_qpart = d_scal = d_epair = d_egrid = nullptr;
-    d_cell_of = d_cnt = d_start = d_sidx = d_scell = d_flags = nullptr;
+    d_cell_of = d_cnt = d_start = d_sidx = d_scell = d_flags = d_rank = nullptr;
+    d_done = nullptr;
+    d_Fp = d_phip = d_ep2 = nullptr;
     d_wpos = d_spos = nullptr;
     d_lpos = nullptr;
     d_sten = nullptr;
@@ -380,13 +387,23 @@ int create_engine(FFX *x, const msm::Plan &plan) {
   TRY(dalloc(&E->d_qpart, 2 * nb));
   TRY(dalloc(&E->d_scal, 4));
   TRY(dalloc(&E->d_epair, std::max(np, E->P.ncell)));
-  TRY(dalloc(&E->d_egrid, msm::grid_blocks(E->gd[1].size())));
+  TRY(dalloc(&E->d_egrid, std::max<long>(msm::grid_blocks(E->gd[1].size()),
+                                          msm::prolong_blocks(E->gd[1]))));
   TRY(dalloc(&E->d_cell_of, N));
+  TRY(dalloc(&E->d_rank, N));
+  TRY(dalloc(&E->d_Fp, 3 * N));
+  TRY(dalloc(&E->d_phip, N));
+  TRY(dalloc(&E->d_ep2, 2 * msm::combine_blocks((int)N)));
+  TRY(dalloc(&E->d_done, 1));
   TRY(dalloc(&E->d_cnt, E->P.ncell));
   TRY(dalloc(&E->d_start, E->P.ncell + 1));
   TRY(dalloc(&E->d_sidx, N));
   TRY(dalloc(&E->d_scell, N));
   TRY(dalloc(&E->d_flags, 4));
+  // counters that each evaluation leaves cleared for the next one
+  HIP_TRY(hipMemset(E->d_cnt, 0, E->P.ncell * sizeof(int)));
+  HIP_TRY(hipMemset(E->d_flags, 0, 4 * sizeof(int)));
+  HIP_TRY(hipMemset(E->d_done, 0, sizeof(unsigned)));
   TRY(dalloc(&E->d_wpos, N));
   TRY(dalloc(&E->d_spos, N));
   TRY(dalloc(&E->d_lpos, N));
@@ -405,13 +422,12 @@ int create_engine(FFX *x, const msm::Plan &plan) {
 // Binning + counting sort of atoms into cells (replaces the stored
 // neighbour list of forcefield0.c:21-164).
 void stage_prepare(Engine &E, const double *pos, const double *q, hipStream_t s) {
-  msm::launch_fill_int(E.d_cnt, E.P.ncell, 0, s);
-  msm::launch_fill_int(E.d_flags, 1, 0, s);
-  msm::launch_bin(E.P, pos, q, E.d_cell_of, E.d_cnt, E.d_wpos, E.d_qpart, E.d_flags, s);
+  // d_cnt is zero here: set at allocation, cleared again by every cell sort
+  msm::launch_bin(E.P, pos, q, E.d_cell_of, E.d_rank, E.d_cnt, E.d_wpos, E.d_qpart, E.d_flags, s);
   msm::launch_scan(E.P, E.d_cnt, E.d_start, E.d_qpart, msm::bin_blocks(E.P.N), E.d_scal, s);
-  msm::launch_scatter(E.P, E.d_cell_of, E.d_wpos, E.d_start, E.d_cnt, E.d_spos, E.d_sidx,
+  msm::launch_scatter(E.P, E.d_cell_of, E.d_rank, E.d_wpos, E.d_start, E.d_spos, E.d_sidx,
                       E.d_scell, s);
-  msm::launch_cell_sort(E.P, E.d_start, E.d_spos, E.d_sidx, E.d_lpos, s);
+  msm::launch_cell_sort(E.P, E.d_start, E.d_cnt, E.d_spos, E.d_sidx, E.d_lpos, s);
 }
 
 void stage_pairs(Engine &E, int precision, double *F, double *phi, hipStream_t s) {
@@ -425,18 +441,25 @@ void stage_pairs(Engine &E, int precision, double *F, double *phi, hipStream_t s
 
 // Spread, restrict to the top, top-level DFT convolution, then prolongate
 // and add each level's stencil pass down to e^1.
-void stage_grids(Engine &E, int precision, int layout, Timing &T, hipStream_t s) {
+// Charges down the hierarchy: spread to level 1, restrict to the top.
+void stage_charges(Engine &E, Timing &T, hipStream_t s) {
   const msm::Plan &p = E.plan;
-  int L = p.L;
-  int pr = precision == MSM_PREC_MIXED ? msm::MSM_G2G_BF16X3 : msm::MSM_G2G_FP64;
   int t = T.open(s);
   msm::launch_spread(E.P, E.gd[1], E.d_spos, E.d_start, E.d_ql[1], s);
   T.close(t, "anterpolation", s);
-  for (int l = 2; l <= L; ++l) {
+  for (int l = 2; l <= p.L; ++l) {
     t = T.open(s);
     msm::launch_restrict(E.P, E.gd[l - 1], E.gd[l], E.d_ql[l - 1], E.d_ql[l], s);
     T.close(t, "restriction", s);
   }
+}
+
+// Potentials up the hierarchy: level stencils, top-level DFT, prolongation.
+void stage_potentials(Engine &E, int precision, int layout, Timing &T, hipStream_t s) {
+  const msm::Plan &p = E.plan;
+  int L = p.L;
+  int pr = precision == MSM_PREC_MIXED ? msm::MSM_G2G_BF16X3 : msm::MSM_G2G_FP64;
+  int t;
   if (L >= 2) {
     // stencil correlations of every level l < L in one launch
     t = T.open(s);
@@ -449,17 +472,47 @@ void stage_grids(Engine &E, int precision, int layout, Timing &T, hipStream_t s)
   T.close(t, "grid2grid_top", s);
   for (int l = L - 1; l >= 1; --l) {
     t = T.open(s);
+    // the finest level also writes the grid-energy partials (sum q1 e1)
     msm::launch_prolong_sum(E.P, E.gd[l], E.gd[l + 1], E.g2g[pr].lv[l - 1].G, E.d_part[pr][l],
-                            E.d_el[l + 1], E.d_el[l], s);
+                            E.d_el[l + 1], E.d_el[l], l == 1 ? E.d_ql[1] : nullptr,
+                            l == 1 ? E.d_egrid : nullptr, s);
     T.close(t, "prolongation", s);
   }
 }
 
-void stage_energy(Engine &E, int layout, hipStream_t s) {
-  msm::launch_grid_energy(E.P, E.gd[1], E.d_ql[1], E.d_el[1], E.d_egrid,
-                          layout == MSM_LAYOUT_ASIS && E.plan.L == 1, s);
-  msm::launch_finalize(E.P, E.d_epair, E.P.ncell, E.d_egrid,
-                       msm::grid_blocks(E.gd[1].size()), E.d_scal, E.d_out, s);
+void stage_grids(Engine &E, int precision, int layout, Timing &T, hipStream_t s) {
+  stage_charges(E, T, s);
+  stage_potentials(E, precision, layout, T, s);
+}
+
+// Grid energy: with L >= 2 the level-1 prolongation has written one partial
+// of sum q1 e1 per workgroup; with L = 1 a separate pass does (it also
+// carries the reference HEAD's L = 1 index map for the as-is layout).
+int grid_energy_parts(const Engine &E) {
+  return E.plan.L >= 2 ? msm::prolong_blocks(E.gd[1]) : msm::grid_blocks(E.gd[1].size());
+}
+
+msm::FinalizeArgs finalize_args(Engine &E) {
+  return msm::FinalizeArgs{E.d_epair, E.P.ncell, E.d_egrid, grid_energy_parts(E), E.d_scal,
+                           E.d_flags, E.d_out, E.d_done};
+}
+
+// Grid-energy partials for L = 1 (the level-1 prolongation writes them
+// otherwise); part of the chain, before the join with the pair stream.
+void stage_grid_energy(Engine &E, int layout, hipStream_t s) {
+  if (E.plan.L == 1)
+    msm::launch_grid_energy(E.P, E.gd[1], E.d_ql[1], E.d_el[1], E.d_egrid,
+                            layout == MSM_LAYOUT_ASIS, s);
+}
+
+// After the join: with F or phi wanted, add the short-range parts (written
+// by the pair kernel into d_Fp / d_phip) and reduce the energy in the same
+// launch; otherwise only the reduction.
+void stage_energy(Engine &E, double *F, double *phi, hipStream_t s) {
+  if (F || phi)
+    msm::launch_combine(E.P, E.d_Fp, E.d_phip, F, phi, E.d_ep2, finalize_args(E), s);
+  else
+    msm::launch_finalize(E.P, finalize_args(E), s);
 }
 
 int check_built(FF *ff) {
@@ -492,15 +545,23 @@ int eval_chain(FFX *x, const double *pos, const double *q, double *out, double *
   int t = T.open(s);
   stage_prepare(E, pos, q, s);
   T.close(t, "nlist", s);
-  TRY(fork_pairs(E, x->precision, F, phi, s, T));
-  stage_grids(E, x->precision, x->layout, T, s);
+  // the pair kernel runs beside the whole grid chain including the
+  // interpolation (the chain's kernels are sized -- registers, LDS -- to fit
+  // next to its resident wavefronts); it writes the short-range parts of F
+  // and phi to its own buffers, added after the join
+  TRY(fork_pairs(E, x->precision, F ? E.d_Fp : nullptr, phi ? E.d_phip : nullptr, s, T));
+  stage_charges(E, T, s);
+  stage_potentials(E, x->precision, x->layout, T, s);
+  stage_grid_energy(E, x->layout, s);
+  if (F || phi) {
+    t = T.open(s);
+    msm::launch_gather(E.P, E.gd[1], E.d_spos, E.d_sidx, E.d_el[1], E.d_scal, F, phi, false, s);
+    T.close(t, "interpolation", s);
+  }
   HIP_TRY(hipStreamWaitEvent(s, E.ev_join, 0));
   t = T.open(s);
-  stage_energy(E, x->layout, s);
+  stage_energy(E, F, phi, s);
   T.close(t, "energy", s);
-  t = T.open(s);
-  if (F || phi) msm::launch_gather(E.P, E.gd[1], E.d_spos, E.d_sidx, E.d_el[1], E.d_scal, F, phi, s);
-  T.close(t, "interpolation", s);
   if (out != E.d_out) HIP_TRY(hipMemcpyAsync(out, E.d_out, 4 * sizeof(double), hipMemcpyDeviceToDevice, s));
   HIP_TRY(hipGetLastError());
   if (T.on) ++E.timed_evals;
@@ -597,6 +658,7 @@ int count_pairs_dev(FFX *x, const double *d_pos, const double *d_q, long long *c
   stage_prepare(E, d_pos, d_q, s);
   HIP_TRY(hipMemsetAsync(E.d_count, 0, sizeof(unsigned long long), s));
   msm::launch_pair_count(E.P, E.d_spos, E.d_scell, E.d_start, E.d_sten, E.nsten, E.d_count, s);
+  msm::launch_fill_int(E.d_flags, 1, 0, s);  // no energy reduction follows to clear it
   unsigned long long c = 0;
   HIP_TRY(hipMemcpyAsync(&c, E.d_count, sizeof c, hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
@@ -674,13 +736,12 @@ int energy_host(FF *ff, double (*force)[3], double (*position)[3], double *phi_o
     return rc;
   }
   size_t N = E.plan.N;
-  double out[4];
-  int flags = 0;
+  double out[5];  // {E, const, pair, grid, non-finite input count}
   HIP_TRY(hipMemcpyAsync(force, E.d_F, 3 * N * sizeof(double), hipMemcpyDeviceToHost, E.stream));
   if (phi_out) HIP_TRY(hipMemcpyAsync(phi_out, E.d_phi, N * sizeof(double), hipMemcpyDeviceToHost, E.stream));
   HIP_TRY(hipMemcpyAsync(out, E.d_out, sizeof out, hipMemcpyDeviceToHost, E.stream));
-  HIP_TRY(hipMemcpyAsync(&flags, E.d_flags, sizeof flags, hipMemcpyDeviceToHost, E.stream));
   HIP_TRY(hipStreamSynchronize(E.stream));
+  const int flags = out[4] != 0.0;
   publish_timing(x, T);
   T.clear();
   E.timed_evals = 0;
@@ -952,7 +1013,7 @@ int msm_eval_device(FF *ff, const double *d_pos, const double *d_q, double *d_en
   TRY(check_built(ff));
   if (!d_pos || !d_q || !d_energy) return fail(MSM_ERR_ARG, "msm_eval_device: NULL input");
   FFX *x = X(ff);
-  hipStream_t s = stream ? (hipStream_t)stream : x->eng->stream;
+  hipStream_t s = (hipStream_t)stream;  // NULL: the default stream, as in HIP
   Timing &T = x->eng->T;
   T.on = x->timing != 0;
   TRY(eval_chain(x, d_pos, d_q, d_energy, d_force, d_phi, s, T));
@@ -986,13 +1047,15 @@ int msm_forward_device(FF *ff, const double *d_pos, const double *d_q, double *d
   if (!d_pos || !d_q || !d_energy) return fail(MSM_ERR_ARG, "msm_forward_device: NULL input");
   FFX *x = X(ff);
   Engine &E = *x->eng;
-  hipStream_t s = stream ? (hipStream_t)stream : E.stream;
+  hipStream_t s = (hipStream_t)stream;  // NULL: the default stream, as in HIP
   Timing T;
   stage_prepare(E, d_pos, d_q, s);
   TRY(fork_pairs(E, x->precision, nullptr, nullptr, s, T));
-  stage_grids(E, x->precision, x->layout, T, s);
+  stage_charges(E, T, s);
+  stage_potentials(E, x->precision, x->layout, T, s);
+  stage_grid_energy(E, x->layout, s);
   HIP_TRY(hipStreamWaitEvent(s, E.ev_join, 0));
-  stage_energy(E, x->layout, s);
+  stage_energy(E, nullptr, nullptr, s);
   HIP_TRY(hipMemcpyAsync(d_energy, E.d_out, 4 * sizeof(double), hipMemcpyDeviceToDevice, s));
   HIP_TRY(hipGetLastError());
   E.kept = keep != 0;
@@ -1007,16 +1070,18 @@ int msm_adjoint_device(FF *ff, const double *d_pos, const double *d_q, double *d
   if (!d_pos || !d_q) return fail(MSM_ERR_ARG, "msm_adjoint_device: NULL input");
   FFX *x = X(ff);
   Engine &E = *x->eng;
-  hipStream_t s = stream ? (hipStream_t)stream : E.stream;
+  hipStream_t s = (hipStream_t)stream;  // NULL: the default stream, as in HIP
   Timing T;
   if (recompute || !E.kept || E.kept_pos != d_pos || E.kept_q != d_q) {
     // grid levels are not stored: rebuild the ordering and e^1 from scratch
     stage_prepare(E, d_pos, d_q, s);
+    msm::launch_fill_int(E.d_flags, 1, 0, s);  // no energy reduction follows to clear it
     stage_grids(E, x->precision, x->layout, T, s);
   }
   stage_pairs(E, x->precision, d_force, d_phi, s);  // the pair kernel writes, gather adds
   if (d_force || d_phi)
-    msm::launch_gather(E.P, E.gd[1], E.d_spos, E.d_sidx, E.d_el[1], E.d_scal, d_force, d_phi, s);
+    msm::launch_gather(E.P, E.gd[1], E.d_spos, E.d_sidx, E.d_el[1], E.d_scal, d_force, d_phi,
+                       true, s);
   HIP_TRY(hipGetLastError());
   return MSM_OK;
 }

@@ -37,20 +37,23 @@ struct GridDims {
 void launch_fill_int(int *p, long n, int v, hipStream_t s);
 void launch_fill_dbl(double *p, long n, double v, hipStream_t s);
 
-// Binning: wrap positions into the box, count atoms per cell, partial sums
-// of q and q^2 per block (2 doubles per block into qpart).
+// Binning: wrap positions into the box, count atoms per cell (cnt must be
+// zero on entry; rank = arrival order within the cell), partial sums of q
+// and q^2 per block (2 doubles per block into qpart), non-finite inputs
+// counted in flags[0].
 int bin_blocks(int N);
-void launch_bin(const DevParams &P, const double *pos, const double *q, int *cell_of, int *cnt,
-                double4 *wpos, double *qpart, int *flags, hipStream_t s);
-// Exclusive scan of cnt into start[0..ncell] (cnt is zeroed for reuse as fill
-// counters) and reduction of qpart into scal[0] = sum q, scal[1] = sum q^2.
-void launch_scan(const DevParams &P, int *cnt, int *start, const double *qpart, int nqpart,
+void launch_bin(const DevParams &P, const double *pos, const double *q, int *cell_of, int *rank,
+                int *cnt, double4 *wpos, double *qpart, int *flags, hipStream_t s);
+// Exclusive scan of cnt into start[0..ncell] and reduction of qpart into
+// scal[0] = sum q, scal[1] = sum q^2.
+void launch_scan(const DevParams &P, const int *cnt, int *start, const double *qpart, int nqpart,
                  double *scal, hipStream_t s);
-void launch_scatter(const DevParams &P, const int *cell_of, const double4 *wpos, const int *start,
-                    int *fill, double4 *spos, int *sidx, int *scell, hipStream_t s);
-// Orders atoms inside each cell by original index (deterministic results)
-// and writes the cell-local FP32 coordinates used by the mixed pair kernel.
-void launch_cell_sort(const DevParams &P, const int *start, double4 *spos, int *sidx,
+void launch_scatter(const DevParams &P, const int *cell_of, const int *rank, const double4 *wpos,
+                    const int *start, double4 *spos, int *sidx, int *scell, hipStream_t s);
+// Orders atoms inside each cell by original index (deterministic results),
+// writes the cell-local FP32 coordinates used by the mixed pair kernel and
+// clears cnt for the next binning.
+void launch_cell_sort(const DevParams &P, const int *start, int *cnt, double4 *spos, int *sidx,
                       float4 *lpos, hipStream_t s);
 
 // Short-range pairs, full neighbour shell, one wavefront per home cell.
@@ -111,20 +114,41 @@ size_t g2g_lds_bytes(int prec, const G2GLevel &L);
 int g2g_max_nch(int prec);
 // All levels' stencil correlations in one launch (one wavefront per task).
 void launch_g2g_tiles(int prec, G2GArgs args, size_t lds, hipStream_t s);
-// e^l = sum of the G partials + prolongation of e^{l+1}.
+// e^l = sum of the G partials + prolongation of e^{l+1}; with q1 != null
+// also one partial of sum q1 e1 per workgroup into epart (prolong_blocks(g)).
+int prolong_blocks(GridDims g);
 void launch_prolong_sum(const DevParams &P, GridDims g, GridDims gc, int G, const double *part,
-                        const double *ec, double *el, hipStream_t s);
+                        const double *ec, double *el, const double *q1, double *epart,
+                        hipStream_t s);
 
 // Grid energy partials: sum q1*e1 per block (asis != 0: the reference
 // HEAD's L = 1 index map).
 int grid_blocks(long n);
 void launch_grid_energy(const DevParams &P, GridDims g, const double *q1, const double *e1,
                         double *epart, int asis, hipStream_t s);
-// Interpolation: F += q E and phi += grid potential + constant self terms.
+// Final energy reduction: out = {E, constant part, pair part, grid part,
+// non-finite input count}; flags[0] is cleared.  `done` is a zero-initialised
+// counter used when the reduction rides on the interpolation launch.
+struct FinalizeArgs {
+  const double *ep;
+  int np;
+  const double *eg;
+  int ng;
+  const double *scal;
+  int *flags;
+  double *out;  // >= 5 doubles; null: no reduction
+  unsigned *done;
+};
+// Interpolation: F (=, or += with add) q E and phi (=, +=) grid potential
+// + constant self terms.
 void launch_gather(const DevParams &P, GridDims g, const double4 *spos, const int *sidx,
-                   const double *e1, const double *scal, double *F, double *phi, hipStream_t s);
-// Final energy: out = {E, constant part, pair part, grid part}.
-void launch_finalize(const DevParams &P, const double *epair, int npair, const double *egrid,
-                     int ngrid, const double *scal, double *out, hipStream_t s);
+                   const double *e1, const double *scal, double *F, double *phi, bool add,
+                   hipStream_t s);
+void launch_finalize(const DevParams &P, const FinalizeArgs &fin, hipStream_t s);
+// F += Fp, phi += phip (either may be null) and the energy reduction; ep2
+// holds 2 doubles per workgroup (combine_blocks(N)).
+int combine_blocks(int N);
+void launch_combine(const DevParams &P, const double *Fp, const double *phip, double *F,
+                    double *phi, double *ep2, const FinalizeArgs &fin, hipStream_t s);
 
 }  // namespace msm

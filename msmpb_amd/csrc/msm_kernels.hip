@@ -94,8 +94,8 @@ constexpr int kBinBS = 256;
 
 __global__ __launch_bounds__(kBinBS) void k_bin(DevParams P, const double *__restrict__ pos,
                                                 const double *__restrict__ q,
-                                                int *__restrict__ cell_of, int *__restrict__ cnt,
-                                                double4 *__restrict__ wpos,
+                                                int *__restrict__ cell_of, int *__restrict__ rank,
+                                                int *__restrict__ cnt, double4 *__restrict__ wpos,
                                                 double *__restrict__ qpart, int *__restrict__ flags) {
   int i = blockIdx.x * kBinBS + threadIdx.x;
   double qi = 0.0;
@@ -122,7 +122,7 @@ __global__ __launch_bounds__(kBinBS) void k_bin(DevParams P, const double *__res
     if (!ok) atomicAdd(flags, 1);
     int cell = (c[0] * P.nc[1] + c[1]) * P.nc[2] + c[2];
     cell_of[i] = cell;
-    atomicAdd(cnt + cell, 1);
+    rank[i] = atomicAdd(cnt + cell, 1);  // arrival order; the cell sort fixes the order
     wpos[i] = make_double4(rw[0], rw[1], rw[2], qi);
   }
   double s1 = block_sum<kBinBS>(qi);
@@ -136,32 +136,53 @@ __global__ __launch_bounds__(kBinBS) void k_bin(DevParams P, const double *__res
 
 constexpr int kScanBS = 1024;
 
-__global__ __launch_bounds__(kScanBS) void k_scan(int n, int *__restrict__ cnt,
+// Exclusive scan of the cell counts (start[0..n]) in tiles of 8 elements per
+// thread: a per-thread prefix, a wavefront shuffle scan of the thread totals
+// and a scan of the 16 wavefront totals; the counts are left in place (the
+// cell sort clears them for the next evaluation).  Also reduces the per-block
+// charge sums of the binning pass.
+__global__ __launch_bounds__(kScanBS) void k_scan(int n, const int *__restrict__ cnt,
                                                   int *__restrict__ start,
                                                   const double *__restrict__ qpart, int nq,
                                                   double *__restrict__ scal) {
-  __shared__ int part[kScanBS];
-  int t = threadIdx.x;
-  int chunk = (n + kScanBS - 1) / kScanBS;
-  int b = t * chunk, e = min(n, b + chunk);
-  int s = 0;
-  for (int k = b; k < e; ++k) s += cnt[k];
-  part[t] = s;
+  constexpr int PER = 8, NW = kScanBS / 64;
+  __shared__ int wtot[NW];
+  __shared__ int carry_s;
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  if (t == 0) carry_s = 0;
   __syncthreads();
-  for (int off = 1; off < kScanBS; off <<= 1) {  // inclusive Hillis-Steele scan
-    int v = t >= off ? part[t - off] : 0;
+  for (int base = 0; base < n; base += kScanBS * PER) {
+    const int k0 = base + t * PER;
+    int v[PER], run = 0;
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      v[u] = k0 + u < n ? cnt[k0 + u] : 0;
+      run += v[u];
+    }
+    int incl = run;
+    for (int off = 1; off < 64; off <<= 1) {
+      int o = __shfl_up(incl, off, 64);
+      if (lane >= off) incl += o;
+    }
+    if (lane == 63) wtot[w] = incl;
     __syncthreads();
-    part[t] += v;
+    int woff = 0, tile = 0;
+#pragma unroll
+    for (int j = 0; j < NW; ++j) {
+      woff += j < w ? wtot[j] : 0;
+      tile += wtot[j];
+    }
+    int pre = carry_s + woff + incl - run;
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      if (k0 + u < n) start[k0 + u] = pre;
+      pre += v[u];
+    }
+    __syncthreads();  // everyone has read wtot and carry_s
+    if (t == 0) carry_s += tile;
     __syncthreads();
   }
-  int run = part[t] - s;
-  for (int k = b; k < e; ++k) {
-    int c = cnt[k];
-    start[k] = run;
-    run += c;
-    cnt[k] = 0;
-  }
-  if (t == kScanBS - 1) start[n] = part[t];
+  if (t == 0) start[n] = carry_s;
   // charge sums for the constant energy terms
   double q1 = 0, q2 = 0;
   for (int k = t; k < nq; k += kScanBS) q1 += qpart[2 * k], q2 += qpart[2 * k + 1];
@@ -173,16 +194,16 @@ __global__ __launch_bounds__(kScanBS) void k_scan(int n, int *__restrict__ cnt,
 }
 
 __global__ __launch_bounds__(256) void k_scatter(int N, const int *__restrict__ cell_of,
+                                                 const int *__restrict__ rank,
                                                  const double4 *__restrict__ wpos,
                                                  const int *__restrict__ start,
-                                                 int *__restrict__ fill,
                                                  double4 *__restrict__ spos,
                                                  int *__restrict__ sidx,
                                                  int *__restrict__ scell) {
   int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= N) return;
   int c = cell_of[i];
-  int slot = start[c] + atomicAdd(fill + c, 1);
+  int slot = start[c] + rank[i];
   spos[slot] = wpos[i];
   sidx[slot] = i;
   scell[slot] = c;
@@ -192,12 +213,14 @@ __global__ __launch_bounds__(256) void k_scatter(int N, const int *__restrict__ 
 // order, and every sum built on it, is deterministic), moves them in place
 // and writes the cell-local FP32 coordinates used by the mixed pair kernel.
 __global__ __launch_bounds__(256) void k_cell_sort(DevParams P, const int *__restrict__ start,
+                                                   int *__restrict__ cnt,
                                                    double4 *__restrict__ spos,
                                                    int *__restrict__ sidx,
                                                    float4 *__restrict__ lpos) {
   int c = blockIdx.x * 4 + (threadIdx.x >> 6);
   int lane = threadIdx.x & 63;
   if (c >= P.ncell) return;
+  if (lane == 0) cnt[c] = 0;  // ready for the next evaluation's binning
   int b = start[c], n = start[c + 1] - b;
   int cz = c % P.nc[2], cy = (c / P.nc[2]) % P.nc[1], cx = c / (P.nc[1] * P.nc[2]);
   double ox = cx * P.cw[0], oy = cy * P.cw[1], oz = cz * P.cw[2];
@@ -594,14 +617,74 @@ __device__ __forceinline__ double4 spread_fetch(const int *roff, const int *rbeg
   return spos[rbeg[lo] + (k - roff[lo])];
 }
 
+// An atom that reaches the brick, queued until 64 of them fill a wavefront:
+// fractional offsets t and charge, and the brick-relative index of the
+// lowest support point per axis.
+struct SpreadItem {
+  double t[3];
+  double q;
+  short rlo[3];  // grid dims stay below 2^15
+  short pad;
+};
+
+// Folded weights of one atom (the wavefront's 64 queued atoms, one per lane)
+// into its records, then Q += WX^T (WY x WZ) by 16 MFMAs.  `valid` lanes
+// hold an atom; the others leave zero records.
+template <int NU>
+__device__ __forceinline__ void spread_flush(const DevParams &P, const SpreadItem &it, bool valid,
+                                             int natoms, const int dims[3], double *RX,
+                                             double *RY, double *RZ, int lane,
+                                             sp_f64x4 &acc) {
+  constexpr int ext[3] = {kSX, kSY, kSZ};
+  double *rec[3] = {RX + lane * kWXs, RY + lane * kWYs, RZ + lane * kWYs};
+#pragma unroll
+  for (int u = 0; u < kSX; ++u) rec[0][u] = 0.0;
+#pragma unroll
+  for (int u = 0; u < kSY; ++u) rec[1][u] = 0.0, rec[2][u] = 0.0;
+  if (valid) {
+    double w[NU];
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax) {
+      bspline<NU, false>(P.Q, it.t[ax], w, nullptr);
+      const double sc = ax == 1 ? it.q : 1.0;
+      const int M = dims[ax];
+      if (M >= NU) {  // the support points are distinct grid points
+#pragma unroll
+        for (int i = 0; i < NU; ++i) {  // weight i sits at point m + NU/2 - i
+          int rr = it.rlo[ax] + (NU - 1 - i);
+          rr -= rr >= M ? M : 0;
+          if (rr < ext[ax]) rec[ax][rr] = sc * w[i];
+        }
+      } else {  // grids narrower than the support: points repeat, accumulate
+        for (int i = 0; i < NU; ++i) {
+          int rr = (it.rlo[ax] + (NU - 1 - i)) % M;
+          if (rr < ext[ax]) rec[ax][rr] += sc * w[i];
+        }
+      }
+    }
+  }
+  // LDS operations of one wavefront complete in order; the barrier keeps the
+  // compiler from moving the reads above the writes
+  __builtin_amdgcn_wave_barrier();
+  const int row = lane & 15, kq = lane >> 4;
+  for (int a0 = 0; a0 < natoms; a0 += 4) {
+    const int a = a0 + kq;
+    const double av = RX[a * kWXs + row];
+    const double bv = RY[a * kWYs + (row >> 2)] * RZ[a * kWYs + (row & 3)];
+    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
+  }
+  __builtin_amdgcn_wave_barrier();
+}
+
 template <int NU>
 __global__ __launch_bounds__(256) void k_spread_mfma(DevParams P, GridDims g,
                                                      const double4 *__restrict__ spos,
                                                      const int *__restrict__ start,
                                                      double *__restrict__ q1) {
-  // per-wavefront records of 64 staged atoms
+  // per wavefront: records of 64 atoms and a queue of up to 127 reaching atoms
   __shared__ double WX[256 * kWXs];
   __shared__ double WY[256 * kWYs], WZ[256 * kWYs];
+  __shared__ SpreadItem queue[4 * 128];
   __shared__ int rbeg[kSRanges], roff[kSRanges + 1];
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int nbz = (g.z + kSZ - 1) / kSZ, nby = (g.y + kSY - 1) / kSY;
@@ -618,7 +701,8 @@ __global__ __launch_bounds__(256) void k_spread_mfma(DevParams P, GridDims g,
   const unsigned long long below = (1ull << lane) - 1ull;
   double *const RX = WX + wv * 64 * kWXs, *const RY = WY + wv * 64 * kWYs,
                *const RZ = WZ + wv * 64 * kWYs;
-  const int row = lane & 15, kq = lane >> 4;
+  SpreadItem *const Qw = queue + wv * 128;
+  int npend = 0;  // queued atoms of this wavefront (uniform)
   sp_f64x4 acc = {0.0, 0.0, 0.0, 0.0};
   for (int c0 = 0; c0 < ncol; c0 += kSRanges / 2) {
     const int nc_b = min(ncol - c0, kSRanges / 2);
@@ -665,65 +749,46 @@ __global__ __launch_bounds__(256) void k_spread_mfma(DevParams P, GridDims g,
     double4 nxt = spread_fetch(roff, rbeg, nr, k, total, lo, spos);
     for (; k - lane < total; k += 256) {
       const double4 at = nxt;
-      const bool valid = k < total;
       nxt = spread_fetch(roff, rbeg, nr, k + 256, total, lo, spos);
-      // ---- which atoms reach the brick, and where their support starts
-      bool reach = valid;
-      double tt[3];
-      int rlo[3];
+      SpreadItem it;
+      bool reach = k < total;
       const double r[3] = {at.x, at.y, at.z};
 #pragma unroll
       for (int ax = 0; ax < 3; ++ax) {
         double sv = r[ax] * P.ibox[ax];
         sv = sv - floor(sv);
-        double t = dims[ax] * sv, m = floor(t);
-        tt[ax] = t - m;
-        // brick-relative index of the lowest support point m - NU/2 + 1
-        rlo[ax] = wrapi((int)m - NU / 2 + 1 - org[ax], dims[ax]);
-        reach = reach && (rlo[ax] < ext[ax] || rlo[ax] > dims[ax] - NU);
+        const double t = dims[ax] * sv, m = floor(t);
+        it.t[ax] = t - m;
+        // brick-relative index of the lowest support point m - NU/2 + 1:
+        // m lies in [0, M] and org in [0, M), so one wrap suffices (NU >= 4)
+        int v = (int)m - NU / 2 + 1 - org[ax];
+        v += v < 0 ? dims[ax] : 0;
+        it.rlo[ax] = (short)v;
+        reach = reach && (v < ext[ax] || v > dims[ax] - NU);
       }
+      it.q = at.w;
       const unsigned long long bm = __ballot(reach);
-      const int nrel = __popcll(bm), npad = (nrel + 3) & ~3;
-      if (nrel == 0) continue;  // wavefront-uniform
-      const int slot = reach ? __popcll(bm & below) : -1;
-      // clear this lane's record and, independently, the padding record with
-      // this lane's index (slots [nrel, npad) hold no atom)
-#pragma unroll
-      for (int pass = 0; pass < 2; ++pass) {
-        const int sl = pass == 0 ? slot : ((lane >= nrel && lane < npad) ? lane : -1);
-        if (sl < 0) continue;
-#pragma unroll
-        for (int u = 0; u < kSX; ++u) RX[sl * kWXs + u] = 0.0;
-#pragma unroll
-        for (int u = 0; u < kSY; ++u) RY[sl * kWYs + u] = 0.0, RZ[sl * kWYs + u] = 0.0;
-      }
-      if (reach) {
-        double w[NU];
-        double *dst[3] = {RX + slot * kWXs, RY + slot * kWYs, RZ + slot * kWYs};
-        const double scale[3] = {1.0, at.w, 1.0};
-#pragma unroll
-        for (int ax = 0; ax < 3; ++ax) {
-          bspline<NU, false>(P.Q, tt[ax], w, nullptr);
-#pragma unroll
-          for (int i = 0; i < NU; ++i) {  // weight i sits at point m + NU/2 - i
-            int rr = rlo[ax] + (NU - 1 - i);
-            while (rr >= dims[ax]) rr -= dims[ax];
-            if (rr < ext[ax]) dst[ax][rr] += scale[ax] * w[i];
-          }
+      const int nrel = __popcll(bm);
+      if (reach) Qw[npend + __popcll(bm & below)] = it;
+      npend += nrel;
+      if (npend >= 64) {
+        __builtin_amdgcn_wave_barrier();
+        const SpreadItem mine = Qw[lane];
+        spread_flush<NU>(P, mine, true, 64, dims, RX, RY, RZ, lane, acc);
+        // move the leftover queue entries [64, npend) to the front
+        if (64 + lane < npend) {
+          const SpreadItem rest = Qw[64 + lane];
+          Qw[lane] = rest;
         }
+        npend -= 64;
+        __builtin_amdgcn_wave_barrier();
       }
-      // LDS operations of one wavefront complete in order; the barrier keeps
-      // the compiler from moving the reads above the writes
-      __builtin_amdgcn_wave_barrier();
-      // ---- Q[x][(y, z)] += WX[a][x] * WY[a][y] * WZ[a][z], four atoms per MFMA
-      for (int a0 = 0; a0 < npad; a0 += 4) {
-        const int a = a0 + kq;
-        const double av = RX[a * kWXs + row];
-        const double bv = RY[a * kWYs + (row >> 2)] * RZ[a * kWYs + (row & 3)];
-        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
-      }
-      __builtin_amdgcn_wave_barrier();
     }
+  }
+  if (npend > 0) {  // the last, partial wavefront of atoms
+    __builtin_amdgcn_wave_barrier();
+    const SpreadItem mine = Qw[lane];
+    spread_flush<NU>(P, mine, lane < npend, (npend + 3) & ~3, dims, RX, RY, RZ, lane, acc);
   }
   // ---- sum the four wavefronts' tiles in a fixed order and store
   __syncthreads();
@@ -746,30 +811,61 @@ __global__ __launch_bounds__(256) void k_spread_mfma(DevParams P, GridDims g,
 
 // ---------------------------------------------------------------- restrict
 // restrict_ (forcefield1.c:401-477): q^l_m = sum_n J_nx J_ny J_nz
-// q^{l-1}_{2m+n}, n in [-NU/2, NU/2]^3, periodic.
-// One wavefront per coarse point: lane k takes tap pairs (nx, ny) = k, k+64,
-// ... of the (NU+1)^2 and sums its NU+1 z taps; a wavefront reduction adds
-// them (plenty of parallelism even for the small coarse grids).
+// q^{l-1}_{2m+n}, n in [-NU/2, NU/2]^3, periodic, as separable passes like
+// the reference's z -> y -> x (:442-472).  A workgroup owns a 2^3 coarse
+// brick: it loads the (NU+3)^3 fine values the brick reads into LDS, then
+// contracts z, y and x in turn.  The small brick keeps the LDS footprint
+// (7.4 KB at NU = 6) low enough to run beside the pair kernel.
+constexpr int kRB = 2;
+
 template <int NU>
 __global__ __launch_bounds__(256) void k_restrict(DevParams P, GridDims f, GridDims c,
                                                   const double *__restrict__ qf,
                                                   double *__restrict__ qc) {
-  long id = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
-  int lane = threadIdx.x & 63;
-  if (id >= c.size()) return;
-  int z = id % c.z, y = (id / c.z) % c.y, x = id / ((long)c.z * c.y);
-  const int H = NU / 2, T = NU + 1;
-  double acc = 0.0;
-  for (int k = lane; k < T * T; k += 64) {
-    int nx = k / T, ny = k - nx * T;
-    const double *row = qf + ((long)wrapi(2 * x + nx - H, f.x) * f.y + wrapi(2 * y + ny - H, f.y)) * f.z;
-    double sz = 0.0;
-#pragma unroll
-    for (int nz = 0; nz <= NU; ++nz) sz += P.J[nz] * row[wrapi(2 * z + nz - H, f.z)];
-    acc += P.J[nx] * P.J[ny] * sz;
+  // coarse brick of kRB^3 points; fine indices 2m + n - H, n in [0, NU]
+  constexpr int H = NU / 2, FW = 2 * (kRB - 1) + NU + 1;
+  __shared__ double fs[FW * FW * FW];
+  __shared__ double t1[FW * FW * kRB];    // [fx][fy][cz]
+  __shared__ double t2[FW * kRB * kRB];   // [fx][cy][cz]
+  const int tid = threadIdx.x;
+  const int nbz = (c.z + kRB - 1) / kRB, nby = (c.y + kRB - 1) / kRB;
+  int b = blockIdx.x;
+  const int cz0 = (b % nbz) * kRB;
+  b /= nbz;
+  const int cy0 = (b % nby) * kRB, cx0 = (b / nby) * kRB;
+  const int fx0 = 2 * cx0 - H, fy0 = 2 * cy0 - H, fz0 = 2 * cz0 - H;
+  for (int k = tid; k < FW * FW * FW; k += 256) {
+    const int a = k / (FW * FW), bb = (k / FW) % FW, cc = k % FW;
+    fs[k] = qf[((long)wrapi(fx0 + a, f.x) * f.y + wrapi(fy0 + bb, f.y)) * f.z + wrapi(fz0 + cc, f.z)];
   }
-  for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
-  if (lane == 0) qc[id] = acc;
+  __syncthreads();
+  for (int k = tid; k < FW * FW * kRB; k += 256) {  // along z
+    const int cz = k % kRB, fxy = k / kRB;
+    const double *row = fs + fxy * FW + 2 * cz;
+    double acc = 0.0;
+#pragma unroll
+    for (int n = 0; n <= NU; ++n) acc += P.J[n] * row[n];
+    t1[k] = acc;
+  }
+  __syncthreads();
+  for (int k = tid; k < FW * kRB * kRB; k += 256) {  // along y
+    const int cz = k % kRB, cy = (k / kRB) % kRB, fx = k / (kRB * kRB);
+    double acc = 0.0;
+#pragma unroll
+    for (int n = 0; n <= NU; ++n) acc += P.J[n] * t1[(fx * FW + 2 * cy + n) * kRB + cz];
+    t2[k] = acc;
+  }
+  __syncthreads();
+  if (tid < kRB * kRB * kRB) {  // along x
+    const int cz = tid % kRB, cy = (tid / kRB) % kRB, cx = tid / (kRB * kRB);
+    const int x = cx0 + cx, y = cy0 + cy, z = cz0 + cz;
+    if (x < c.x && y < c.y && z < c.z) {
+      double acc = 0.0;
+#pragma unroll
+      for (int n = 0; n <= NU; ++n) acc += P.J[n] * t2[((2 * cx + n) * kRB + cy) * kRB + cz];
+      qc[((long)x * c.y + y) * c.z + z] = acc;
+    }
+  }
 }
 
 // ---------------------------------------------------------------- top level
@@ -786,33 +882,36 @@ __device__ __forceinline__ double top_input(GridDims g, const double *qL, int x,
   return f < g.size() ? qL[f] : 0.0;
 }
 
-constexpr int kTopBS = 1024;
-constexpr int kTopSmall = 2048;  // largest M^3 done in one workgroup (LDS)
+// One workgroup of 256 threads with LDS sized to the grid (32 bytes per
+// point), so that it finds room beside the concurrently running pair kernel.
+constexpr int kTopBS = 256;
+constexpr int kTopSmall = 2048;  // largest M^3 done in one workgroup (64 KB of LDS)
 
-__device__ void dft_pass_lds(GridDims g, int axis, double sign, double *re, double *im,
-                             const double *tw) {
-  int M = axis == 0 ? g.x : axis == 1 ? g.y : g.z;
+// One DFT pass along `axis` from (ire, iim) into (ore, oim), all in LDS.
+__device__ void dft_pass_lds(GridDims g, int axis, double sign, const double *ire,
+                             const double *iim, double *ore, double *oim, const double *tw) {
+  // (kept lean in registers: it runs beside the pair kernel's wavefronts)
+  const int M = axis == 0 ? g.x : axis == 1 ? g.y : g.z;
   const double *c = tw + 2 * (axis == 0 ? 0 : axis == 1 ? g.x : g.x + g.y);
-  long stride = axis == 0 ? (long)g.y * g.z : axis == 1 ? g.z : 1;
-  long n = g.size();
-  double orr[kTopSmall / kTopBS], oii[kTopSmall / kTopBS];
-  int cnt = 0;
-  for (long e = threadIdx.x; e < n; e += kTopBS, ++cnt) {
-    int k = axis == 0 ? e / ((long)g.y * g.z) : axis == 1 ? (e / g.z) % g.y : e % g.z;
-    long base = e - k * stride;
+  const int stride = axis == 0 ? g.y * g.z : axis == 1 ? g.z : 1;
+  const int n = (int)g.size();
+#pragma unroll 1
+  for (int e = threadIdx.x; e < n; e += kTopBS) {
+    const int k = axis == 0 ? e / (g.y * g.z) : axis == 1 ? (e / g.z) % g.y : e % g.z;
+    const int base = e - k * stride;
     double sr = 0, si = 0;
-    for (int m = 0; m < M; ++m) {
-      int km = (int)(((long)k * m) % M);
-      double cr = c[2 * km], ci = sign * c[2 * km + 1];
-      double xr = re[base + m * stride], xi = im[base + m * stride];
+    int km = 0;  // k m mod M
+#pragma unroll 1
+    for (int m = 0, o = base; m < M; ++m, o += stride) {
+      const double cr = c[2 * km], ci = sign * c[2 * km + 1];
+      const double xr = ire[o], xi = iim[o];
       sr += xr * cr - xi * ci;
       si += xr * ci + xi * cr;
+      km += k;
+      km -= km >= M ? M : 0;
     }
-    orr[cnt] = sr, oii[cnt] = si;
+    ore[e] = sr, oim[e] = si;
   }
-  __syncthreads();
-  cnt = 0;
-  for (long e = threadIdx.x; e < n; e += kTopBS, ++cnt) re[e] = orr[cnt], im[e] = oii[cnt];
   __syncthreads();
 }
 
@@ -821,22 +920,23 @@ __global__ __launch_bounds__(kTopBS) void k_top_small(DevParams P, GridDims g,
                                                       const double *__restrict__ khat,
                                                       const double *__restrict__ tw,
                                                       double *__restrict__ eL, int asis) {
-  __shared__ double re[kTopSmall], im[kTopSmall];
+  extern __shared__ double top_lds[];  // 4 n doubles: two complex buffers
   long n = g.size();
+  double *re = top_lds, *im = top_lds + n, *re2 = top_lds + 2 * n, *im2 = top_lds + 3 * n;
   for (long e = threadIdx.x; e < n; e += kTopBS) {
     int z = e % g.z, y = (e / g.z) % g.y, x = e / ((long)g.y * g.z);
     re[e] = top_input(g, qL, x, y, z, asis, P.nu);
     im[e] = 0.0;
   }
   __syncthreads();
-  dft_pass_lds(g, 2, -1.0, re, im, tw);
-  dft_pass_lds(g, 1, -1.0, re, im, tw);
-  dft_pass_lds(g, 0, -1.0, re, im, tw);
-  for (long e = threadIdx.x; e < n; e += kTopBS) re[e] *= khat[e], im[e] *= khat[e];
+  dft_pass_lds(g, 2, -1.0, re, im, re2, im2, tw);
+  dft_pass_lds(g, 1, -1.0, re2, im2, re, im, tw);
+  dft_pass_lds(g, 0, -1.0, re, im, re2, im2, tw);
+  for (long e = threadIdx.x; e < n; e += kTopBS) re2[e] *= khat[e], im2[e] *= khat[e];
   __syncthreads();
-  dft_pass_lds(g, 0, 1.0, re, im, tw);
-  dft_pass_lds(g, 1, 1.0, re, im, tw);
-  dft_pass_lds(g, 2, 1.0, re, im, tw);
+  dft_pass_lds(g, 0, 1.0, re2, im2, re, im, tw);
+  dft_pass_lds(g, 1, 1.0, re, im, re2, im2, tw);
+  dft_pass_lds(g, 2, 1.0, re2, im2, re, im, tw);
   for (long e = threadIdx.x; e < n; e += kTopBS) eL[e] = re[e] / (double)n;
 }
 
@@ -1016,17 +1116,84 @@ __global__ __launch_bounds__(kGridBS) void k_grid_energy(DevParams P, GridDims g
   if (threadIdx.x == 0) epart[blockIdx.x] = t;
 }
 
+__device__ __forceinline__ void neumaier(double &s, double &c, double v) {
+  double t = s + v;
+  if (fabs(s) >= fabs(v)) c += (s - t) + v;
+  else c += (v - t) + s;
+  s = t;
+}
+
+// Adds two compensated partial sums (s, c): error-free sum of the heads,
+// carries added.
+__device__ __forceinline__ void csum_add(double &s, double &c, double s2, double c2) {
+  double t = s + s2, bb = t - s;
+  double err = (s - (t - bb)) + (s2 - bb);
+  s = t;
+  c += c2 + err;
+}
+
+// Compensated sums of the per-cell pair partials and per-block grid partials:
+// each of 256 threads sums a fixed strided subset, then a fixed binary tree
+// combines the (sum, carry) pairs, so the result does not depend on
+// scheduling.  out = {E, constant part, pair part, grid part, flags}; the
+// non-finite-input counter is copied to out[4] and cleared for the next
+// evaluation.
+// Tree-combines the 256 threads' compensated pair sums (sv, cv) with the
+// grid partials and writes out.
+__device__ void finalize_tail(const DevParams &P, const FinalizeArgs &f, double sv, double cv) {
+  __shared__ double ss[2][256], cc[2][256];
+  int t = threadIdx.x;
+  ss[0][t] = sv, cc[0][t] = cv;
+  double s = 0, c = 0;
+  for (int k = t; k < f.ng; k += 256) neumaier(s, c, f.eg[k]);
+  ss[1][t] = s, cc[1][t] = c;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (t < h)
+      for (int w = 0; w < 2; ++w) {
+        double a = ss[w][t], b = cc[w][t];
+        csum_add(a, b, ss[w][t + h], cc[w][t + h]);
+        ss[w][t] = a, cc[w][t] = b;
+      }
+    __syncthreads();
+  }
+  if (t != 0) return;
+  double epair = ss[0][0] + cc[0][0], egrid = 0.5 * (ss[1][0] + cc[1][0]);
+  double qs = f.scal[0], q2 = f.scal[1];
+  // constant part (forcefield1.c:114-130)
+  double econst = 0.5 * q2 * P.coeff1 - 0.5 * (qs * qs) * P.coeff2 - 0.5 * q2 * P.gamma0 * P.inv_a0;
+  s = 0, c = 0;
+  neumaier(s, c, econst);
+  neumaier(s, c, epair);
+  neumaier(s, c, egrid);
+  f.out[0] = s + c;
+  f.out[1] = econst;
+  f.out[2] = epair;
+  f.out[3] = egrid;
+  f.out[4] = (double)*f.flags;
+  *f.flags = 0;
+}
+
+__device__ void finalize_body(const DevParams &P, const FinalizeArgs &f) {
+  double s = 0, c = 0;
+  for (int k = threadIdx.x; k < f.np; k += 256) neumaier(s, c, f.ep[k]);
+  finalize_tail(P, f, s, c);
+}
+
+__global__ __launch_bounds__(256) void k_finalize(DevParams P, FinalizeArgs f) {
+  finalize_body(P, f);
+}
+
 // interpolate (forcefield1.c:570-607) + F += q E (:228-231), and the
 // potential phi_i = dE/dq_i: grid part plus the constant self terms.
-template <int NU>
-__global__ __launch_bounds__(256) void k_gather(DevParams P, GridDims g,
-                                                const double4 *__restrict__ spos,
-                                                const int *__restrict__ sidx,
-                                                const double *__restrict__ e1,
-                                                const double *__restrict__ scal,
-                                                double *__restrict__ F, double *__restrict__ phi) {
-  int i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= P.N) return;
+template <int NU, bool ADD>
+__device__ __forceinline__ void gather_atom(const DevParams &P, GridDims g,
+                                            const double4 *__restrict__ spos,
+                                            const int *__restrict__ sidx,
+                                            const double *__restrict__ e1,
+                                            const double *__restrict__ scal,
+                                            double *__restrict__ F, double *__restrict__ phi,
+                                            int i) {
   double4 a = spos[i];
   double r[3] = {a.x, a.y, a.z};
   int dims[3] = {g.x, g.y, g.z};
@@ -1074,68 +1241,74 @@ __global__ __launch_bounds__(256) void k_gather(DevParams P, GridDims g,
   // E = -A^{-T} (M * grad_t), orthorhombic A^{-T} = diag(1/box)
   double Ex = -ex * g.x * P.ibox[0], Ey = -ey * g.y * P.ibox[1], Ez = -ez * g.z * P.ibox[2];
   int oi = sidx[i];
+  const double pv = ph + a.w * P.coeff_self - scal[0] * P.coeff2;
   if (F) {
-    F[3 * (long)oi] += a.w * Ex;
-    F[3 * (long)oi + 1] += a.w * Ey;
-    F[3 * (long)oi + 2] += a.w * Ez;
+    double *f = F + 3 * (long)oi;
+    f[0] = (ADD ? f[0] : 0.0) + a.w * Ex;
+    f[1] = (ADD ? f[1] : 0.0) + a.w * Ey;
+    f[2] = (ADD ? f[2] : 0.0) + a.w * Ez;
   }
-  if (phi) phi[oi] += ph + a.w * P.coeff_self - scal[0] * P.coeff2;
+  if (phi) phi[oi] = (ADD ? phi[oi] : 0.0) + pv;
 }
 
-__device__ __forceinline__ void neumaier(double &s, double &c, double v) {
-  double t = s + v;
-  if (fabs(s) >= fabs(v)) c += (s - t) + v;
-  else c += (v - t) + s;
-  s = t;
+template <int NU, bool ADD>
+__global__ __launch_bounds__(256) void k_gather(DevParams P, GridDims g,
+                                                const double4 *__restrict__ spos,
+                                                const int *__restrict__ sidx,
+                                                const double *__restrict__ e1,
+                                                const double *__restrict__ scal,
+                                                double *__restrict__ F, double *__restrict__ phi) {
+  int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < P.N) gather_atom<NU, ADD>(P, g, spos, sidx, e1, scal, F, phi, i);
 }
 
-// Adds two compensated partial sums (s, c): error-free sum of the heads,
-// carries added.
-__device__ __forceinline__ void csum_add(double &s, double &c, double s2, double c2) {
-  double t = s + s2, bb = t - s;
-  double err = (s - (t - bb)) + (s2 - bb);
-  s = t;
-  c += c2 + err;
-}
-
-// Compensated sums of the per-cell pair partials and per-block grid partials:
-// each thread sums a fixed strided subset, then a fixed binary tree combines
-// the 256 (sum, carry) pairs, so the result does not depend on scheduling.
-__global__ __launch_bounds__(256) void k_finalize(DevParams P, const double *__restrict__ ep, int np,
-                                                  const double *__restrict__ eg, int ng,
-                                                  const double *__restrict__ scal,
-                                                  double *__restrict__ out) {
-  __shared__ double ss[2][256], cc[2][256];
-  int t = threadIdx.x;
-  double s = 0, c = 0;
-  for (int k = t; k < np; k += 256) neumaier(s, c, ep[k]);
-  ss[0][t] = s, cc[0][t] = c;
-  s = 0, c = 0;
-  for (int k = t; k < ng; k += 256) neumaier(s, c, eg[k]);
-  ss[1][t] = s, cc[1][t] = c;
+// F += Fp and phi += phip (the short-range parts, written by the pair kernel
+// beside the interpolation), then the energy reduction: every workgroup
+// sums a fixed slice of the per-cell pair partials into a compensated
+// (sum, carry) pair, and the last workgroup to finish combines those and the
+// grid partials in a fixed order.
+__global__ __launch_bounds__(256) void k_combine(DevParams P, const double *__restrict__ Fp,
+                                                 const double *__restrict__ phip,
+                                                 double *__restrict__ F,
+                                                 double *__restrict__ phi, double *__restrict__ ep2,
+                                                 FinalizeArgs fin) {
+  __shared__ double ss[256], cc[256];
+  __shared__ int last;
+  const int t = threadIdx.x, b = blockIdx.x;
+  const long i = (long)b * 256 + t;
+  if (i < P.N) {
+    if (F)
+      for (int d = 0; d < 3; ++d) F[3 * i + d] += Fp[3 * i + d];
+    if (phi) phi[i] += phip[i];
+  }
+  const int chunk = (fin.np + gridDim.x - 1) / gridDim.x;
+  const int k0 = b * chunk, k1 = min(fin.np, k0 + chunk);
+  double sv = 0.0, cv = 0.0;
+  for (int k = k0 + t; k < k1; k += 256) neumaier(sv, cv, fin.ep[k]);
+  ss[t] = sv, cc[t] = cv;
   __syncthreads();
   for (int h = 128; h > 0; h >>= 1) {
-    if (t < h)
-      for (int w = 0; w < 2; ++w) {
-        double a = ss[w][t], b = cc[w][t];
-        csum_add(a, b, ss[w][t + h], cc[w][t + h]);
-        ss[w][t] = a, cc[w][t] = b;
-      }
+    if (t < h) {
+      double x = ss[t], y = cc[t];
+      csum_add(x, y, ss[t + h], cc[t + h]);
+      ss[t] = x, cc[t] = y;
+    }
     __syncthreads();
   }
-  if (t != 0) return;
-  double epair = ss[0][0] + cc[0][0], egrid = 0.5 * (ss[1][0] + cc[1][0]);
-  double qs = scal[0], q2 = scal[1];
-  // constant part (forcefield1.c:114-130)
-  double econst = 0.5 * q2 * P.coeff1 - 0.5 * (qs * qs) * P.coeff2 - 0.5 * q2 * P.gamma0 * P.inv_a0;
-  s = 0, c = 0;
-  neumaier(s, c, econst);
-  neumaier(s, c, epair);
-  neumaier(s, c, egrid);
-  out[0] = s + c;
-  out[1] = econst;
-  out[2] = epair;
-  out[3] = egrid;
+  if (t == 0) {
+    ep2[2 * b] = ss[0];
+    ep2[2 * b + 1] = cc[0];
+    __threadfence();
+    last = atomicAdd(fin.done, 1u) == gridDim.x - 1;
+  }
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  if (t == 0) *fin.done = 0u;
+  // the pair partials are now the gridDim.x (sum, carry) pairs of ep2
+  sv = 0.0, cv = 0.0;
+  for (int k = t; k < (int)gridDim.x; k += 256) csum_add(sv, cv, ep2[2 * k], ep2[2 * k + 1]);
+  finalize_tail(P, fin, sv, cv);
 }
 
 inline unsigned nblk(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
@@ -1152,24 +1325,24 @@ void launch_fill_dbl(double *p, long n, double v, hipStream_t s) {
 
 int bin_blocks(int N) { return (int)nblk(N, kBinBS); }
 
-void launch_bin(const DevParams &P, const double *pos, const double *q, int *cell_of, int *cnt,
-                double4 *wpos, double *qpart, int *flags, hipStream_t s) {
-  k_bin<<<bin_blocks(P.N), kBinBS, 0, s>>>(P, pos, q, cell_of, cnt, wpos, qpart, flags);
+void launch_bin(const DevParams &P, const double *pos, const double *q, int *cell_of, int *rank,
+                int *cnt, double4 *wpos, double *qpart, int *flags, hipStream_t s) {
+  k_bin<<<bin_blocks(P.N), kBinBS, 0, s>>>(P, pos, q, cell_of, rank, cnt, wpos, qpart, flags);
 }
 
-void launch_scan(const DevParams &P, int *cnt, int *start, const double *qpart, int nqpart,
+void launch_scan(const DevParams &P, const int *cnt, int *start, const double *qpart, int nqpart,
                  double *scal, hipStream_t s) {
   k_scan<<<1, kScanBS, 0, s>>>(P.ncell, cnt, start, qpart, nqpart, scal);
 }
 
-void launch_scatter(const DevParams &P, const int *cell_of, const double4 *wpos, const int *start,
-                    int *fill, double4 *spos, int *sidx, int *scell, hipStream_t s) {
-  k_scatter<<<nblk(P.N, 256), 256, 0, s>>>(P.N, cell_of, wpos, start, fill, spos, sidx, scell);
+void launch_scatter(const DevParams &P, const int *cell_of, const int *rank, const double4 *wpos,
+                    const int *start, double4 *spos, int *sidx, int *scell, hipStream_t s) {
+  k_scatter<<<nblk(P.N, 256), 256, 0, s>>>(P.N, cell_of, rank, wpos, start, spos, sidx, scell);
 }
 
-void launch_cell_sort(const DevParams &P, const int *start, double4 *spos, int *sidx,
+void launch_cell_sort(const DevParams &P, const int *start, int *cnt, double4 *spos, int *sidx,
                       float4 *lpos, hipStream_t s) {
-  k_cell_sort<<<nblk(P.ncell, 4), 256, 0, s>>>(P, start, spos, sidx, lpos);
+  k_cell_sort<<<nblk(P.ncell, 4), 256, 0, s>>>(P, start, cnt, spos, sidx, lpos);
 }
 
 int pair_blocks(int N) { return (int)nblk(N, kPairBS); }
@@ -1220,7 +1393,9 @@ void launch_spread(const DevParams &P, GridDims g, const double4 *spos, const in
 
 void launch_restrict(const DevParams &P, GridDims fine, GridDims coarse, const double *qf,
                      double *qc, hipStream_t s) {
-#define CALL(NU) k_restrict<NU><<<nblk(coarse.size(), 4), 256, 0, s>>>(P, fine, coarse, qf, qc)
+  const unsigned nb = (unsigned)(((coarse.x + kRB - 1) / kRB) * ((coarse.y + kRB - 1) / kRB) *
+                                 ((coarse.z + kRB - 1) / kRB));
+#define CALL(NU) k_restrict<NU><<<nb, 256, 0, s>>>(P, fine, coarse, qf, qc)
   MSM_NU_SWITCH(P.nu, CALL)
 #undef CALL
 }
@@ -1229,7 +1404,7 @@ void launch_top(const DevParams &P, GridDims g, const double *qL, const double *
                 const double *tw, double *work, double *eL, int asis, hipStream_t s) {
   long n = g.size();
   if (n <= kTopSmall) {
-    k_top_small<<<1, kTopBS, 0, s>>>(P, g, qL, khat, tw, eL, asis);
+    k_top_small<<<1, kTopBS, 4 * n * sizeof(double), s>>>(P, g, qL, khat, tw, eL, asis);
     return;
   }
   double *r0 = work, *i0 = work + n, *r1 = work + 2 * n, *i1 = work + 3 * n;
@@ -1272,15 +1447,24 @@ void launch_grid_energy(const DevParams &P, GridDims g, const double *q1, const 
 }
 
 void launch_gather(const DevParams &P, GridDims g, const double4 *spos, const int *sidx,
-                   const double *e1, const double *scal, double *F, double *phi, hipStream_t s) {
-#define CALL(NU) k_gather<NU><<<nblk(P.N, 256), 256, 0, s>>>(P, g, spos, sidx, e1, scal, F, phi)
+                   const double *e1, const double *scal, double *F, double *phi, bool add,
+                   hipStream_t s) {
+#define CALL(NU)                                                                        \
+  if (add) k_gather<NU, true><<<nblk(P.N, 256), 256, 0, s>>>(P, g, spos, sidx, e1, scal, F, phi); \
+  else k_gather<NU, false><<<nblk(P.N, 256), 256, 0, s>>>(P, g, spos, sidx, e1, scal, F, phi)
   MSM_NU_SWITCH(P.nu, CALL)
 #undef CALL
 }
 
-void launch_finalize(const DevParams &P, const double *epair, int npair, const double *egrid,
-                     int ngrid, const double *scal, double *out, hipStream_t s) {
-  k_finalize<<<1, 256, 0, s>>>(P, epair, npair, egrid, ngrid, scal, out);
+int combine_blocks(int N) { return (int)nblk(N, 256); }
+
+void launch_combine(const DevParams &P, const double *Fp, const double *phip, double *F,
+                    double *phi, double *ep2, const FinalizeArgs &fin, hipStream_t s) {
+  k_combine<<<combine_blocks(P.N), 256, 0, s>>>(P, Fp, phip, F, phi, ep2, fin);
+}
+
+void launch_finalize(const DevParams &P, const FinalizeArgs &fin, hipStream_t s) {
+  k_finalize<<<1, 256, 0, s>>>(P, fin);
 }
 
 }  // namespace msm

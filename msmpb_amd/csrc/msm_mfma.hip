@@ -187,56 +187,81 @@ __global__ __launch_bounds__(64) void k_g2g_tiles(G2GArgs args) {
   }
 }
 
-// e^l = sum_g part[g] + P e^{l+1}.  Fine index v receives J_n from coarse m
-// with 2m + n == v (mod 2 M_c): n has the parity of v, m = (v - n)/2 mod M_c.
-// Per-axis contributors of prolongation: fine index v receives J_n e_m from
-// coarse m with 2m + n == v (mod 2 M_c), i.e. n of the parity of v and
-// m = (v - n)/2 mod M_c; at most NU/2 + 1 of them.
-template <int NU>
-__device__ __forceinline__ int prolong_axis(const double *J, int v, int Mc, int *m, double *w) {
-  const int H = NU / 2;
-  int k = 0;
-#pragma unroll
-  for (int n = -H; n <= H; ++n)
-    if (((v - n) & 1) == 0) {
-      m[k] = wrapm((v - n) / 2, Mc);
-      w[k] = J[n + H];
-      ++k;
-    }
-  return k;
-}
+// e^l = sum_g part[g] + P e^{l+1} (prolongate, forcefield1.c:530-567), as
+// separable passes over an LDS tile.  Fine index v receives J_{v-2m+H} e_m
+// from every coarse m with |v - 2m| <= H (H = NU/2), periodic.  A workgroup
+// owns an 8^3 fine brick: it loads the (H+4)^3 coarse values that reach it,
+// applies the stencil along z, y and x in turn, adds the kx-group partials
+// of the stencil pass in a fixed order and stores every point once.  With
+// q1 != null (the finest level) it also writes the brick's sum of q1 e1,
+// the grid-energy partial.
+__device__ __forceinline__ int floor_half(int a) { return a >> 1; }
+__device__ __forceinline__ int ceil_half(int a) { return -((-a) >> 1); }
+
+constexpr int kPB = 8;  // fine brick edge of the prolongation
 
 template <int NU>
 __global__ __launch_bounds__(256) void k_prolong_sum(DevParams P, GridDims g, GridDims gc, int G,
                                                      const double *__restrict__ part,
                                                      const double *__restrict__ ec,
-                                                     double *__restrict__ el) {
-  long id = (long)blockIdx.x * 256 + threadIdx.x;
-  long n = g.size();
-  if (id >= n) return;
-  int z = id % g.z, y = (id / g.z) % g.y, x = id / ((long)g.z * g.y);
-  constexpr int K = NU / 2 + 1;
-  int mx[K], my[K], mz[K];
-  double wx[K], wy[K], wz[K];
-  int nx = prolong_axis<NU>(P.J, x, gc.x, mx, wx);
-  int ny = prolong_axis<NU>(P.J, y, gc.y, my, wy);
-  int nz = prolong_axis<NU>(P.J, z, gc.z, mz, wz);
-  double p = 0.0;
-  for (int a = 0; a < nx; ++a) {
-    double sy = 0.0;
-    for (int b = 0; b < ny; ++b) {
-      const double *rowp = ec + ((long)mx[a] * gc.y + my[b]) * gc.z;
-      double sz = 0.0;
-#pragma unroll
-      for (int c = 0; c < K; ++c)
-        if (c < nz) sz += wz[c] * rowp[mz[c]];
-      sy += wy[b] * sz;
-    }
-    p += wx[a] * sy;
+                                                     double *__restrict__ el,
+                                                     const double *__restrict__ q1,
+                                                     double *__restrict__ epart) {
+  constexpr int H = NU / 2, CW = H + 4, B = kPB;
+  __shared__ double cs[CW * CW * CW];
+  __shared__ double t1[CW * CW * B];  // [cx][cy][fz]
+  __shared__ double t2[CW * B * B];   // [cx][fy][fz]
+  __shared__ double wsum[4];
+  const int tid = threadIdx.x;
+  const int nbz = (g.z + B - 1) / B, nby = (g.y + B - 1) / B;
+  int b = blockIdx.x;
+  const int fz0 = (b % nbz) * B;
+  b /= nbz;
+  const int fy0 = (b % nby) * B, fx0 = (b / nby) * B;
+  // lowest coarse index reaching the brick: ceil((v0 - H) / 2) = v0/2 - H/2
+  const int cx0 = fx0 / 2 - H / 2, cy0 = fy0 / 2 - H / 2, cz0 = fz0 / 2 - H / 2;
+  for (int k = tid; k < CW * CW * CW; k += 256) {
+    const int a = k / (CW * CW), bb = (k / CW) % CW, c = k % CW;
+    cs[k] = ec[((long)wrapm(cx0 + a, gc.x) * gc.y + wrapm(cy0 + bb, gc.y)) * gc.z +
+               wrapm(cz0 + c, gc.z)];
   }
-  double c = 0.0;
-  for (int k = 0; k < G; ++k) c += part[(long)k * n + id];
-  el[id] = p + c;
+  __syncthreads();
+  for (int k = tid; k < CW * CW * B; k += 256) {  // along z
+    const int fz = k % B, cxy = k / B, v = fz0 + fz;
+    double acc = 0.0;
+    for (int m = ceil_half(v - H); m <= floor_half(v + H); ++m)
+      acc += P.J[v - 2 * m + H] * cs[cxy * CW + (m - cz0)];
+    t1[k] = acc;
+  }
+  __syncthreads();
+  for (int k = tid; k < CW * B * B; k += 256) {  // along y
+    const int fz = k % B, fy = (k / B) % B, cx = k / (B * B), v = fy0 + fy;
+    double acc = 0.0;
+    for (int m = ceil_half(v - H); m <= floor_half(v + H); ++m)
+      acc += P.J[v - 2 * m + H] * t1[(cx * CW + (m - cy0)) * B + fz];
+    t2[k] = acc;
+  }
+  __syncthreads();
+  const long n = g.size();
+  double es = 0.0;
+  for (int k = tid; k < B * B * B; k += 256) {  // along x, plus the stencil partials
+    const int fz = k % B, fy = (k / B) % B, fx = k / (B * B);
+    const int x = fx0 + fx, y = fy0 + fy, z = fz0 + fz;
+    if (x >= g.x || y >= g.y || z >= g.z) continue;
+    double acc = 0.0;
+    for (int m = ceil_half(x - H); m <= floor_half(x + H); ++m)
+      acc += P.J[x - 2 * m + H] * t2[((m - cx0) * B + fy) * B + fz];
+    const long id = ((long)x * g.y + y) * g.z + z;
+    for (int gi = 0; gi < G; ++gi) acc += part[(long)gi * n + id];
+    el[id] = acc;
+    if (q1) es += q1[id] * acc;
+  }
+  if (q1) {  // fixed-order block sum of q1 e1
+    for (int off = 32; off > 0; off >>= 1) es += __shfl_down(es, off, 64);
+    if ((tid & 63) == 0) wsum[tid >> 6] = es;
+    __syncthreads();
+    if (tid == 0) epart[blockIdx.x] = ((wsum[0] + wsum[1]) + wsum[2]) + wsum[3];
+  }
 }
 
 }  // namespace
@@ -333,14 +358,19 @@ void launch_g2g_tiles(int prec, G2GArgs args, size_t lds, hipStream_t s) {
   }
 }
 
+int prolong_blocks(GridDims g) {
+  return ((g.x + kPB - 1) / kPB) * ((g.y + kPB - 1) / kPB) * ((g.z + kPB - 1) / kPB);
+}
+
 void launch_prolong_sum(const DevParams &P, GridDims g, GridDims gc, int G, const double *part,
-                        const double *ec, double *el, hipStream_t s) {
-  unsigned nb = (unsigned)((g.size() + 255) / 256);
+                        const double *ec, double *el, const double *q1, double *epart,
+                        hipStream_t s) {
+  unsigned nb = (unsigned)prolong_blocks(g);
   switch (P.nu) {
-    case 4: k_prolong_sum<4><<<nb, 256, 0, s>>>(P, g, gc, G, part, ec, el); break;
-    case 6: k_prolong_sum<6><<<nb, 256, 0, s>>>(P, g, gc, G, part, ec, el); break;
-    case 8: k_prolong_sum<8><<<nb, 256, 0, s>>>(P, g, gc, G, part, ec, el); break;
-    default: k_prolong_sum<10><<<nb, 256, 0, s>>>(P, g, gc, G, part, ec, el); break;
+    case 4: k_prolong_sum<4><<<nb, 256, 0, s>>>(P, g, gc, G, part, ec, el, q1, epart); break;
+    case 6: k_prolong_sum<6><<<nb, 256, 0, s>>>(P, g, gc, G, part, ec, el, q1, epart); break;
+    case 8: k_prolong_sum<8><<<nb, 256, 0, s>>>(P, g, gc, G, part, ec, el, q1, epart); break;
+    default: k_prolong_sum<10><<<nb, 256, 0, s>>>(P, g, gc, G, part, ec, el, q1, epart); break;
   }
 }
 
