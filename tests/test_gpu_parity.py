@@ -128,6 +128,48 @@ def test_autograd_adjoint(recompute, oracle_mod, torch_cuda):
     assert np.max(np.abs(q.grad.cpu().numpy() - phio)) / np.sqrt(np.mean(phio ** 2)) < TOL
 
 
+def test_adjoint_central_differences(oracle_mod, torch_cuda):
+    """dE/dr and dE/dq from the adjoint chain against central differences of
+    the forward energy, FP64 (3000 charges in 30 x 35 x 40 A; steps 1e-5 A
+    and 1e-6 e) on a few atoms, to 1e-6 of the force / potential scale (the
+    differences' own truncation and rounding error).  The mixed path is not
+    differenced: FP32 rounding of the pair terms dominates energy changes at
+    these steps."""
+    precision = "fp64"
+    torch = torch_cuda
+    from msmpb_amd import msm_energy
+    from msmpb_amd.inputs import cfg2_random3000
+    print(f"[parity] central differences {precision}", flush=True)
+    s = cfg2_random3000()
+    m, _ = _pair(s, oracle_mod, precision=precision)
+    pos = torch.tensor(s.pos, device="cuda", requires_grad=True)
+    q = torch.tensor(s.q, device="cuda", requires_grad=True)
+    msm_energy(m, pos, q).backward()
+    gpos, gq = pos.grad.cpu().numpy(), q.grad.cpu().numpy()
+    fscale, pscale = np.sqrt(np.mean(gpos ** 2)), np.sqrt(np.mean(gq ** 2))
+
+    def energy(p, qq):
+        e, _, _ = m.eval_device(p, qq, forces=False)
+        return e[0].item()
+
+    P0 = torch.tensor(s.pos, device="cuda")
+    Q0 = torch.tensor(s.q, device="cuda")
+    tol = 1e-6
+    hr, hq = 1e-5, 1e-6
+    for i in (0, 17, 1234, 2999):
+        for d in range(3):
+            Pp, Pm = P0.clone(), P0.clone()
+            Pp[i, d] += hr
+            Pm[i, d] -= hr
+            fd = (energy(Pp, Q0) - energy(Pm, Q0)) / (2 * hr)
+            assert abs(fd - gpos[i, d]) < tol * fscale, (i, d, fd, gpos[i, d])
+        Qp, Qm = Q0.clone(), Q0.clone()
+        Qp[i] += hq
+        Qm[i] -= hq
+        fd = (energy(P0, Qp) - energy(P0, Qm)) / (2 * hq)
+        assert abs(fd - gq[i]) < tol * pscale, (i, fd, gq[i])
+
+
 def test_mixed_precision_accuracy(oracle_mod, torch_cuda):
     for case in ("water300", "crystal8192", "random3000"):
         print(f"[parity] mixed {case}", flush=True)
