@@ -492,9 +492,9 @@ int grid_energy_parts(const Engine &E) {
   return E.plan.L >= 2 ? msm::prolong_blocks(E.gd[1]) : msm::grid_blocks(E.gd[1].size());
 }
 
-msm::FinalizeArgs finalize_args(Engine &E) {
+msm::FinalizeArgs finalize_args(Engine &E, double *user) {
   return msm::FinalizeArgs{E.d_epair, E.P.ncell, E.d_egrid, grid_energy_parts(E), E.d_scal,
-                           E.d_flags, E.d_out, E.d_done};
+                           E.d_flags, E.d_out, E.d_done, user == E.d_out ? nullptr : user};
 }
 
 // Grid-energy partials for L = 1 (the level-1 prolongation writes them
@@ -508,11 +508,13 @@ void stage_grid_energy(Engine &E, int layout, hipStream_t s) {
 // After the join: with F or phi wanted, add the short-range parts (written
 // by the pair kernel into d_Fp / d_phip) and reduce the energy in the same
 // launch; otherwise only the reduction.
-void stage_energy(Engine &E, double *F, double *phi, hipStream_t s) {
+// `out` (caller's buffer of 4 doubles, may be E.d_out) receives the terms
+// from the reduction kernel itself.
+void stage_energy(Engine &E, double *F, double *phi, double *out, hipStream_t s) {
   if (F || phi)
-    msm::launch_combine(E.P, E.d_Fp, E.d_phip, F, phi, E.d_ep2, finalize_args(E), s);
+    msm::launch_combine(E.P, E.d_Fp, E.d_phip, F, phi, E.d_ep2, finalize_args(E, out), s);
   else
-    msm::launch_finalize(E.P, finalize_args(E), s);
+    msm::launch_finalize(E.P, finalize_args(E, out), s);
 }
 
 int check_built(FF *ff) {
@@ -560,9 +562,8 @@ int eval_chain(FFX *x, const double *pos, const double *q, double *out, double *
   }
   HIP_TRY(hipStreamWaitEvent(s, E.ev_join, 0));
   t = T.open(s);
-  stage_energy(E, F, phi, s);
+  stage_energy(E, F, phi, out, s);
   T.close(t, "energy", s);
-  if (out != E.d_out) HIP_TRY(hipMemcpyAsync(out, E.d_out, 4 * sizeof(double), hipMemcpyDeviceToDevice, s));
   HIP_TRY(hipGetLastError());
   if (T.on) ++E.timed_evals;
   E.kept = false;
@@ -1055,8 +1056,7 @@ int msm_forward_device(FF *ff, const double *d_pos, const double *d_q, double *d
   stage_potentials(E, x->precision, x->layout, T, s);
   stage_grid_energy(E, x->layout, s);
   HIP_TRY(hipStreamWaitEvent(s, E.ev_join, 0));
-  stage_energy(E, nullptr, nullptr, s);
-  HIP_TRY(hipMemcpyAsync(d_energy, E.d_out, 4 * sizeof(double), hipMemcpyDeviceToDevice, s));
+  stage_energy(E, nullptr, nullptr, d_energy, s);
   HIP_TRY(hipGetLastError());
   E.kept = keep != 0;
   E.kept_pos = d_pos;

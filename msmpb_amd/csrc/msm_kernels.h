@@ -138,6 +138,7 @@ struct FinalizeArgs {
   int *flags;
   double *out;  // >= 5 doubles; null: no reduction
   unsigned *done;
+  double *user;  // optional caller buffer that also receives out[0..3]
 };
 // Interpolation: F (=, or += with add) q E and phi (=, +=) grid potential
 // + constant self terms.

@@ -1171,6 +1171,12 @@ __device__ void finalize_tail(const DevParams &P, const FinalizeArgs &f, double 
   f.out[2] = epair;
   f.out[3] = egrid;
   f.out[4] = (double)*f.flags;
+  if (f.user) {
+    f.user[0] = f.out[0];
+    f.user[1] = econst;
+    f.user[2] = epair;
+    f.user[3] = egrid;
+  }
   *f.flags = 0;
 }
 
