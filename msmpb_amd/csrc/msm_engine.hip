@@ -86,7 +86,7 @@ struct Engine {
   hipStream_t stream = nullptr;    // the library's streams of this device (shared, not owned)
   hipStream_t stream2 = nullptr;   // short-range pairs run here, beside the grid chain
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
-  int nsten = 0;
+  int nsten = 0;  // neighbour cell columns of the pair search
   // device buffers
   double *d_pos = nullptr, *d_q = nullptr, *d_F = nullptr, *d_phi = nullptr;
   double *d_out = nullptr, *d_qpart = nullptr, *d_scal = nullptr;
@@ -98,7 +98,6 @@ struct Engine {
   unsigned *d_done = nullptr;  // workgroup counter of the interpolation's energy reduction
   int *d_scell = nullptr, *d_flags = nullptr;
   double4 *d_wpos = nullptr, *d_spos = nullptr;
-  float4 *d_lpos = nullptr;
   int4 *d_sten = nullptr;
   unsigned long long *d_count = nullptr;
   double *d_ql[msm::kMaxLevel + 1] = {}, *d_el[msm::kMaxLevel + 1] = {};
@@ -125,7 +124,7 @@ struct Engine {
     void *ptrs[] = {d_pos,  d_q,    d_F,    d_phi,   d_out,  d_qpart, d_scal,  d_epair,
                     d_egrid, d_cell_of, d_cnt, d_start, d_sidx, d_scell, d_flags, d_wpos,
                     d_rank, d_done, d_Fp, d_phip, d_ep2,
-                    d_spos, d_lpos, d_sten, d_count, d_qsplit, d_tw, d_topwork};
+                    d_spos, d_sten, d_count, d_qsplit, d_tw, d_topwork};
     for (void *p : ptrs)
       if (p) hipFree(p);
     for (int pr = 0; pr < 2; ++pr)
@@ -148,7 +147,6 @@ struct Engine {
     d_done = nullptr;
     d_Fp = d_phip = d_ep2 = nullptr;
     d_wpos = d_spos = nullptr;
-    d_lpos = nullptr;
     d_sten = nullptr;
     d_count = nullptr;
     d_qsplit = nullptr;
@@ -193,14 +191,22 @@ int dalloc(T **p, size_t n) {
     if (r_ != MSM_OK) return r_; \
   } while (0)
 
-// Cells of width >= a0/2 and the list of neighbour-cell offsets whose
-// nearest points lie inside the cutoff.
-void pair_geometry(const msm::Plan &p, msm::DevParams &P, std::vector<int4> &sten) {
-  double a0 = p.aCut[0];
+// The pair-search cell grid and its neighbour cell columns.  Cells are at
+// least a0/3 wide (a0/2 where that would leave fewer than about two atoms per
+// cell).  A column (dx, dy, dzlo, dzhi) lists the offsets dz in [dzlo, dzhi]
+// of the cells (dx, dy, dz) whose nearest points lie inside the cutoff; longer
+// ranges are split so that each spans fewer than nc[2] cells (one wrap at
+// most).  Also the self-image constant: sum of g(|k L|) over the images
+// k != 0 of an atom itself inside the cutoff.
+void pair_geometry(const msm::Plan &p, msm::DevParams &P, std::vector<int4> &cols) {
+  const double a0 = p.aCut[0];
+  const double vol = p.A[0][0] * p.A[1][1] * p.A[2][2];
+  const double dens = p.N / vol, w3 = a0 / 3.0;
+  const double div = dens * w3 * w3 * w3 >= 2.0 ? 3.0 : 2.0;
   int lim[3];
   for (int a = 0; a < 3; ++a) {
     double L = p.A[a][a];
-    int n = (int)std::floor(L / (0.5 * a0));
+    int n = (int)std::floor(L / (a0 / div));
     if (n < 1) n = 1;
     if (n > 256) n = 256;
     P.nc[a] = n;
@@ -208,22 +214,37 @@ void pair_geometry(const msm::Plan &p, msm::DevParams &P, std::vector<int4> &ste
     lim[a] = (int)std::ceil(a0 / P.cw[a]);
   }
   P.ncell = P.nc[0] * P.nc[1] * P.nc[2];
-  sten.clear();
+  auto gap2 = [&](int d, int a) {
+    double g = (std::abs(d) - 1) * P.cw[a];
+    return g > 0 ? g * g : 0.0;
+  };
+  cols.clear();
   for (int dx = -lim[0]; dx <= lim[0]; ++dx)
-    for (int dy = -lim[1]; dy <= lim[1]; ++dy)
-      for (int dz = -lim[2]; dz <= lim[2]; ++dz) {
-        int d[3] = {dx, dy, dz};
-        double m2 = 0;
-        for (int a = 0; a < 3; ++a) {
-          double g = (std::abs(d[a]) - 1) * P.cw[a];
-          if (g > 0) m2 += g * g;
-        }
-        if (m2 < a0 * a0) sten.push_back(make_int4(dx, dy, dz, 0));
+    for (int dy = -lim[1]; dy <= lim[1]; ++dy) {
+      const double gxy = gap2(dx, 0) + gap2(dy, 1);
+      if (gxy >= a0 * a0) continue;
+      int zmax = 0;
+      while (zmax + 1 <= lim[2] && gxy + gap2(zmax + 1, 2) < a0 * a0) ++zmax;
+      for (int z0 = -zmax; z0 <= zmax; z0 += P.nc[2])
+        cols.push_back(make_int4(dx, dy, z0, std::min(zmax, z0 + P.nc[2] - 1)));
+    }
+  // g over the self images (forcefield1.c:240-321 skips i == j at every image)
+  double C = 0.0;
+  int K[3];
+  for (int a = 0; a < 3; ++a) K[a] = (int)std::ceil(a0 / p.A[a][a]);
+  for (int kx = -K[0]; kx <= K[0]; ++kx)
+    for (int ky = -K[1]; ky <= K[1]; ++ky)
+      for (int kz = -K[2]; kz <= K[2]; ++kz) {
+        if (!kx && !ky && !kz) continue;
+        const double x = kx * p.A[0][0], y = ky * p.A[1][1], z = kz * p.A[2][2];
+        const double r2 = x * x + y * y + z * z;
+        if (r2 >= a0 * a0) continue;
+        const double sv = r2 / (a0 * a0) - 1.0;
+        double ts = p.tau[p.nu - 1];
+        for (int k = p.nu - 2; k >= 0; --k) ts = p.tau[k] + sv * ts;
+        C += 1.0 / std::sqrt(r2) - ts / a0;
       }
-  P.self_wrap = 0;
-  for (const int4 &d : sten)
-    if ((d.x || d.y || d.z) && d.x % P.nc[0] == 0 && d.y % P.nc[1] == 0 && d.z % P.nc[2] == 0)
-      P.self_wrap = 1;
+  P.self_img = C;
 }
 
 void fill_params(const msm::Plan &p, int N, msm::DevParams &P) {
@@ -249,6 +270,22 @@ void fill_params(const msm::Plan &p, int N, msm::DevParams &P) {
   P.coeff2 = p.coeff2;
   P.gamma0 = p.gamma0;
   P.coeff_self = p.coeff1 - p.gamma0 / a0;
+  P.cut2 = P.a02 * (1.0 + 1e-12);
+  std::memset(P.tau_s, 0, sizeof P.tau_s);
+  std::memset(P.dtau_s, 0, sizeof P.dtau_s);
+  std::memset(P.ftau, 0, sizeof P.ftau);
+  std::memset(P.fdtau, 0, sizeof P.fdtau);
+  for (int k = 0; k < p.nu; ++k) {
+    P.tau_s[k] = p.tau[k] / a0;
+    P.ftau[k] = (float)P.tau_s[k];
+  }
+  for (int k = 0; k + 1 < p.nu; ++k) {
+    P.dtau_s[k] = (k + 1) * p.tau[k + 1] * P.two_inv_a03;
+    P.fdtau[k] = (float)P.dtau_s[k];
+  }
+  P.fa02 = (float)P.a02;
+  P.fia02 = (float)P.inv_a02;
+  P.fcut2 = (float)(P.a02 * 1.0001);
 }
 
 // Uploads the box-dependent device data (stencils, twiddles, parameters).
@@ -406,7 +443,6 @@ int create_engine(FFX *x, const msm::Plan &plan) {
   HIP_TRY(hipMemset(E->d_done, 0, sizeof(unsigned)));
   TRY(dalloc(&E->d_wpos, N));
   TRY(dalloc(&E->d_spos, N));
-  TRY(dalloc(&E->d_lpos, N));
   TRY(dalloc(&E->d_count, 1));
   for (int l = 1; l <= p.L; ++l) {
     TRY(dalloc(&E->d_ql[l], E->gd[l].size()));
@@ -427,15 +463,15 @@ void stage_prepare(Engine &E, const double *pos, const double *q, hipStream_t s)
   msm::launch_scan(E.P, E.d_cnt, E.d_start, E.d_qpart, msm::bin_blocks(E.P.N), E.d_scal, s);
   msm::launch_scatter(E.P, E.d_cell_of, E.d_rank, E.d_wpos, E.d_start, E.d_spos, E.d_sidx,
                       E.d_scell, s);
-  msm::launch_cell_sort(E.P, E.d_start, E.d_cnt, E.d_spos, E.d_sidx, E.d_lpos, s);
+  msm::launch_cell_sort(E.P, E.d_start, E.d_cnt, E.d_spos, E.d_sidx, s);
 }
 
 void stage_pairs(Engine &E, int precision, double *F, double *phi, hipStream_t s) {
   if (precision == MSM_PREC_MIXED)
-    msm::launch_pair_fp32(E.P, E.d_lpos, E.d_sidx, E.d_scell, E.d_start, E.d_sten, E.nsten, F, phi,
+    msm::launch_pair_fp32(E.P, E.d_spos, E.d_sidx, E.d_start, E.d_sten, E.nsten, F, phi,
                           E.d_epair, s);
   else
-    msm::launch_pair_fp64(E.P, E.d_spos, E.d_sidx, E.d_scell, E.d_start, E.d_sten, E.nsten, F, phi,
+    msm::launch_pair_fp64(E.P, E.d_spos, E.d_sidx, E.d_start, E.d_sten, E.nsten, F, phi,
                           E.d_epair, s);
 }
 
@@ -557,7 +593,7 @@ int eval_chain(FFX *x, const double *pos, const double *q, double *out, double *
   stage_grid_energy(E, x->layout, s);
   if (F || phi) {
     t = T.open(s);
-    msm::launch_gather(E.P, E.gd[1], E.d_spos, E.d_sidx, E.d_el[1], E.d_scal, F, phi, false, s);
+    msm::launch_gather(E.P, E.gd[1], E.d_spos, E.d_sidx, E.d_start, E.d_el[1], E.d_scal, F, phi, false, s);
     T.close(t, "interpolation", s);
   }
   HIP_TRY(hipStreamWaitEvent(s, E.ev_join, 0));
@@ -1080,7 +1116,7 @@ int msm_adjoint_device(FF *ff, const double *d_pos, const double *d_q, double *d
   }
   stage_pairs(E, x->precision, d_force, d_phi, s);  // the pair kernel writes, gather adds
   if (d_force || d_phi)
-    msm::launch_gather(E.P, E.gd[1], E.d_spos, E.d_sidx, E.d_el[1], E.d_scal, d_force, d_phi,
+    msm::launch_gather(E.P, E.gd[1], E.d_spos, E.d_sidx, E.d_start, E.d_el[1], E.d_scal, d_force, d_phi,
                        true, s);
   HIP_TRY(hipGetLastError());
   return MSM_OK;

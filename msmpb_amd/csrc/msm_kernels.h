@@ -15,10 +15,14 @@ struct DevParams {
   double ibox[3];      // 1 / edge lengths (diagonal of Ai)
   int nc[3];           // pair-search cells per axis
   int ncell;
-  int self_wrap;       // some stencil offset != 0 wraps onto the home cell (narrow
-                       // cell grid): periodic self images must be excluded by index
+  double self_img;     // sum of g(r) over the periodic images of an atom itself
+                       // inside the cutoff (nonzero only for boxes narrower than a0)
   double cw[3];        // cell widths
   double a0, a02, inv_a0, inv_a02, two_inv_a03;
+  double cut2;         // a0^2 enlarged by 1e-12: bounding-box tests never drop a pair
+  double tau_s[kMaxNu], dtau_s[kMaxNu];  // tau_k / a0 and (k + 1) tau_{k+1} 2 / a0^3
+  float ftau[kMaxNu], fdtau[kMaxNu];     // the same in FP32
+  float fa02, fia02, fcut2;              // fcut2: a0^2 enlarged by 1e-4
   int nu;
   int N;
   double tau[kMaxNu];
@@ -50,25 +54,26 @@ void launch_scan(const DevParams &P, const int *cnt, int *start, const double *q
                  double *scal, hipStream_t s);
 void launch_scatter(const DevParams &P, const int *cell_of, const int *rank, const double4 *wpos,
                     const int *start, double4 *spos, int *sidx, int *scell, hipStream_t s);
-// Orders atoms inside each cell by original index (deterministic results),
-// writes the cell-local FP32 coordinates used by the mixed pair kernel and
-// clears cnt for the next binning.
+// Orders atoms inside each cell by original index (deterministic results)
+// and clears cnt for the next binning.
 void launch_cell_sort(const DevParams &P, const int *start, int *cnt, double4 *spos, int *sidx,
-                      float4 *lpos, hipStream_t s);
+                      hipStream_t s);
 
-// Short-range pairs, full neighbour shell, one wavefront per home cell.
+// Short-range pairs, full neighbour shell, one wavefront per home cell;
+// cols: (dx, dy, dzlo, dzhi) neighbour cell columns, dzhi - dzlo < nc[2].
 // Writes force (if F) and potential (if phi) at the original atom index,
-// and one energy partial per cell into epart (ncell doubles).
+// and one energy partial per cell into epart (ncell doubles).  The mixed
+// kernel evaluates in FP32 relative to the home cell and sums in FP64.
 int pair_blocks(int N);
-void launch_pair_fp64(const DevParams &P, const double4 *spos, const int *sidx, const int *scell,
-                      const int *start, const int4 *sten, int nsten, double *F, double *phi,
-                      double *epart, hipStream_t s);
-void launch_pair_fp32(const DevParams &P, const float4 *lpos, const int *sidx, const int *scell,
-                      const int *start, const int4 *sten, int nsten, double *F, double *phi,
-                      double *epart, hipStream_t s);
+void launch_pair_fp64(const DevParams &P, const double4 *spos, const int *sidx, const int *start,
+                      const int4 *cols, int ncol, double *F, double *phi, double *epart,
+                      hipStream_t s);
+void launch_pair_fp32(const DevParams &P, const double4 *spos, const int *sidx, const int *start,
+                      const int4 *cols, int ncol, double *F, double *phi, double *epart,
+                      hipStream_t s);
 // Pair count within the cutoff (i < j, all images counted once per pair).
 void launch_pair_count(const DevParams &P, const double4 *spos, const int *scell, const int *start,
-                       const int4 *sten, int nsten, unsigned long long *count, hipStream_t s);
+                       const int4 *cols, int ncol, unsigned long long *count, hipStream_t s);
 
 // B-spline spreading of charges onto the level-1 grid, one workgroup per
 // grid brick (every grid point written exactly once).
@@ -141,10 +146,11 @@ struct FinalizeArgs {
   double *user;  // optional caller buffer that also receives out[0..3]
 };
 // Interpolation: F (=, or += with add) q E and phi (=, +=) grid potential
-// + constant self terms.
+// + constant self terms.  Tiles of cells (start: the cell ranges) read the
+// grid through LDS.
 void launch_gather(const DevParams &P, GridDims g, const double4 *spos, const int *sidx,
-                   const double *e1, const double *scal, double *F, double *phi, bool add,
-                   hipStream_t s);
+                   const int *start, const double *e1, const double *scal, double *F,
+                   double *phi, bool add, hipStream_t s);
 void launch_finalize(const DevParams &P, const FinalizeArgs &fin, hipStream_t s);
 // F += Fp, phi += phip (either may be null) and the energy reduction; ep2
 // holds 2 doubles per workgroup (combine_blocks(N)).

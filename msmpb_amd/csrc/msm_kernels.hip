@@ -136,60 +136,80 @@ __global__ __launch_bounds__(kBinBS) void k_bin(DevParams P, const double *__res
 
 constexpr int kScanBS = 1024;
 
-// Exclusive scan of the cell counts (start[0..n]) in tiles of 8 elements per
-// thread: a per-thread prefix, a wavefront shuffle scan of the thread totals
-// and a scan of the 16 wavefront totals; the counts are left in place (the
-// cell sort clears them for the next evaluation).  Also reduces the per-block
-// charge sums of the binning pass.
+// Exclusive scan of the cell counts (start[0..n]) by one workgroup, in
+// passes of kScanTiles tiles of 4 x kScanBS counts: thread t loads counts
+// 4t .. 4t + 3 of every tile of the pass (coalesced, all loads in flight
+// together), the per-thread sums are scanned per tile across the workgroup
+// (wavefront shuffles, then the wavefront totals), and the prefixes are
+// written back four at a time.  The counts are left in place (the cell sort
+// clears them for the next evaluation).  Also reduces the per-block charge
+// sums of the binning pass.
+constexpr int kScanTiles = 8;
 __global__ __launch_bounds__(kScanBS) void k_scan(int n, const int *__restrict__ cnt,
                                                   int *__restrict__ start,
                                                   const double *__restrict__ qpart, int nq,
                                                   double *__restrict__ scal) {
-  constexpr int PER = 8, NW = kScanBS / 64;
-  __shared__ int wtot[NW];
-  __shared__ int carry_s;
+  constexpr int NW = kScanBS / 64, TILE = 4 * kScanBS;
+  __shared__ int wtot[kScanTiles][NW];
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-  if (t == 0) carry_s = 0;
-  __syncthreads();
-  for (int base = 0; base < n; base += kScanBS * PER) {
-    const int k0 = base + t * PER;
-    int v[PER], run = 0;
-#pragma unroll
-    for (int u = 0; u < PER; ++u) {
-      v[u] = k0 + u < n ? cnt[k0 + u] : 0;
-      run += v[u];
-    }
-    int incl = run;
-    for (int off = 1; off < 64; off <<= 1) {
-      int o = __shfl_up(incl, off, 64);
-      if (lane >= off) incl += o;
-    }
-    if (lane == 63) wtot[w] = incl;
-    __syncthreads();
-    int woff = 0, tile = 0;
-#pragma unroll
-    for (int j = 0; j < NW; ++j) {
-      woff += j < w ? wtot[j] : 0;
-      tile += wtot[j];
-    }
-    int pre = carry_s + woff + incl - run;
-#pragma unroll
-    for (int u = 0; u < PER; ++u) {
-      if (k0 + u < n) start[k0 + u] = pre;
-      pre += v[u];
-    }
-    __syncthreads();  // everyone has read wtot and carry_s
-    if (t == 0) carry_s += tile;
-    __syncthreads();
-  }
-  if (t == 0) start[n] = carry_s;
-  // charge sums for the constant energy terms
   double q1 = 0, q2 = 0;
   for (int k = t; k < nq; k += kScanBS) q1 += qpart[2 * k], q2 += qpart[2 * k + 1];
+  int carry = 0;
+  for (int base = 0; base < n; base += kScanTiles * TILE) {
+    int4 v[kScanTiles];
+    int sum[kScanTiles], incl[kScanTiles];
+#pragma unroll
+    for (int j = 0; j < kScanTiles; ++j) {
+      const int i = base + j * TILE + 4 * t;
+      if (i + 3 < n) {
+        v[j] = *reinterpret_cast<const int4 *>(cnt + i);
+      } else {
+        v[j].x = i < n ? cnt[i] : 0;
+        v[j].y = i + 1 < n ? cnt[i + 1] : 0;
+        v[j].z = i + 2 < n ? cnt[i + 2] : 0;
+        v[j].w = 0;
+      }
+      sum[j] = v[j].x + v[j].y + v[j].z + v[j].w;
+      incl[j] = sum[j];
+    }
+    for (int off = 1; off < 64; off <<= 1)
+#pragma unroll
+      for (int j = 0; j < kScanTiles; ++j) {
+        const int o = __shfl_up(incl[j], off, 64);
+        if (lane >= off) incl[j] += o;
+      }
+    if (lane == 63)
+#pragma unroll
+      for (int j = 0; j < kScanTiles; ++j) wtot[j][w] = incl[j];
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kScanTiles; ++j) {
+      int woff = 0, tile = 0;
+#pragma unroll
+      for (int k = 0; k < NW; ++k) {
+        const int x = wtot[j][k];
+        woff += k < w ? x : 0;
+        tile += x;
+      }
+      const int p = carry + woff + incl[j] - sum[j];
+      const int4 o = make_int4(p, p + v[j].x, p + v[j].x + v[j].y, p + v[j].x + v[j].y + v[j].z);
+      const int i = base + j * TILE + 4 * t;
+      if (i + 3 < n) {
+        *reinterpret_cast<int4 *>(start + i) = o;
+      } else {
+        if (i < n) start[i] = o.x;
+        if (i + 1 < n) start[i + 1] = o.y;
+        if (i + 2 < n) start[i + 2] = o.z;
+      }
+      carry += tile;  // the same value in every thread
+    }
+    __syncthreads();  // wtot is rewritten by the next pass
+  }
+  if (t == 0) start[n] = carry;
+  // charge sums for the constant energy terms
+  const double a = block_sum<kScanBS>(q1);
   __syncthreads();
-  double a = block_sum<kScanBS>(q1);
-  __syncthreads();
-  double c2 = block_sum<kScanBS>(q2);
+  const double c2 = block_sum<kScanBS>(q2);
   if (t == 0) scal[0] = a, scal[1] = c2;
 }
 
@@ -209,35 +229,32 @@ __global__ __launch_bounds__(256) void k_scatter(int N, const int *__restrict__ 
   scell[slot] = c;
 }
 
-// One wavefront per cell: ranks the cell's atoms by original index (so the
-// order, and every sum built on it, is deterministic), moves them in place
-// and writes the cell-local FP32 coordinates used by the mixed pair kernel.
+// W lanes per cell (W = 16 when cells hold a few atoms, else 64): ranks the
+// cell's atoms by original index (so the order, and every sum built on it, is
+// deterministic) and moves them in place.
+template <int W>
 __global__ __launch_bounds__(256) void k_cell_sort(DevParams P, const int *__restrict__ start,
                                                    int *__restrict__ cnt,
                                                    double4 *__restrict__ spos,
-                                                   int *__restrict__ sidx,
-                                                   float4 *__restrict__ lpos) {
-  int c = blockIdx.x * 4 + (threadIdx.x >> 6);
-  int lane = threadIdx.x & 63;
+                                                   int *__restrict__ sidx) {
+  const int c = blockIdx.x * (256 / W) + threadIdx.x / W;
+  const int lane = threadIdx.x % W;
   if (c >= P.ncell) return;
   if (lane == 0) cnt[c] = 0;  // ready for the next evaluation's binning
-  int b = start[c], n = start[c + 1] - b;
-  int cz = c % P.nc[2], cy = (c / P.nc[2]) % P.nc[1], cx = c / (P.nc[1] * P.nc[2]);
-  double ox = cx * P.cw[0], oy = cy * P.cw[1], oz = cz * P.cw[2];
-  if (n <= 64) {
-    int key = lane < n ? sidx[b + lane] : 0x7fffffff;
-    double4 v = lane < n ? spos[b + lane] : make_double4(0, 0, 0, 0);
+  const int b = start[c], n = start[c + 1] - b;
+  if (n <= W) {
+    const int key = lane < n ? sidx[b + lane] : 0x7fffffff;
+    const double4 v = lane < n ? spos[b + lane] : make_double4(0, 0, 0, 0);
     int rank = 0;
-    for (int j = 0; j < n; ++j) rank += __shfl(key, j, 64) < key;
+    for (int j = 0; j < n; ++j) rank += __shfl(key, j, W) < key;
     if (lane < n) {
       sidx[b + rank] = key;
       spos[b + rank] = v;
-      lpos[b + rank] = make_float4((float)(v.x - ox), (float)(v.y - oy), (float)(v.z - oz), (float)v.w);
     }
-  } else if (lane == 0) {  // rare crowded cell: insertion sort by one lane
+  } else if (lane == 0) {  // crowded cell: insertion sort by one lane
     for (int k = b + 1; k < b + n; ++k) {
-      int key = sidx[k];
-      double4 v = spos[k];
+      const int key = sidx[k];
+      const double4 v = spos[k];
       int m = k - 1;
       while (m >= b && sidx[m] > key) {
         sidx[m + 1] = sidx[m];
@@ -247,91 +264,57 @@ __global__ __launch_bounds__(256) void k_cell_sort(DevParams P, const int *__res
       sidx[m + 1] = key;
       spos[m + 1] = v;
     }
-    for (int k = b; k < b + n; ++k) {
-      double4 v = spos[k];
-      lpos[k] = make_float4((float)(v.x - ox), (float)(v.y - oy), (float)(v.z - oz), (float)v.w);
-    }
   }
 }
 
 // ---------------------------------------------------------------- pairs
 // partcl2partcl (forcefield1.c:240-321) as a full-shell gather: each atom
 // sums q_j g(r) over every neighbour j != i and every periodic image inside
-// the cutoff, with g(r) = 1/r - tau(r^2/a0^2 - 1)/a0.  Neighbour cells are
-// visited through a stencil of cell offsets; an offset that leaves the box
-// carries the image shift, so small boxes get every image exactly once.
+// the cutoff, with g(r) = 1/r - tau(r^2/a0^2 - 1)/a0.
 //
-// One wavefront per home cell: the cell's atoms (up to 64 per pass) occupy
-// lane groups of S lanes (S = 4, 2 or 1 as the cell holds <= 16, <= 32 or
-// more atoms), and for every stencil offset -- the same for the whole
-// wavefront -- the S lanes of a group stride over that neighbour cell's
-// atoms.  All lanes run the same loop, a j atom is fetched once for all
-// groups, and a group's partial sums are combined with shuffles; no
-// atomics, so the result is deterministic.
-constexpr int kPairBS = 256;
+// One wavefront per home cell.  The home atoms are taken up to kPassAtoms at
+// a time (a pass); their coordinates are wavefront-uniform.  The neighbour
+// atoms that can come within the cutoff of the pass's bounding box are
+// staged in LDS (coordinates relative to the home-cell origin), and lane l
+// evaluates staged atoms l, l + 64, ... against every home atom of the pass,
+// so no lane idles whatever the cell population.  Neighbours are found by
+// cell columns: for each (dx, dy) offset the cells dz in [dzlo, dzhi] are
+// contiguous in the sorted order, so a column is one index range (two where
+// it wraps around the box); the z range is first trimmed to the cells the
+// bounding box can reach.  Each lane's partial sums are combined across the
+// wavefront in a fixed butterfly order: no atomics, so the result is
+// deterministic.
+//
+// Periodic images of an atom itself (boxes narrower than the cutoff) are
+// evaluated like any other neighbour, and their sum q_i * self_img, which
+// depends on the box only, is taken off phi_i; their forces cancel pairwise.
+constexpr int kPairBS = 256;    // workgroup of the pair-count kernel
+constexpr int kPassAtoms = 8;   // home atoms per pass
+constexpr int kSegMax = 128;    // atom ranges per round: 64 columns x <= 2 pieces
 
-// Neighbour atoms are staged through LDS in chunks: the 64 lanes take 64
-// stencil offsets at a time, a wavefront scan of their cell populations
-// decides how many cells fit the chunk, and the lanes copy those atoms --
-// already shifted into the home cell's frame -- cooperatively.  The
-// interaction loop then reads only LDS.
 template <bool MIXED>
-struct PairStage;
+struct PairPrec;
 template <>
-struct PairStage<true> {
+struct PairPrec<true> {
   using T = float4;
-  static constexpr int CAP = 512;
+  using R = float;
+  static constexpr int CAP = 512;  // staged neighbours per interaction run
 };
 template <>
-struct PairStage<false> {
+struct PairPrec<false> {
   using T = double4;
+  using R = double;
   static constexpr int CAP = 256;
 };
 
-// Interactions of one home atom (position ai, sorted index ii) with the
-// staged atoms e = bs, bs + S, ... < npad (npad a multiple of 16 S; the tail
-// holds far-away padding).  Self pairs have r2 == 0; with CHK, periodic self
-// images (narrow cell grids only) are excluded by index as well.
-template <bool CHK, int NU>
-__device__ __forceinline__ void pair_run_fp32(const DevParams &P, const float4 *jbuf,
-                                              const int *jidx, int npad, int bs, int S, float4 ai,
-                                              int ii, double &acc, double &fx, double &fy,
-                                              double &fz) {
-  const float a02 = (float)P.a02, ia02 = (float)P.inv_a02, ia0 = (float)P.inv_a0,
-              t3 = (float)P.two_inv_a03;
-  float tau[NU], dtau[NU - 1];  // tau(s) and tau'(s) coefficients
-#pragma unroll
-  for (int k = 0; k < NU; ++k) tau[k] = (float)P.tau[k];
-#pragma unroll
-  for (int k = 0; k + 1 < NU; ++k) dtau[k] = (float)((k + 1) * P.tau[k + 1]);
-  for (int e0 = bs; e0 < npad; e0 += 16 * S) {
-    // FP32 partial sums over 16 neighbours, then added in FP64
-    float sa = 0.f, sx = 0.f, sy = 0.f, sz = 0.f;
-#pragma unroll 8
-    for (int u = 0; u < 16; ++u) {
-      // branch-free: some lane of the wavefront nearly always accepts
-      const int e = e0 + u * S;
-      const float4 aj = jbuf[e];
-      const float dx = aj.x - ai.x, dy = aj.y - ai.y, dz = aj.z - ai.z;
-      const float r2 = dx * dx + dy * dy + dz * dz;
-      bool ok = r2 < a02 && r2 > 0.f;
-      if constexpr (CHK) ok = ok && jidx[e] != ii;
-      const float r2s = ok ? r2 : 1.0f;
-      const float qj = ok ? aj.w : 0.0f;
-      const float sv = r2s * ia02 - 1.0f;
-      float ts = tau[NU - 1], t1 = dtau[NU - 2];
-#pragma unroll
-      for (int k = NU - 2; k >= 0; --k) ts = tau[k] + sv * ts;
-#pragma unroll
-      for (int k = NU - 3; k >= 0; --k) t1 = dtau[k] + sv * t1;
-      const float rinv = __builtin_amdgcn_rsqf(r2s);  // r2 >= 1e-2 for physical input
-      sa += qj * (rinv - ts * ia0);
-      const float qc = qj * (-rinv * rinv * rinv - t1 * t3);
-      sx += qc * dx, sy += qc * dy, sz += qc * dz;
-    }
-    acc += (double)sa;
-    fx += (double)sx, fy += (double)sy, fz += (double)sz;
-  }
+__device__ __forceinline__ float wave_uniform(float v) {
+  return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+__device__ __forceinline__ double wave_uniform(double v) {
+  const long long b = __double_as_longlong(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(b & 0xffffffffll));
+  const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(b >> 32));
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
 // 1/sqrt(r2) in FP64: hardware estimate refined by two Newton steps.
@@ -343,210 +326,326 @@ __device__ __forceinline__ double rsqrt_f64(double r2) {
   return y;
 }
 
-template <bool CHK, int NU>
-__device__ __forceinline__ void pair_run_fp64(const DevParams &P, const double4 *jbuf,
-                                              const int *jidx, int npad, int bs, int S, double4 ai,
-                                              int ii, double &acc, double &fx, double &fy,
-                                              double &fz) {
-  const double a02 = P.a02;
-  double dtau[NU - 1];
+// One neighbour (aj: position relative to the home-cell origin, charge) and
+// one home atom (ix, iy, iz): sa += q_j g(r), (sx, sy, sz) += q_j g'(r)/r d
+// with d = r_j - r_i.  Out-of-range and self pairs get q_j = 0 and a finite
+// stand-in r^2; the pre-scaled coefficients fold 1/a0 into tau and 2/a0^3
+// into tau'.
+template <int NU>
+__device__ __forceinline__ void pair_term(const DevParams &P, float4 aj, float ix, float iy,
+                                          float iz, float &sa, float &sx, float &sy, float &sz) {
+  const float dx = aj.x - ix, dy = aj.y - iy, dz = aj.z - iz;
+  const float r2 = dx * dx + dy * dy + dz * dz;
+  const bool ok = r2 < P.fa02 && r2 > 0.f;
+  const float r2s = __builtin_amdgcn_fmed3f(r2, 1e-20f, P.fa02);
+  const float qj = ok ? aj.w : 0.f;
+  const float sv = r2s * P.fia02 - 1.f;
+  float ts = P.ftau[NU - 1], t1 = P.fdtau[NU - 2];
 #pragma unroll
-  for (int k = 0; k + 1 < NU; ++k) dtau[k] = (k + 1) * P.tau[k + 1];
-  for (int e = bs; e < npad; e += S) {
-    const double4 aj = jbuf[e];
-    const double dx = aj.x - ai.x, dy = aj.y - ai.y, dz = aj.z - ai.z;
-    const double r2 = dx * dx + dy * dy + dz * dz;
-    bool ok = r2 < a02 && r2 > 0.0;
-    if constexpr (CHK) ok = ok && jidx[e] != ii;
-    if (ok) {
-      const double sv = r2 * P.inv_a02 - 1.0;
-      double ts = P.tau[NU - 1], t1 = dtau[NU - 2];
+  for (int k = NU - 2; k >= 0; --k) ts = P.ftau[k] + sv * ts;
 #pragma unroll
-      for (int k = NU - 2; k >= 0; --k) ts = P.tau[k] + sv * ts;
+  for (int k = NU - 3; k >= 0; --k) t1 = P.fdtau[k] + sv * t1;
+  const float rinv = __builtin_amdgcn_rsqf(r2s);  // r2 >= 1e-2 for physical input
+  sa += qj * (rinv - ts);
+  const float qc = qj * (-rinv * rinv * rinv - t1);
+  sx += qc * dx, sy += qc * dy, sz += qc * dz;
+}
+
+template <int NU>
+__device__ __forceinline__ void pair_term(const DevParams &P, double4 aj, double ix, double iy,
+                                          double iz, double &sa, double &sx, double &sy,
+                                          double &sz) {
+  const double dx = aj.x - ix, dy = aj.y - iy, dz = aj.z - iz;
+  const double r2 = dx * dx + dy * dy + dz * dz;
+  const bool ok = r2 < P.a02 && r2 > 0.0;
+  const double r2s = ok ? r2 : P.a02;
+  const double qj = ok ? aj.w : 0.0;
+  const double sv = r2s * P.inv_a02 - 1.0;
+  double ts = P.tau_s[NU - 1], t1 = P.dtau_s[NU - 2];
 #pragma unroll
-      for (int k = NU - 3; k >= 0; --k) t1 = dtau[k] + sv * t1;
-      const double rinv = rsqrt_f64(r2);
-      acc += aj.w * (rinv - ts * P.inv_a0);
-      const double qc = aj.w * (-rinv * rinv * rinv - t1 * P.two_inv_a03);
-      fx += qc * dx, fy += qc * dy, fz += qc * dz;
-    }
+  for (int k = NU - 2; k >= 0; --k) ts = P.tau_s[k] + sv * ts;
+#pragma unroll
+  for (int k = NU - 3; k >= 0; --k) t1 = P.dtau_s[k] + sv * t1;
+  const double rinv = rsqrt_f64(r2s);
+  sa += qj * (rinv - ts);
+  const double qc = qj * (-rinv * rinv * rinv - t1);
+  sx += qc * dx, sy += qc * dy, sz += qc * dz;
+}
+
+// The staged atoms [0, npad) against the pass's G home atoms: lane l takes
+// staged atoms l, l + 64, ... and adds into s[g] = {phi, fx, fy, fz} (FP32
+// for the mixed kernel: at most CAP / 64 terms per run).
+template <bool MIXED, int NU, int G>
+__device__ __forceinline__ void pair_block(const DevParams &P,
+                                           const typename PairPrec<MIXED>::T *jbuf, int npad,
+                                           const typename PairPrec<MIXED>::R *ix,
+                                           const typename PairPrec<MIXED>::R *iy,
+                                           const typename PairPrec<MIXED>::R *iz,
+                                           typename PairPrec<MIXED>::R (*s)[4]) {
+  for (int e = threadIdx.x; e < npad; e += 64) {
+    const auto aj = jbuf[e];
+#pragma unroll
+    for (int g = 0; g < G; ++g)
+      pair_term<NU>(P, aj, ix[g], iy[g], iz[g], s[g][0], s[g][1], s[g][2], s[g][3]);
   }
 }
 
 template <bool MIXED, int NU>
-__global__ __launch_bounds__(64) void k_pair_cell(
-    DevParams P, const double4 *__restrict__ spos, const float4 *__restrict__ lpos,
-    const int *__restrict__ sidx, const int *__restrict__ start, const int4 *__restrict__ sten,
-    int nsten, double *__restrict__ F, double *__restrict__ phi, double *__restrict__ epart) {
-  using T = typename PairStage<MIXED>::T;
-  using S1 = decltype(T().x);
-  constexpr int CAP = PairStage<MIXED>::CAP;  // multiple of 64
+__device__ __forceinline__ void pair_run(const DevParams &P, const typename PairPrec<MIXED>::T *jbuf,
+                                         int npad, int ni, const typename PairPrec<MIXED>::R *ix,
+                                         const typename PairPrec<MIXED>::R *iy,
+                                         const typename PairPrec<MIXED>::R *iz,
+                                         typename PairPrec<MIXED>::R (*s)[4]) {
+  switch (ni) {
+    case 8: pair_block<MIXED, NU, 8>(P, jbuf, npad, ix, iy, iz, s); break;
+    case 7: pair_block<MIXED, NU, 7>(P, jbuf, npad, ix, iy, iz, s); break;
+    case 6: pair_block<MIXED, NU, 6>(P, jbuf, npad, ix, iy, iz, s); break;
+    case 5: pair_block<MIXED, NU, 5>(P, jbuf, npad, ix, iy, iz, s); break;
+    case 4: pair_block<MIXED, NU, 4>(P, jbuf, npad, ix, iy, iz, s); break;
+    case 3: pair_block<MIXED, NU, 3>(P, jbuf, npad, ix, iy, iz, s); break;
+    case 2: pair_block<MIXED, NU, 2>(P, jbuf, npad, ix, iy, iz, s); break;
+    default: pair_block<MIXED, NU, 1>(P, jbuf, npad, ix, iy, iz, s); break;
+  }
+}
+
+// Per-lane select without branches on the array (keeps it in registers).
+__device__ __forceinline__ float lane_pick(float a, float b, int up) {
+  const int x = __float_as_int(a), y = __float_as_int(b);
+  return __int_as_float((y & up) | (x & ~up));
+}
+__device__ __forceinline__ double lane_pick(double a, double b, long long up) {
+  const long long x = __double_as_longlong(a), y = __double_as_longlong(b);
+  return __longlong_as_double((y & up) | (x & ~up));
+}
+
+// One halving exchange: lanes with bit m set keep values [H, 2H), the others
+// [0, H), each summed with the partner lane's (lane ^ m) copy.
+template <int H, typename X, int NV>
+__device__ __forceinline__ void halve(X (&v)[NV], int lane, int m) {
+  using I = typename std::conditional<sizeof(X) == 8, long long, int>::type;
+  const I up = (lane & m) ? (I)-1 : (I)0;
+#pragma unroll
+  for (int k = 0; k < H; ++k) {
+    const X keep = lane_pick(v[k], v[k + H], up);
+    const X give = lane_pick(v[k + H], v[k], up);
+    v[k] = keep + __shfl_xor(give, m, 64);
+  }
+}
+
+// Sum over the wavefront of the run's partials s[g][d], scattered: lane l
+// receives value (l >> 1) & 31 (W: eight atoms) or (l >> 2) & 15 (four),
+// value 4 g + d.  The first exchange is FP32 for the mixed kernel (sums of
+// two lanes' partials), the rest FP64, in a fixed order.
+template <bool MIXED, bool W>
+__device__ __forceinline__ double wave_sum_scatter(typename PairPrec<MIXED>::R (&s)[kPassAtoms][4],
+                                                   int lane) {
+  using R = typename PairPrec<MIXED>::R;
+  constexpr int V = W ? 32 : 16;
+  R f[32];
+#pragma unroll
+  for (int k = 0; k < 32; ++k) f[k] = s[k >> 2][k & 3];
+  halve<V / 2>(f, lane, 32);
+  double d[16];
+#pragma unroll
+  for (int k = 0; k < V / 2; ++k) d[k] = (double)f[k];
+  halve<V / 4>(d, lane, 16);
+  halve<V / 8>(d, lane, 8);
+  halve<V / 16>(d, lane, 4);
+  if constexpr (W) halve<1>(d, lane, 2);
+  double r = d[0];
+  if constexpr (!W) r += __shfl_xor(r, 2, 64);
+  r += __shfl_xor(r, 1, 64);
+  return r;
+}
+
+template <bool MIXED, int NU>
+__global__ __launch_bounds__(64) void k_pair_cell(DevParams P, const double4 *__restrict__ spos,
+                                                  const int *__restrict__ sidx,
+                                                  const int *__restrict__ start,
+                                                  const int4 *__restrict__ cols, int ncol,
+                                                  double *__restrict__ F, double *__restrict__ phi,
+                                                  double *__restrict__ epart) {
+  using T = typename PairPrec<MIXED>::T;
+  using R = typename PairPrec<MIXED>::R;
+  constexpr int CAP = PairPrec<MIXED>::CAP;  // multiple of 64
+  constexpr int LB = 2;                      // candidate loads in flight per lane
   __shared__ T jbuf[CAP];
-  __shared__ int jidx[CAP];
-  __shared__ int seg_off[65], seg_beg[64];
-  __shared__ T seg_shift[64];
+  __shared__ int seg_off[kSegMax + 1], seg_beg[kSegMax];
+  __shared__ double seg_sx[kSegMax], seg_sy[kSegMax], seg_sz[kSegMax];
   const int c = blockIdx.x, lane = threadIdx.x;
   const int b0 = start[c], n = start[c + 1] - b0;
   const int cz = c % P.nc[2], cy = (c / P.nc[2]) % P.nc[1], cx = c / (P.nc[1] * P.nc[2]);
-  // bounding box of the home atoms: neighbour atoms farther than the cutoff
-  // from it interact with none of them and are not staged
-  S1 lo[3] = {(S1)1e30, (S1)1e30, (S1)1e30}, hi[3] = {(S1)-1e30, (S1)-1e30, (S1)-1e30};
-  for (int k = lane; k < n; k += 64) {
-    T v;
-    if constexpr (MIXED) v = lpos[b0 + k];
-    else v = spos[b0 + k];
-    lo[0] = fmin(lo[0], v.x), lo[1] = fmin(lo[1], v.y), lo[2] = fmin(lo[2], v.z);
-    hi[0] = fmax(hi[0], v.x), hi[1] = fmax(hi[1], v.y), hi[2] = fmax(hi[2], v.z);
-  }
-#pragma unroll
-  for (int off = 1; off < 64; off <<= 1)
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      lo[d] = fmin(lo[d], __shfl_xor(lo[d], off, 64));
-      hi[d] = fmax(hi[d], __shfl_xor(hi[d], off, 64));
-    }
-  // slightly enlarged cutoff: the box test must never drop a pair that the
-  // interaction test (different rounding) would keep
-  const S1 cut2 = (S1)P.a02 * (MIXED ? (S1)1.0001 : (S1)(1.0 + 1e-12));
+  // home-cell origin: all coordinates below are relative to it
+  const double ox = cx * P.cw[0], oy = cy * P.cw[1], oz = cz * P.cw[2];
+  const R cut2 = MIXED ? (R)P.fcut2 : (R)P.cut2;
+  const float fcw0 = (float)P.cw[0], fcw1 = (float)P.cw[1], ficw2 = (float)(1.0 / P.cw[2]);
+  const float fcut2 = P.fcut2;
+  // cell membership may be off by binning round-off; FP32 column trimming too
+  const float mrg = 1e-3f * (float)fmin(P.cw[0], fmin(P.cw[1], P.cw[2]));
   const unsigned long long lanes_below = (1ull << lane) - 1ull;
-  // padding atoms and idle home slots sit far apart and far from the box
-  const S1 far = (S1)1e18;
+  const R far = (R)1e18;  // padding atoms sit far from everything
   double esum = 0.0;
-  for (int i0 = 0; i0 < n;) {
-    const int rem = n - i0;
-    const int S = rem <= 16 ? 4 : (rem <= 32 ? 2 : 1);
-    const int a = lane / S, bs = lane - a * S;
-    const bool act = a < rem;
-    const int ii = b0 + i0 + a;
-    T ai;
-    if constexpr (MIXED) ai = act ? lpos[ii] : make_float4(-far, -far, -far, 0.f);
-    else ai = act ? spos[ii] : make_double4(-far, -far, -far, 0.0);
-    double acc = 0.0, fx = 0.0, fy = 0.0, fz = 0.0;
-    for (int o0 = 0; o0 < nsten;) {
-      // ---- stage: lane k takes stencil offset o0 + k
-      const int o = o0 + lane;
-      int cnt = 0, jb = 0;
-      T sh;
-      if (o < nsten) {
-        int4 d = sten[o];
-        int ox = cx + d.x, oy = cy + d.y, oz = cz + d.z;
-        int kx = floordiv(ox, P.nc[0]), ky = floordiv(oy, P.nc[1]), kz = floordiv(oz, P.nc[2]);
-        ox -= kx * P.nc[0], oy -= ky * P.nc[1], oz -= kz * P.nc[2];
-        int cn = (ox * P.nc[1] + oy) * P.nc[2] + oz;
-        jb = start[cn];
-        cnt = start[cn + 1] - jb;
-        if constexpr (MIXED)  // neighbour-cell origin relative to the home-cell origin
-          sh = make_float4(d.x * (float)P.cw[0], d.y * (float)P.cw[1], d.z * (float)P.cw[2], 0.f);
-        else  // periodic image of the neighbour cell
-          sh = make_double4(kx * P.box[0], ky * P.box[1], kz * P.box[2], 0.0);
-      }
-      int incl = cnt;
-      for (int off = 1; off < 64; off <<= 1) {
-        int t = __shfl_up(incl, off, 64);
-        if (lane >= off) incl += t;
-      }
-      unsigned long long fit = __ballot(o < nsten && incl <= CAP);
-      int m = __popcll(fit);  // leading lanes whose cells fit (counts are >= 0)
-      if (m == 0) m = 1;      // one cell larger than the chunk: read it in pieces below
-      const int total = __shfl(incl, m - 1, 64);
-      if (lane < m) {
-        seg_off[lane] = incl - cnt;
-        seg_beg[lane] = jb;
-        seg_shift[lane] = sh;
-      }
-      if (lane == 0) seg_off[m] = total;
-      __syncthreads();
-      for (int p0 = 0; p0 < total; p0 += CAP) {
-        const int ng = min(CAP, total - p0);
-        int np = 0;  // staged atoms that pass the bounding-box test
-        // the chunk's global loads are issued four at a time before any of
-        // them is used; the segment of element e only moves forward as e grows
-        constexpr int IT = CAP / 64, LB = 4;
-        int sl = 0;
+  for (int ip = 0; ip < n; ip += kPassAtoms) {
+    const int ni = min(kPassAtoms, n - ip);
+    R ix[kPassAtoms], iy[kPassAtoms], iz[kPassAtoms];
+    R lo[3] = {far, far, far}, hi[3] = {-far, -far, -far};
 #pragma unroll
-        for (int it0 = 0; it0 < IT; it0 += LB) {
-          T v[LB];
-          int jj[LB];
+    for (int k = 0; k < kPassAtoms; ++k) {
+      const double4 a = spos[b0 + ip + min(k, ni - 1)];
+      ix[k] = wave_uniform((R)(a.x - ox));
+      iy[k] = wave_uniform((R)(a.y - oy));
+      iz[k] = wave_uniform((R)(a.z - oz));
+      lo[0] = fmin(lo[0], ix[k]), hi[0] = fmax(hi[0], ix[k]);
+      lo[1] = fmin(lo[1], iy[k]), hi[1] = fmax(hi[1], iy[k]);
+      lo[2] = fmin(lo[2], iz[k]), hi[2] = fmax(hi[2], iz[k]);
+    }
+    const float flo0 = (float)lo[0], flo1 = (float)lo[1], flo2 = (float)lo[2];
+    const float fhi0 = (float)hi[0], fhi1 = (float)hi[1], fhi2 = (float)hi[2];
+    R s[kPassAtoms][4];
 #pragma unroll
-          for (int u = 0; u < LB; ++u) {
-            const int e = 64 * (it0 + u) + lane;
-            jj[u] = -1;
-            if (e < ng) {
-              const int g = p0 + e;
-              while (sl + 1 < m && seg_off[sl + 1] <= g) ++sl;
-              jj[u] = seg_beg[sl] + (g - seg_off[sl]);
-              if constexpr (MIXED) v[u] = lpos[jj[u]];
-              else v[u] = spos[jj[u]];
-              const T s2 = seg_shift[sl];
-              v[u].x += s2.x, v[u].y += s2.y, v[u].z += s2.z;
+    for (int k = 0; k < kPassAtoms; ++k) s[k][0] = s[k][1] = s[k][2] = s[k][3] = (R)0;
+    double tot = 0.0;  // this lane's share of the pass totals
+    const bool wide = ni > 4;
+    int np = 0;        // staged atoms in jbuf
+    int c0 = 0, g0 = 0, total = 0, sl = 0;
+    bool fresh = true, done = false;
+    while (!done) {
+      // ---- fill jbuf: rounds of 64 columns, batches of 64 x LB candidates
+      for (;;) {
+        if (fresh) {
+          if (c0 >= ncol) {
+            done = true;
+            break;
+          }
+          // lane k: column c0 + k -> at most two contiguous atom ranges
+          int beg0 = 0, cnt0 = 0, beg1 = 0, cnt1 = 0;
+          double sx = 0, sy = 0, sz0 = 0, sz1 = 0;
+          if (c0 + lane < ncol) {
+            const int4 col = cols[c0 + lane];  // dx, dy, dzlo, dzhi
+            const float gx = fmaxf(fmaxf(col.x * fcw0 - mrg - fhi0, flo0 - (col.x + 1) * fcw0 - mrg), 0.f);
+            const float gy = fmaxf(fmaxf(col.y * fcw1 - mrg - fhi1, flo1 - (col.y + 1) * fcw1 - mrg), 0.f);
+            const float rz2 = fcut2 - gx * gx - gy * gy;
+            if (rz2 > 0.f) {
+              const float rz = sqrtf(rz2) + mrg;
+              const int zl = max(col.z, (int)floorf((flo2 - rz) * ficw2));
+              const int zh = min(col.w, (int)floorf((fhi2 + rz) * ficw2));
+              if (zl <= zh) {
+                int X = cx + col.x, Y = cy + col.y;
+                const int kx = floordiv(X, P.nc[0]), ky = floordiv(Y, P.nc[1]);
+                X -= kx * P.nc[0], Y -= ky * P.nc[1];
+                const int base = (X * P.nc[1] + Y) * P.nc[2];
+                const int Z0 = cz + zl, Z1 = cz + zh;  // Z1 - Z0 < nc[2]: one wrap at most
+                const int kz0 = floordiv(Z0, P.nc[2]);
+                const int e0 = min(Z1, (kz0 + 1) * P.nc[2] - 1);
+                beg0 = start[base + Z0 - kz0 * P.nc[2]];
+                cnt0 = start[base + e0 - kz0 * P.nc[2] + 1] - beg0;
+                sx = kx * P.box[0] - ox, sy = ky * P.box[1] - oy, sz0 = kz0 * P.box[2] - oz;
+                if (e0 < Z1) {  // the rest lies in the next periodic image along z
+                  beg1 = start[base];
+                  cnt1 = start[base + Z1 - (kz0 + 1) * P.nc[2] + 1] - beg1;
+                  sz1 = (kz0 + 1) * P.box[2] - oz;
+                }
+              }
             }
           }
-#pragma unroll
-          for (int u = 0; u < LB; ++u) {
-            bool keep = false;
-            if (jj[u] >= 0) {
-              S1 bx = fmax(fmax(lo[0] - v[u].x, v[u].x - hi[0]), (S1)0);
-              S1 by = fmax(fmax(lo[1] - v[u].y, v[u].y - hi[1]), (S1)0);
-              S1 bz = fmax(fmax(lo[2] - v[u].z, v[u].z - hi[2]), (S1)0);
-              keep = bx * bx + by * by + bz * bz < cut2;
-            }
-            unsigned long long bm = __ballot(keep);
-            if (keep) {
-              int slot = np + __popcll(bm & lanes_below);
-              jbuf[slot] = v[u];
-              jidx[slot] = jj[u];
-            }
-            np += __popcll(bm);
+          // non-empty ranges only, in column order: (ranges << 24) + atoms
+          const int mine = ((int)(cnt0 > 0) + (int)(cnt1 > 0)) * (1 << 24) + cnt0 + cnt1;
+          int incl = mine;
+          for (int off = 1; off < 64; off <<= 1) {
+            const int t = __shfl_up(incl, off, 64);
+            if (lane >= off) incl += t;
           }
+          const int excl = incl - mine, last = __shfl(incl, 63, 64);
+          int k = excl >> 24;
+          const int o = excl & 0xffffff;
+          if (cnt0 > 0) {
+            seg_off[k] = o, seg_beg[k] = beg0;
+            seg_sx[k] = sx, seg_sy[k] = sy, seg_sz[k] = sz0;
+            ++k;
+          }
+          if (cnt1 > 0) {
+            seg_off[k] = o + cnt0, seg_beg[k] = beg1;
+            seg_sx[k] = sx, seg_sy[k] = sy, seg_sz[k] = sz1;
+          }
+          total = last & 0xffffff;
+          if (lane == 0) seg_off[last >> 24] = total;  // sentinel
+          __syncthreads();
+          g0 = 0, sl = 0, fresh = false;
         }
-        // pad to a multiple of 64 (np <= CAP, CAP a multiple of 64)
+        if (g0 >= total) {
+          __syncthreads();  // the tables are rewritten by the next round
+          c0 += 64, fresh = true;
+          continue;
+        }
+        if (np > CAP - 64 * LB) break;  // evaluate before the next batch
+        double4 a[LB];
+        int sg[LB];
+#pragma unroll
+        for (int u = 0; u < LB; ++u) {  // loads first (clamped index: always valid)
+          const int g = min(g0 + 64 * u + lane, total - 1);
+          while (seg_off[sl + 1] <= g) ++sl;  // the sentinel total > g ends it
+          sg[u] = sl;
+          a[u] = spos[(unsigned)(seg_beg[sl] + g - seg_off[sl])];
+        }
+#pragma unroll
+        for (int u = 0; u < LB; ++u) {
+          T v;
+          v.x = (R)(a[u].x + seg_sx[sg[u]]);
+          v.y = (R)(a[u].y + seg_sy[sg[u]]);
+          v.z = (R)(a[u].z + seg_sz[sg[u]]);
+          v.w = (R)a[u].w;
+          const R bx = fmax(fmax(lo[0] - v.x, v.x - hi[0]), (R)0);
+          const R by = fmax(fmax(lo[1] - v.y, v.y - hi[1]), (R)0);
+          const R bz = fmax(fmax(lo[2] - v.z, v.z - hi[2]), (R)0);
+          const bool keep = g0 + 64 * u + lane < total && bx * bx + by * by + bz * bz < cut2;
+          const unsigned long long bm = __ballot(keep);
+          if (keep) jbuf[np + __popcll(bm & lanes_below)] = v;
+          np += __popcll(bm);
+        }
+        g0 += 64 * LB;
+      }
+      if (np > 0) {
+        // ---- evaluate the staged atoms (padded to a multiple of 64)
         const int npad = (np + 63) & ~63;
         if (np + lane < npad) {
-          T v;
-          v.x = far, v.y = far, v.z = far, v.w = 0;
-          jbuf[np + lane] = v;
-          jidx[np + lane] = -1;
+          T pv;
+          pv.x = far, pv.y = far, pv.z = far, pv.w = 0;
+          jbuf[np + lane] = pv;
         }
         __syncthreads();
-        // ---- interactions of the staged atoms with this lane group's atom
-        if constexpr (MIXED) {
-          if (P.self_wrap) pair_run_fp32<true, NU>(P, jbuf, jidx, npad, bs, S, ai, ii, acc, fx, fy, fz);
-          else pair_run_fp32<false, NU>(P, jbuf, jidx, npad, bs, S, ai, ii, acc, fx, fy, fz);
-        } else {
-          if (P.self_wrap) pair_run_fp64<true, NU>(P, jbuf, jidx, npad, bs, S, ai, ii, acc, fx, fy, fz);
-          else pair_run_fp64<false, NU>(P, jbuf, jidx, npad, bs, S, ai, ii, acc, fx, fy, fz);
-        }
+        pair_run<MIXED, NU>(P, jbuf, npad, ni, ix, iy, iz, s);
         __syncthreads();
+        tot += wide ? wave_sum_scatter<MIXED, true>(s, lane) : wave_sum_scatter<MIXED, false>(s, lane);
+#pragma unroll
+        for (int k = 0; k < kPassAtoms; ++k) s[k][0] = s[k][1] = s[k][2] = s[k][3] = (R)0;
+        np = 0;
       }
-      o0 += m;
     }
-    // combine the S lanes of each group (consecutive lanes)
-    for (int off = 1; off < S; off <<= 1) {
-      acc += __shfl_xor(acc, off, 64);
-      fx += __shfl_xor(fx, off, 64);
-      fy += __shfl_xor(fy, off, 64);
-      fz += __shfl_xor(fz, off, 64);
-    }
-    if (act && bs == 0) {
-      double qi = ai.w;
-      int oi = sidx[ii];
-      if (F) {
-        F[3 * (long)oi] = qi * fx;
-        F[3 * (long)oi + 1] = qi * fy;
-        F[3 * (long)oi + 2] = qi * fz;
+    // ---- lane l holds value vi = 4 g + d of home atom g
+    const int vi = wide ? (lane >> 1) & 31 : (lane >> 2) & 15;
+    const bool writer = (lane & (wide ? 1 : 3)) == 0 && (vi >> 2) < ni;
+    if (writer) {
+      const int si = b0 + ip + (vi >> 2), d = vi & 3;
+      const double qi = spos[si].w;
+      const int oi = sidx[si];
+      if (d == 0) {
+        const double pv = tot - qi * P.self_img;
+        if (phi) phi[oi] = pv;
+        esum += 0.5 * qi * pv;
+      } else if (F) {
+        F[3 * (long)oi + d - 1] = qi * tot;
       }
-      if (phi) phi[oi] = acc;
-      esum += 0.5 * qi * acc;
     }
-    i0 += 64 / S;
   }
   for (int off = 32; off > 0; off >>= 1) esum += __shfl_down(esum, off, 64);
   if (lane == 0) epart[c] = esum;
 }
 
+// Pairs i < j whose nearest image lies inside the cutoff (the reference's
+// neighbour-list statistic), by the same cell columns.
 __global__ __launch_bounds__(kPairBS) void k_pair_count(
     DevParams P, const double4 *__restrict__ spos, const int *__restrict__ scell,
-    const int *__restrict__ start, const int4 *__restrict__ sten, int nsten,
+    const int *__restrict__ start, const int4 *__restrict__ cols, int ncol,
     unsigned long long *__restrict__ count) {
   int i = blockIdx.x * kPairBS + threadIdx.x;
   double n = 0;
@@ -554,22 +653,24 @@ __global__ __launch_bounds__(kPairBS) void k_pair_count(
     double4 ai = spos[i];
     int c = scell[i];
     int cz = c % P.nc[2], cy = (c / P.nc[2]) % P.nc[1], cx = c / (P.nc[1] * P.nc[2]);
-    for (int o = 0; o < nsten; ++o) {
-      int4 d = sten[o];
-      int ox = cx + d.x, oy = cy + d.y, oz = cz + d.z;
-      int kx = floordiv(ox, P.nc[0]), ky = floordiv(oy, P.nc[1]), kz = floordiv(oz, P.nc[2]);
-      ox -= kx * P.nc[0], oy -= ky * P.nc[1], oz -= kz * P.nc[2];
-      double bx = kx * P.box[0] - ai.x, by = ky * P.box[1] - ai.y, bz = kz * P.box[2] - ai.z;
-      int cn = (ox * P.nc[1] + oy) * P.nc[2] + oz;
-      for (int j = start[cn]; j < start[cn + 1]; ++j) {
-        if (j <= i) continue;
-        double4 aj = spos[j];
-        double dx = aj.x + bx, dy = aj.y + by, dz = aj.z + bz;
-        // count each pair once, at its nearest image
-        if (rint(dx * P.ibox[0]) != 0.0 || rint(dy * P.ibox[1]) != 0.0 ||
-            rint(dz * P.ibox[2]) != 0.0)
-          continue;
-        if (dx * dx + dy * dy + dz * dz < P.a02) n += 1.0;
+    for (int o = 0; o < ncol; ++o) {
+      const int4 col = cols[o];
+      for (int dz = col.z; dz <= col.w; ++dz) {
+        int ox = cx + col.x, oy = cy + col.y, oz = cz + dz;
+        int kx = floordiv(ox, P.nc[0]), ky = floordiv(oy, P.nc[1]), kz = floordiv(oz, P.nc[2]);
+        ox -= kx * P.nc[0], oy -= ky * P.nc[1], oz -= kz * P.nc[2];
+        double bx = kx * P.box[0] - ai.x, by = ky * P.box[1] - ai.y, bz = kz * P.box[2] - ai.z;
+        int cn = (ox * P.nc[1] + oy) * P.nc[2] + oz;
+        for (int j = start[cn]; j < start[cn + 1]; ++j) {
+          if (j <= i) continue;
+          double4 aj = spos[j];
+          double dx = aj.x + bx, dy = aj.y + by, dz2 = aj.z + bz;
+          // count each pair once, at its nearest image
+          if (rint(dx * P.ibox[0]) != 0.0 || rint(dy * P.ibox[1]) != 0.0 ||
+              rint(dz2 * P.ibox[2]) != 0.0)
+            continue;
+          if (dx * dx + dy * dy + dz2 * dz2 < P.a02) n += 1.0;
+        }
       }
     }
   }
@@ -1138,17 +1239,18 @@ __device__ __forceinline__ void csum_add(double &s, double &c, double s2, double
 // scheduling.  out = {E, constant part, pair part, grid part, flags}; the
 // non-finite-input counter is copied to out[4] and cleared for the next
 // evaluation.
-// Tree-combines the 256 threads' compensated pair sums (sv, cv) with the
+// Tree-combines the BS threads' compensated pair sums (sv, cv) with the
 // grid partials and writes out.
+template <int BS>
 __device__ void finalize_tail(const DevParams &P, const FinalizeArgs &f, double sv, double cv) {
-  __shared__ double ss[2][256], cc[2][256];
+  __shared__ double ss[2][BS], cc[2][BS];
   int t = threadIdx.x;
   ss[0][t] = sv, cc[0][t] = cv;
   double s = 0, c = 0;
-  for (int k = t; k < f.ng; k += 256) neumaier(s, c, f.eg[k]);
+  for (int k = t; k < f.ng; k += BS) neumaier(s, c, f.eg[k]);
   ss[1][t] = s, cc[1][t] = c;
   __syncthreads();
-  for (int h = 128; h > 0; h >>= 1) {
+  for (int h = BS / 2; h > 0; h >>= 1) {
     if (t < h)
       for (int w = 0; w < 2; ++w) {
         double a = ss[w][t], b = cc[w][t];
@@ -1180,14 +1282,11 @@ __device__ void finalize_tail(const DevParams &P, const FinalizeArgs &f, double 
   *f.flags = 0;
 }
 
-__device__ void finalize_body(const DevParams &P, const FinalizeArgs &f) {
+constexpr int kFinBS = 1024;
+__global__ __launch_bounds__(kFinBS) void k_finalize(DevParams P, FinalizeArgs f) {
   double s = 0, c = 0;
-  for (int k = threadIdx.x; k < f.np; k += 256) neumaier(s, c, f.ep[k]);
-  finalize_tail(P, f, s, c);
-}
-
-__global__ __launch_bounds__(256) void k_finalize(DevParams P, FinalizeArgs f) {
-  finalize_body(P, f);
+  for (int k = threadIdx.x; k < f.np; k += kFinBS) neumaier(s, c, f.ep[k]);
+  finalize_tail<kFinBS>(P, f, s, c);
 }
 
 // interpolate (forcefield1.c:570-607) + F += q E (:228-231), and the
@@ -1268,6 +1367,180 @@ __global__ __launch_bounds__(256) void k_gather(DevParams P, GridDims g,
   if (i < P.N) gather_atom<NU, ADD>(P, g, spos, sidx, e1, scal, F, phi, i);
 }
 
+// The same interpolation from an LDS copy of the level-1 grid around a tile
+// of pair-search cells.  A workgroup takes tb x tb x tb cells, copies the
+// grid points their atoms can reach -- the box [o, o + R) per axis, wrapped
+// periodically -- into LDS (coalesced along z), then its threads take the
+// tile's atoms (column by column, as sorted) and read their NU^3 grid values
+// from LDS instead of global memory.
+constexpr int kTileMax = 8;
+
+struct TileGeom {
+  int tb;         // cells per tile edge; 0: tiles do not fit, use k_gather
+  int ntile[3];   // tiles per axis
+  int R[3];       // LDS region extent per axis (grid points)
+};
+
+TileGeom gather_tiles(const DevParams &P, GridDims g) {
+  TileGeom tg{};
+  const int dims[3] = {g.x, g.y, g.z};
+  const double per_cell = (double)P.N / P.ncell;
+  for (int tb = kTileMax; tb >= 1; --tb) {
+    int R[3];
+    long vol = 1;
+    for (int a = 0; a < 3; ++a) {
+      R[a] = (int)std::ceil(tb * P.cw[a] * dims[a] / P.box[a]) + P.nu + 4;
+      vol *= R[a];
+    }
+    if (vol * (long)sizeof(double) > 48 * 1024 || R[2] > 64) continue;
+    if (tb > 1 && tb * tb * tb * per_cell > 768.0) continue;
+    tg.tb = tb;
+    for (int a = 0; a < 3; ++a) {
+      tg.R[a] = R[a];
+      tg.ntile[a] = (P.nc[a] + tb - 1) / tb;
+    }
+    return tg;
+  }
+  return tg;
+}
+
+template <int NU, bool ADD>
+__device__ __forceinline__ void gather_atom_tile(const DevParams &P, GridDims g,
+                                                 const double *reg, int Ry, int Rz,
+                                                 const int (&o)[3],
+                                                 const double4 *__restrict__ spos,
+                                                 const int *__restrict__ sidx,
+                                                 const double *__restrict__ scal,
+                                                 double *__restrict__ F, double *__restrict__ phi,
+                                                 int i) {
+  const double4 a = spos[i];
+  const double r[3] = {a.x, a.y, a.z};
+  const int dims[3] = {g.x, g.y, g.z};
+  double t[3];
+  int m[3];
+#pragma unroll
+  for (int ax = 0; ax < 3; ++ax) {
+    const double tt = dims[ax] * (r[ax] * P.ibox[ax]), mm = floor(tt);
+    t[ax] = tt - mm;
+    m[ax] = (int)mm - o[ax] + NU / 2;  // local index of weight 0's grid point
+  }
+  // all weights up front (register arrays, fully unrolled loops)
+  double wx[NU], dwx[NU], wy[NU], dwy[NU], wz[NU], dwz[NU];
+  bspline<NU, true>(P.Q, t[0], wx, dwx);
+  bspline<NU, true>(P.Q, t[1], wy, dwy);
+  bspline<NU, true>(P.Q, t[2], wz, dwz);
+  double ph = 0, ex = 0, ey = 0, ez = 0;
+#pragma unroll 1
+  for (int x = 0; x < NU; ++x) {
+    const int xo = (m[0] - x) * Ry;
+    double px = 0, py = 0, pz = 0;
+#pragma unroll
+    for (int y = 0; y < NU; ++y) {
+      const double *row = reg + (xo + m[1] - y) * Rz + m[2];
+      double s0 = 0, s1 = 0;
+#pragma unroll
+      for (int z = 0; z < NU; ++z) {
+        const double v = row[-z];
+        s0 += wz[z] * v;
+        s1 += dwz[z] * v;
+      }
+      px += wy[y] * s0;
+      py += dwy[y] * s0;
+      pz += wy[y] * s1;
+    }
+    ph += wx[0] * px;
+    ex += dwx[0] * px;
+    ey += wx[0] * py;
+    ez += wx[0] * pz;
+#pragma unroll
+    for (int k = 0; k + 1 < NU; ++k) wx[k] = wx[k + 1], dwx[k] = dwx[k + 1];  // next x weight first
+  }
+  const double Ex = -ex * g.x * P.ibox[0], Ey = -ey * g.y * P.ibox[1], Ez = -ez * g.z * P.ibox[2];
+  const int oi = sidx[i];
+  const double pv = ph + a.w * P.coeff_self - scal[0] * P.coeff2;
+  if (F) {
+    double *f = F + 3 * (long)oi;
+    f[0] = (ADD ? f[0] : 0.0) + a.w * Ex;
+    f[1] = (ADD ? f[1] : 0.0) + a.w * Ey;
+    f[2] = (ADD ? f[2] : 0.0) + a.w * Ez;
+  }
+  if (phi) phi[oi] = (ADD ? phi[oi] : 0.0) + pv;
+}
+
+template <int NU, bool ADD>
+__global__ __launch_bounds__(256) void k_gather_tile(DevParams P, GridDims g, TileGeom tg,
+                                                     const double4 *__restrict__ spos,
+                                                     const int *__restrict__ sidx,
+                                                     const int *__restrict__ start,
+                                                     const double *__restrict__ e1,
+                                                     const double *__restrict__ scal,
+                                                     double *__restrict__ F,
+                                                     double *__restrict__ phi) {
+  extern __shared__ double reg[];
+  __shared__ int seg_off[kTileMax * kTileMax + 1], seg_beg[kTileMax * kTileMax];
+  const int b = blockIdx.x, t = threadIdx.x;
+  const int tz = b % tg.ntile[2], ty = (b / tg.ntile[2]) % tg.ntile[1],
+            tx = b / (tg.ntile[1] * tg.ntile[2]);
+  const int c0[3] = {tx * tg.tb, ty * tg.tb, tz * tg.tb};
+  const int c1[3] = {min(P.nc[0], c0[0] + tg.tb), min(P.nc[1], c0[1] + tg.tb),
+                     min(P.nc[2], c0[2] + tg.tb)};
+  const int dims[3] = {g.x, g.y, g.z};
+  int o[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+    o[a] = (int)floor(c0[a] * P.cw[a] * (dims[a] * P.ibox[a])) - NU / 2 - 1;
+  const int Ry = tg.R[1], Rz = tg.R[2], nreg = tg.R[0] * Ry * Rz;
+  {  // eight loads in flight per thread (unconditional: clamped index); the
+     // divisions and periodic wraps by FP32 reciprocals, exact for these
+     // small integers
+    const float iRz = 1.0f / Rz, iRy = 1.0f / Ry;
+    const float igx = 1.0f / g.x, igy = 1.0f / g.y, igz = 1.0f / g.z;
+    auto wrapf = [](int v, int m, float im) { return v - m * (int)floorf((v + 0.5f) * im); };
+    for (int k0 = t; k0 < nreg; k0 += 256 * 8) {
+      double v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int k = min(k0 + 256 * u, nreg - 1);
+        const int q = (int)((k + 0.5f) * iRz), lz = k - q * Rz;
+        const int lx = (int)((q + 0.5f) * iRy), ly = q - lx * Ry;
+        v[u] = e1[((long)wrapf(o[0] + lx, g.x, igx) * g.y + wrapf(o[1] + ly, g.y, igy)) * g.z +
+                  wrapf(o[2] + lz, g.z, igz)];
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        if (k0 + 256 * u < nreg) reg[k0 + 256 * u] = v[u];
+    }
+  }
+  // the tile's cell columns (cx, cy, c0z .. c1z - 1) are contiguous atom ranges
+  const int ncy = c1[1] - c0[1], ncol = (c1[0] - c0[0]) * ncy;
+  if (t < 64) {
+    int beg = 0, cnt = 0;
+    if (t < ncol) {
+      const int base = ((c0[0] + t / ncy) * P.nc[1] + c0[1] + t % ncy) * P.nc[2];
+      beg = start[base + c0[2]];
+      cnt = start[base + c1[2]] - beg;
+    }
+    int incl = cnt;
+    for (int off = 1; off < 64; off <<= 1) {
+      const int v = __shfl_up(incl, off, 64);
+      if (t >= off) incl += v;
+    }
+    if (t < ncol) {
+      seg_off[t] = incl - cnt;
+      seg_beg[t] = beg;
+    }
+    if (t == ncol - 1) seg_off[ncol] = incl;
+  }
+  __syncthreads();
+  const int total = seg_off[ncol];
+  int sl = 0;
+  for (int k = t; k < total; k += 256) {
+    while (seg_off[sl + 1] <= k) ++sl;
+    gather_atom_tile<NU, ADD>(P, g, reg, Ry, Rz, o, spos, sidx, scal, F, phi,
+                              seg_beg[sl] + k - seg_off[sl]);
+  }
+}
+
 // F += Fp and phi += phip (the short-range parts, written by the pair kernel
 // beside the interpolation), then the energy reduction: every workgroup
 // sums a fixed slice of the per-cell pair partials into a compensated
@@ -1314,7 +1587,7 @@ __global__ __launch_bounds__(256) void k_combine(DevParams P, const double *__re
   // the pair partials are now the gridDim.x (sum, carry) pairs of ep2
   sv = 0.0, cv = 0.0;
   for (int k = t; k < (int)gridDim.x; k += 256) csum_add(sv, cv, ep2[2 * k], ep2[2 * k + 1]);
-  finalize_tail(P, fin, sv, cv);
+  finalize_tail<256>(P, fin, sv, cv);
 }
 
 inline unsigned nblk(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
@@ -1347,8 +1620,11 @@ void launch_scatter(const DevParams &P, const int *cell_of, const int *rank, con
 }
 
 void launch_cell_sort(const DevParams &P, const int *start, int *cnt, double4 *spos, int *sidx,
-                      float4 *lpos, hipStream_t s) {
-  k_cell_sort<<<nblk(P.ncell, 4), 256, 0, s>>>(P, start, cnt, spos, sidx, lpos);
+                      hipStream_t s) {
+  if (P.N <= 6L * P.ncell)  // a few atoms per cell: 16 lanes each
+    k_cell_sort<16><<<nblk(P.ncell, 16), 256, 0, s>>>(P, start, cnt, spos, sidx);
+  else
+    k_cell_sort<64><<<nblk(P.ncell, 4), 256, 0, s>>>(P, start, cnt, spos, sidx);
 }
 
 int pair_blocks(int N) { return (int)nblk(N, kPairBS); }
@@ -1362,31 +1638,27 @@ int pair_blocks(int N) { return (int)nblk(N, kPairBS); }
   }
 
 
-void launch_pair_fp64(const DevParams &P, const double4 *spos, const int *sidx, const int *scell,
-                      const int *start, const int4 *sten, int nsten, double *F, double *phi,
-                      double *epart, hipStream_t s) {
-  (void)scell;
-#define CALL(NU)                                                                             \
-  k_pair_cell<false, NU><<<P.ncell, 64, 0, s>>>(P, spos, nullptr, sidx, start, sten, nsten, F, \
-                                                 phi, epart)
+void launch_pair_fp64(const DevParams &P, const double4 *spos, const int *sidx, const int *start,
+                      const int4 *cols, int ncol, double *F, double *phi, double *epart,
+                      hipStream_t s) {
+#define CALL(NU) \
+  k_pair_cell<false, NU><<<P.ncell, 64, 0, s>>>(P, spos, sidx, start, cols, ncol, F, phi, epart)
   MSM_NU_SWITCH(P.nu, CALL)
 #undef CALL
 }
 
-void launch_pair_fp32(const DevParams &P, const float4 *lpos, const int *sidx, const int *scell,
-                      const int *start, const int4 *sten, int nsten, double *F, double *phi,
-                      double *epart, hipStream_t s) {
-  (void)scell;
-#define CALL(NU)                                                                             \
-  k_pair_cell<true, NU><<<P.ncell, 64, 0, s>>>(P, nullptr, lpos, sidx, start, sten, nsten, F,  \
-                                                phi, epart)
+void launch_pair_fp32(const DevParams &P, const double4 *spos, const int *sidx, const int *start,
+                      const int4 *cols, int ncol, double *F, double *phi, double *epart,
+                      hipStream_t s) {
+#define CALL(NU) \
+  k_pair_cell<true, NU><<<P.ncell, 64, 0, s>>>(P, spos, sidx, start, cols, ncol, F, phi, epart)
   MSM_NU_SWITCH(P.nu, CALL)
 #undef CALL
 }
 
 void launch_pair_count(const DevParams &P, const double4 *spos, const int *scell, const int *start,
-                       const int4 *sten, int nsten, unsigned long long *count, hipStream_t s) {
-  k_pair_count<<<pair_blocks(P.N), kPairBS, 0, s>>>(P, spos, scell, start, sten, nsten, count);
+                       const int4 *cols, int ncol, unsigned long long *count, hipStream_t s) {
+  k_pair_count<<<pair_blocks(P.N), kPairBS, 0, s>>>(P, spos, scell, start, cols, ncol, count);
 }
 
 void launch_spread(const DevParams &P, GridDims g, const double4 *spos, const int *start,
@@ -1453,8 +1725,21 @@ void launch_grid_energy(const DevParams &P, GridDims g, const double *q1, const 
 }
 
 void launch_gather(const DevParams &P, GridDims g, const double4 *spos, const int *sidx,
-                   const double *e1, const double *scal, double *F, double *phi, bool add,
-                   hipStream_t s) {
+                   const int *start, const double *e1, const double *scal, double *F,
+                   double *phi, bool add, hipStream_t s) {
+  const TileGeom tg = gather_tiles(P, g);
+  if (tg.tb > 0) {
+    const unsigned nb = (unsigned)(tg.ntile[0] * tg.ntile[1] * tg.ntile[2]);
+    const size_t lds = sizeof(double) * tg.R[0] * tg.R[1] * tg.R[2];
+#define CALL(NU)                                                                              \
+  if (add)                                                                                    \
+    k_gather_tile<NU, true><<<nb, 256, lds, s>>>(P, g, tg, spos, sidx, start, e1, scal, F, phi); \
+  else                                                                                        \
+    k_gather_tile<NU, false><<<nb, 256, lds, s>>>(P, g, tg, spos, sidx, start, e1, scal, F, phi)
+    MSM_NU_SWITCH(P.nu, CALL)
+#undef CALL
+    return;
+  }
 #define CALL(NU)                                                                        \
   if (add) k_gather<NU, true><<<nblk(P.N, 256), 256, 0, s>>>(P, g, spos, sidx, e1, scal, F, phi); \
   else k_gather<NU, false><<<nblk(P.N, 256), 256, 0, s>>>(P, g, spos, sidx, e1, scal, F, phi)
@@ -1470,7 +1755,7 @@ void launch_combine(const DevParams &P, const double *Fp, const double *phip, do
 }
 
 void launch_finalize(const DevParams &P, const FinalizeArgs &fin, hipStream_t s) {
-  k_finalize<<<1, 256, 0, s>>>(P, fin);
+  k_finalize<<<1, kFinBS, 0, s>>>(P, fin);
 }
 
 }  // namespace msm

@@ -39,6 +39,18 @@ def _dataset_system(name, nu, L, M, a0):
     return inputs.System(name, pos, q, edges, mass, dict(nu=nu, levels=L, top_grid=M, cutoff=a0))
 
 
+def _random_system(name, n, box, a0, levels, top, seed):
+    """Neutral random charges; small boxes exercise periodic self images
+    (box edges below the cutoff) and cells of a0/3 (dense)."""
+    r = np.random.default_rng(seed)
+    box = np.asarray(box, dtype=float)
+    pos = r.uniform(0, 1, (n, 3)) * box
+    q = r.uniform(-1, 1, n)
+    q -= q.mean()
+    return inputs.System(name, pos, q, np.diag(box), None,
+                         dict(nu=6, levels=levels, top_grid=top, cutoff=a0))
+
+
 CASES = {
     "water300": inputs.cfg0_water300,
     "crystal8192": inputs.cfg1_crystal8192,
@@ -47,6 +59,8 @@ CASES = {
     "spce300_L2_nu6": lambda: _dataset_system("spceN300", 6, 2, 6, 9.0),
     "spce300_L1_nu8_a11": lambda: _dataset_system("spceN300", 8, 1, 30, 11.0),
     "nacl512_L3_nu6": lambda: _dataset_system("NaClN512", 6, 3, 4, 3.0),
+    "tiny_box_self_images": lambda: _random_system("tiny", 60, (12.0, 13.0, 14.0), 13.0, 1, 8, 7),
+    "dense400_L2": lambda: _random_system("dense", 400, (10.0, 11.0, 12.0), 6.0, 2, 5, 7),
 }
 
 
@@ -171,7 +185,7 @@ def test_adjoint_central_differences(oracle_mod, torch_cuda):
 
 
 def test_mixed_precision_accuracy(oracle_mod, torch_cuda):
-    for case in ("water300", "crystal8192", "random3000"):
+    for case in ("water300", "crystal8192", "random3000", "tiny_box_self_images", "dense400_L2"):
         print(f"[parity] mixed {case}", flush=True)
         s = CASES[case]()
         m, o = _pair(s, oracle_mod, precision="mixed")
