@@ -85,7 +85,9 @@ struct Engine {
   int device = 0;
   hipStream_t stream = nullptr;    // the library's streams of this device (shared, not owned)
   hipStream_t stream2 = nullptr;   // short-range pairs run here, beside the grid chain
+  hipStream_t stream3 = nullptr;   // the level-1 stencil pass, beside the coarser levels
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  hipEvent_t ev_q1 = nullptr, ev_g1 = nullptr;  // q^1 ready; level-1 stencil pass done
   int nsten = 0;  // neighbour cell columns of the pair search
   // device buffers
   double *d_pos = nullptr, *d_q = nullptr, *d_F = nullptr, *d_phi = nullptr;
@@ -153,8 +155,10 @@ struct Engine {
     d_tw = d_topwork = nullptr;
     if (ev_fork) hipEventDestroy(ev_fork);
     if (ev_join) hipEventDestroy(ev_join);
-    ev_fork = ev_join = nullptr;
-    stream = stream2 = nullptr;
+    if (ev_q1) hipEventDestroy(ev_q1);
+    if (ev_g1) hipEventDestroy(ev_g1);
+    ev_fork = ev_join = ev_q1 = ev_g1 = nullptr;
+    stream = stream2 = stream3 = nullptr;
   }
 };
 
@@ -286,6 +290,7 @@ void fill_params(const msm::Plan &p, int N, msm::DevParams &P) {
   P.fa02 = (float)P.a02;
   P.fia02 = (float)P.inv_a02;
   P.fcut2 = (float)(P.a02 * 1.0001);
+  P.fr2min = 1e-4f;
 }
 
 // Uploads the box-dependent device data (stencils, twiddles, parameters).
@@ -334,31 +339,35 @@ int upload_box(Engine &E) {
   return MSM_OK;
 }
 
-// Tiles, kx groups and partial buffers of the MFMA level-stencil launch.
-// Two streams per device, created once and shared by every FF of the
-// process: the evaluation chain and the concurrent pair kernel.  HIP maps
-// streams onto a few hardware queues; creating two per FF would make later
-// FFs share a queue between the pair stream and the chain, which serialises
-// them.  FFs on one device are thereby ordered with each other.
-int device_streams(int dev, hipStream_t &main, hipStream_t &side) {
+// Three streams per device, created once and shared by every FF of the
+// process: the library's own chain stream (host-buffer entry points), the
+// concurrent pair kernel and the level-1 stencil pass.  HIP maps streams onto
+// a few hardware queues (four by default); creating streams per FF would make
+// later FFs share a queue between concurrent stages, which serialises them.
+// FFs on one device are thereby ordered with each other.
+int device_streams(int dev, hipStream_t (&out)[3]) {
   static std::mutex mu;
-  static std::vector<std::pair<hipStream_t, hipStream_t>> tab;
+  struct Trio {
+    hipStream_t s[3] = {nullptr, nullptr, nullptr};
+  };
+  static std::vector<Trio> tab;
   std::lock_guard<std::mutex> lk(mu);
-  if ((int)tab.size() <= dev) tab.resize(dev + 1, {nullptr, nullptr});
-  auto &t = tab[dev];
-  if (!t.first) {
-    hipStream_t a = nullptr, b = nullptr;
-    HIP_TRY(hipStreamCreateWithFlags(&a, hipStreamNonBlocking));
-    if (hipStreamCreateWithFlags(&b, hipStreamNonBlocking) != hipSuccess) {
-      hipStreamDestroy(a);
-      return fail(MSM_ERR_HIP, "hipStreamCreateWithFlags failed");
-    }
-    t = {a, b};
+  if ((int)tab.size() <= dev) tab.resize(dev + 1);
+  Trio &t = tab[dev];
+  if (!t.s[0]) {
+    Trio n;
+    for (int k = 0; k < 3; ++k)
+      if (hipStreamCreateWithFlags(&n.s[k], hipStreamNonBlocking) != hipSuccess) {
+        for (int j = 0; j < k; ++j) hipStreamDestroy(n.s[j]);
+        return fail(MSM_ERR_HIP, "hipStreamCreateWithFlags failed");
+      }
+    t = n;
   }
-  main = t.first;
-  side = t.second;
+  for (int k = 0; k < 3; ++k) out[k] = t.s[k];
   return MSM_OK;
 }
+
+// Tiles, kx groups and partial buffers of the MFMA level-stencil launch.
 
 int plan_levels(Engine &E) {
   const msm::Plan &p = E.plan;
@@ -397,9 +406,13 @@ int create_engine(FFX *x, const msm::Plan &plan) {
   E->plan = plan;
   x->eng = E;
   HIP_TRY(hipGetDevice(&E->device));
-  TRY(device_streams(E->device, E->stream, E->stream2));
-  HIP_TRY(hipEventCreateWithFlags(&E->ev_fork, hipEventDisableTiming));
-  HIP_TRY(hipEventCreateWithFlags(&E->ev_join, hipEventDisableTiming));
+  {
+    hipStream_t st[3];
+    TRY(device_streams(E->device, st));
+    E->stream = st[0], E->stream2 = st[1], E->stream3 = st[2];
+  }
+  for (hipEvent_t *e : {&E->ev_fork, &E->ev_join, &E->ev_q1, &E->ev_g1})
+    HIP_TRY(hipEventCreateWithFlags(e, hipEventDisableTiming));
   const msm::Plan &p = E->plan;
   for (int l = 1; l <= p.L; ++l) {
     E->gd[l] = {p.M[l][0], p.M[l][1], p.M[l][2]};
@@ -475,31 +488,49 @@ void stage_pairs(Engine &E, int precision, double *F, double *phi, hipStream_t s
                           E.d_epair, s);
 }
 
-// Spread, restrict to the top, top-level DFT convolution, then prolongate
-// and add each level's stencil pass down to e^1.
-// Charges down the hierarchy: spread to level 1, restrict to the top.
-void stage_charges(Engine &E, Timing &T, hipStream_t s) {
+// The levels [first, last) of the level-stencil launch description, as a
+// launch of their own (task numbering rebased to the first of them).
+msm::G2GArgs g2g_levels(const msm::G2GArgs &A, int first, int last) {
+  msm::G2GArgs R{};
+  R.nlev = 0;
+  R.ntasks = 0;
+  for (int k = first; k < last && k < A.nlev; ++k) {
+    R.lv[R.nlev] = A.lv[k];
+    R.lv[R.nlev].task0 = R.ntasks;
+    R.ntasks += (long)A.lv[k].g.x * A.lv[k].ty * A.lv[k].tz * A.lv[k].G;
+    ++R.nlev;
+  }
+  return R;
+}
+
+// The grid chain: spread to level 1, restrict to the top, top-level DFT
+// convolution, then prolongate and add each level's stencil pass down to
+// e^1.  The level-1 stencil pass, the largest, needs q^1 only: it runs on the
+// third stream beside the restrictions, the coarser levels and their
+// prolongations, and is joined before the level-1 prolongation.
+int stage_grids(Engine &E, int precision, int layout, Timing &T, hipStream_t s) {
   const msm::Plan &p = E.plan;
+  const int L = p.L;
+  const int pr = precision == MSM_PREC_MIXED ? msm::MSM_G2G_BF16X3 : msm::MSM_G2G_FP64;
   int t = T.open(s);
   msm::launch_spread(E.P, E.gd[1], E.d_spos, E.d_start, E.d_ql[1], s);
   T.close(t, "anterpolation", s);
-  for (int l = 2; l <= p.L; ++l) {
+  if (L >= 2) {
+    HIP_TRY(hipEventRecord(E.ev_q1, s));
+    HIP_TRY(hipStreamWaitEvent(E.stream3, E.ev_q1, 0));
+    t = T.open(E.stream3);
+    msm::launch_g2g_tiles(pr, g2g_levels(E.g2g[pr], 0, 1), E.g2g_lds[pr], E.stream3);
+    T.close(t, "grid2grid", E.stream3);
+    HIP_TRY(hipEventRecord(E.ev_g1, E.stream3));
+  }
+  for (int l = 2; l <= L; ++l) {
     t = T.open(s);
     msm::launch_restrict(E.P, E.gd[l - 1], E.gd[l], E.d_ql[l - 1], E.d_ql[l], s);
     T.close(t, "restriction", s);
   }
-}
-
-// Potentials up the hierarchy: level stencils, top-level DFT, prolongation.
-void stage_potentials(Engine &E, int precision, int layout, Timing &T, hipStream_t s) {
-  const msm::Plan &p = E.plan;
-  int L = p.L;
-  int pr = precision == MSM_PREC_MIXED ? msm::MSM_G2G_BF16X3 : msm::MSM_G2G_FP64;
-  int t;
-  if (L >= 2) {
-    // stencil correlations of every level l < L in one launch
+  if (L >= 3) {  // stencil passes of the levels 2 .. L-1 in one launch
     t = T.open(s);
-    msm::launch_g2g_tiles(pr, E.g2g[pr], E.g2g_lds[pr], s);
+    msm::launch_g2g_tiles(pr, g2g_levels(E.g2g[pr], 1, L - 1), E.g2g_lds[pr], s);
     T.close(t, "grid2grid", s);
   }
   t = T.open(s);
@@ -507,6 +538,7 @@ void stage_potentials(Engine &E, int precision, int layout, Timing &T, hipStream
                   layout == MSM_LAYOUT_ASIS && L >= 2, s);
   T.close(t, "grid2grid_top", s);
   for (int l = L - 1; l >= 1; --l) {
+    if (l == 1) HIP_TRY(hipStreamWaitEvent(s, E.ev_g1, 0));
     t = T.open(s);
     // the finest level also writes the grid-energy partials (sum q1 e1)
     msm::launch_prolong_sum(E.P, E.gd[l], E.gd[l + 1], E.g2g[pr].lv[l - 1].G, E.d_part[pr][l],
@@ -514,11 +546,7 @@ void stage_potentials(Engine &E, int precision, int layout, Timing &T, hipStream
                             l == 1 ? E.d_egrid : nullptr, s);
     T.close(t, "prolongation", s);
   }
-}
-
-void stage_grids(Engine &E, int precision, int layout, Timing &T, hipStream_t s) {
-  stage_charges(E, T, s);
-  stage_potentials(E, precision, layout, T, s);
+  return MSM_OK;
 }
 
 // Grid energy: with L >= 2 the level-1 prolongation has written one partial
@@ -588,8 +616,7 @@ int eval_chain(FFX *x, const double *pos, const double *q, double *out, double *
   // next to its resident wavefronts); it writes the short-range parts of F
   // and phi to its own buffers, added after the join
   TRY(fork_pairs(E, x->precision, F ? E.d_Fp : nullptr, phi ? E.d_phip : nullptr, s, T));
-  stage_charges(E, T, s);
-  stage_potentials(E, x->precision, x->layout, T, s);
+  TRY(stage_grids(E, x->precision, x->layout, T, s));
   stage_grid_energy(E, x->layout, s);
   if (F || phi) {
     t = T.open(s);
@@ -614,7 +641,7 @@ void publish_timing(FFX *x, Timing &T) {
   for (int l = 0; l < 10; ++l)
     ff->time_grid2grid[l] = ff->time_restriction[l] = ff->time_prolongation[l] =
         ff->time_padding[l] = 0;
-  int L = x->eng->plan.L, lr = 2, lg = L - 1;
+  int L = x->eng->plan.L, lr = 2, lg = L - 1, lc = 1;
   for (auto &v : T.iv) {
     float ms = 0;
     hipEventElapsedTime(&ms, v.a, v.b);
@@ -625,7 +652,10 @@ void publish_timing(FFX *x, Timing &T) {
     else if (t == "anterpolation") ff->time_anterpolation = sec;
     else if (t == "restriction") { if (lr <= 9) ff->time_restriction[lr] = sec; ++lr; }
     else if (t == "grid2grid_top") { if (L <= 9) ff->time_grid2grid[L] = sec; }
-    else if (t == "grid2grid") ff->time_grid2grid[1] = sec;  // all levels l < L, one launch
+    else if (t == "grid2grid") {  // level 1, then the levels 2 .. L-1 in one launch
+      if (lc <= 9) ff->time_grid2grid[lc] = sec;
+      ++lc;
+    }
     else if (t == "prolongation") { if (lg >= 1 && lg <= 9) ff->time_prolongation[lg] = sec; --lg; }
     else if (t == "interpolation") ff->time_interpolation = sec;
   }
@@ -1088,8 +1118,7 @@ int msm_forward_device(FF *ff, const double *d_pos, const double *d_q, double *d
   Timing T;
   stage_prepare(E, d_pos, d_q, s);
   TRY(fork_pairs(E, x->precision, nullptr, nullptr, s, T));
-  stage_charges(E, T, s);
-  stage_potentials(E, x->precision, x->layout, T, s);
+  TRY(stage_grids(E, x->precision, x->layout, T, s));
   stage_grid_energy(E, x->layout, s);
   HIP_TRY(hipStreamWaitEvent(s, E.ev_join, 0));
   stage_energy(E, nullptr, nullptr, d_energy, s);
@@ -1112,7 +1141,7 @@ int msm_adjoint_device(FF *ff, const double *d_pos, const double *d_q, double *d
     // grid levels are not stored: rebuild the ordering and e^1 from scratch
     stage_prepare(E, d_pos, d_q, s);
     msm::launch_fill_int(E.d_flags, 1, 0, s);  // no energy reduction follows to clear it
-    stage_grids(E, x->precision, x->layout, T, s);
+    TRY(stage_grids(E, x->precision, x->layout, T, s));
   }
   stage_pairs(E, x->precision, d_force, d_phi, s);  // the pair kernel writes, gather adds
   if (d_force || d_phi)
