@@ -944,9 +944,11 @@ __global__ __launch_bounds__(256) void k_spread_mfma(DevParams P, GridDims g,
 // contracts z, y and x in turn.  The small brick keeps the LDS footprint
 // (7.4 KB at NU = 6) low enough to run beside the pair kernel.
 constexpr int kRB = 2;
+constexpr int kRBS = 64;  // one wavefront per brick: the work is small, and few
+                          // wavefronts find slots quickly beside the pair kernel
 
 template <int NU>
-__global__ __launch_bounds__(256) void k_restrict(DevParams P, GridDims f, GridDims c,
+__global__ __launch_bounds__(kRBS) void k_restrict(DevParams P, GridDims f, GridDims c,
                                                   const double *__restrict__ qf,
                                                   double *__restrict__ qc) {
   // coarse brick of kRB^3 points; fine indices 2m + n - H, n in [0, NU]
@@ -961,12 +963,12 @@ __global__ __launch_bounds__(256) void k_restrict(DevParams P, GridDims f, GridD
   b /= nbz;
   const int cy0 = (b % nby) * kRB, cx0 = (b / nby) * kRB;
   const int fx0 = 2 * cx0 - H, fy0 = 2 * cy0 - H, fz0 = 2 * cz0 - H;
-  for (int k = tid; k < FW * FW * FW; k += 256) {
+  for (int k = tid; k < FW * FW * FW; k += kRBS) {
     const int a = k / (FW * FW), bb = (k / FW) % FW, cc = k % FW;
     fs[k] = qf[((long)wrapi(fx0 + a, f.x) * f.y + wrapi(fy0 + bb, f.y)) * f.z + wrapi(fz0 + cc, f.z)];
   }
   __syncthreads();
-  for (int k = tid; k < FW * FW * kRB; k += 256) {  // along z
+  for (int k = tid; k < FW * FW * kRB; k += kRBS) {  // along z
     const int cz = k % kRB, fxy = k / kRB;
     const double *row = fs + fxy * FW + 2 * cz;
     double acc = 0.0;
@@ -975,7 +977,7 @@ __global__ __launch_bounds__(256) void k_restrict(DevParams P, GridDims f, GridD
     t1[k] = acc;
   }
   __syncthreads();
-  for (int k = tid; k < FW * kRB * kRB; k += 256) {  // along y
+  for (int k = tid; k < FW * kRB * kRB; k += kRBS) {  // along y
     const int cz = k % kRB, cy = (k / kRB) % kRB, fx = k / (kRB * kRB);
     double acc = 0.0;
 #pragma unroll
@@ -1699,7 +1701,7 @@ void launch_restrict(const DevParams &P, GridDims fine, GridDims coarse, const d
                      double *qc, hipStream_t s) {
   const unsigned nb = (unsigned)(((coarse.x + kRB - 1) / kRB) * ((coarse.y + kRB - 1) / kRB) *
                                  ((coarse.z + kRB - 1) / kRB));
-#define CALL(NU) k_restrict<NU><<<nb, 256, 0, s>>>(P, fine, coarse, qf, qc)
+#define CALL(NU) k_restrict<NU><<<nb, kRBS, 0, s>>>(P, fine, coarse, qf, qc)
   MSM_NU_SWITCH(P.nu, CALL)
 #undef CALL
 }
