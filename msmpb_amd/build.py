@@ -17,7 +17,8 @@ CLI = os.path.join(HERE, "bin", "msmpb")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("MSMPB_OFFLOAD_ARCH", "gfx950")
 
-HIP_SOURCES = ["msm_engine.hip", "msm_kernels.hip", "msm_mfma.hip", "msm_plan.cpp"]
+HIP_SOURCES = ["msm_engine.hip", "msm_kernels.hip", "msm_mfma.hip", "msm_stencil.hip",
+               "msm_plan.cpp"]
 HIP_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared",
     # packed FP32 arithmetic stays disabled on this target

@@ -107,6 +107,9 @@ struct Engine {
   void *d_ksplit[msm::kMaxLevel + 1] = {};
   void *d_qsplit = nullptr;
   double *d_tw = nullptr, *d_topwork = nullptr;
+  // FF_rebuild on the GPU: omega' and cL tables, stencil scratch (2 x S^3)
+  double *d_stab = nullptr, *d_sscr = nullptr;
+  size_t stab_cap = 0, sscr_cap = 0;
   // level stencils on MFMA, per precision (0: FP64, 1: BF16 hi/lo)
   msm::G2GArgs g2g[2]{};
   size_t g2g_lds[2] = {0, 0};
@@ -126,7 +129,7 @@ struct Engine {
     void *ptrs[] = {d_pos,  d_q,    d_F,    d_phi,   d_out,  d_qpart, d_scal,  d_epair,
                     d_egrid, d_cell_of, d_cnt, d_start, d_sidx, d_scell, d_flags, d_wpos,
                     d_rank, d_done, d_Fp, d_phip, d_ep2,
-                    d_spos, d_sten, d_count, d_qsplit, d_tw, d_topwork};
+                    d_spos, d_sten, d_count, d_qsplit, d_tw, d_topwork, d_stab, d_sscr};
     for (void *p : ptrs)
       if (p) hipFree(p);
     for (int pr = 0; pr < 2; ++pr)
@@ -153,6 +156,8 @@ struct Engine {
     d_count = nullptr;
     d_qsplit = nullptr;
     d_tw = d_topwork = nullptr;
+    d_stab = d_sscr = nullptr;
+    stab_cap = sscr_cap = 0;
     if (ev_fork) hipEventDestroy(ev_fork);
     if (ev_join) hipEventDestroy(ev_join);
     if (ev_q1) hipEventDestroy(ev_q1);
@@ -336,6 +341,85 @@ int upload_box(Engine &E) {
       HIP_TRY(hipMemcpy(E.d_bfrag[pr][l], host.data(), bytes, hipMemcpyHostToDevice));
       lv.bfrag = E.d_bfrag[pr][l];
     }
+  return MSM_OK;
+}
+
+// FF_rebuild's device path.  The host plan (rebuild_plan without stencils)
+// supplies the constants, omega' and cL; the level stencils, the top-level
+// kernel and the Toeplitz B operands of both precisions are generated on
+// the GPU into the buffers FF_build allocated (a rebuild keeps nu, the grids
+// and nLim, so every size is unchanged), then copied back into the plan's
+// host khat, which FF::khat exposes.  Blocking, like the reference's
+// FF_rebuild: it first waits for work in flight that may read the stencils.
+int rebuild_box(Engine &E) {
+  msm::Plan &p = E.plan;
+  const hipStream_t s = E.stream;
+  HIP_TRY(hipDeviceSynchronize());
+  const int ncell_alloc = E.P.ncell;
+  fill_params(p, p.N, E.P);
+  std::vector<int4> sten;
+  pair_geometry(p, E.P, sten);
+  if (E.P.ncell > ncell_alloc)
+    return fail(MSM_ERR_ARG, "box change enlarges the pair-search cell grid beyond its buffers; "
+                             "rebuild the FF with FF_build");
+  E.nsten = (int)sten.size();
+  if (E.d_sten) hipFree(E.d_sten), E.d_sten = nullptr;
+  TRY(dalloc(&E.d_sten, sten.size()));
+  HIP_TRY(hipMemcpy(E.d_sten, sten.data(), sten.size() * sizeof(int4), hipMemcpyHostToDevice));
+  // omega'_l along x, y, z for the levels below the top, then cL_x, cL_y, cL_z
+  std::vector<double> tab;
+  size_t woff[msm::kMaxLevel + 1][3] = {}, cloff = 0;
+  size_t smax = 1;
+  for (int l = 1; l < p.L; ++l) {
+    for (int a = 0; a < 3; ++a) {
+      woff[l][a] = tab.size();
+      tab.insert(tab.end(), p.omegap[l][a].begin(), p.omegap[l][a].end());
+    }
+    smax = std::max(smax, (size_t)p.S[l][0] * p.S[l][1] * p.S[l][2]);
+  }
+  cloff = tab.size();
+  for (int a = 0; a < 3; ++a) tab.insert(tab.end(), p.cL[a].begin(), p.cL[a].end());
+  if (tab.size() > E.stab_cap) {
+    if (E.d_stab) hipFree(E.d_stab), E.d_stab = nullptr;
+    TRY(dalloc(&E.d_stab, tab.size()));
+    E.stab_cap = tab.size();
+  }
+  if (2 * smax > E.sscr_cap) {
+    if (E.d_sscr) hipFree(E.d_sscr), E.d_sscr = nullptr;
+    TRY(dalloc(&E.d_sscr, 2 * smax));
+    E.sscr_cap = 2 * smax;
+  }
+  HIP_TRY(hipMemcpy(E.d_stab, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice));
+  msm::StencilBox b{};
+  std::memcpy(b.A, p.A, sizeof b.A);
+  std::memcpy(b.Ai, p.Ai, sizeof b.Ai);
+  for (int a = 0; a < 3; ++a)
+    b.as[a] = std::sqrt(p.Ai[0][a] * p.Ai[0][a] + p.Ai[1][a] * p.Ai[1][a] + p.Ai[2][a] * p.Ai[2][a]);
+  for (int l = 0; l <= p.L; ++l) b.aCut[l] = p.aCut[l];
+  for (int k = 0; k < p.nu; ++k) b.tau[k] = p.tau[k];
+  msm::top_series(p.nu, b.taylor);
+  b.detA = p.detA;
+  b.nu = p.nu;
+  b.L = p.L;
+  std::memcpy(b.M, p.M, sizeof b.M);
+  std::memcpy(b.S, p.S, sizeof b.S);
+  for (int l = 1; l < p.L; ++l)
+    msm::launch_stencil_level(b, l, E.d_stab + woff[l][0], E.d_stab + woff[l][1],
+                              E.d_stab + woff[l][2], E.d_sscr, E.d_sscr + smax, E.d_khat[l], s);
+  msm::launch_stencil_top(b, E.d_stab + cloff, E.d_khat[p.L], s);
+  for (int pr = 0; pr < 2; ++pr)
+    for (int l = 1; l < p.L; ++l)
+      msm::launch_g2g_bfrag_fill(pr, E.g2g[pr].lv[l - 1], E.d_khat[l], E.d_bfrag[pr][l], s);
+  for (int l = 1; l <= p.L; ++l) {
+    const size_t n = l < p.L ? (size_t)p.S[l][0] * p.S[l][1] * p.S[l][2]
+                             : (size_t)p.M[l][0] * p.M[l][1] * p.M[l][2];
+    p.khat[l].resize(n);
+    HIP_TRY(hipMemcpyAsync(p.khat[l].data(), E.d_khat[l], n * sizeof(double),
+                           hipMemcpyDeviceToHost, s));
+  }
+  HIP_TRY(hipStreamSynchronize(s));
+  HIP_TRY(hipGetLastError());
+  E.kept = false;
   return MSM_OK;
 }
 
@@ -890,7 +974,7 @@ void FF_rebuild(FF *ff, double edges[3][3], double (*position)[3], double margin
   FFX *x = X(ff);
   std::string err;
   Engine &E = *x->eng;
-  if (!msm::rebuild_plan(E.plan, edges, err) || upload_box(E) != MSM_OK) {
+  if (!msm::rebuild_plan(E.plan, edges, err, /*stencils=*/false) || rebuild_box(E) != MSM_OK) {
     if (!err.empty()) g_err = "FF_rebuild: " + err;
     x->built = false;
     std::fprintf(stderr, "msmpb_hip: %s\n", g_err.c_str());

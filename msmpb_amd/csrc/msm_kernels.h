@@ -4,6 +4,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include "msm_plan.h"  // kMaxLevel
+
 namespace msm {
 
 constexpr int kMaxNu = 10;
@@ -158,5 +160,30 @@ void launch_finalize(const DevParams &P, const FinalizeArgs &fin, hipStream_t s)
 int combine_blocks(int N);
 void launch_combine(const DevParams &P, const double *Fp, const double *phip, double *F,
                     double *phi, double *ep2, const FinalizeArgs &fin, hipStream_t s);
+
+
+// ---- box-dependent stencils on the GPU (FF_rebuild) ----
+struct StencilBox {
+  double A[3][3], Ai[3][3];  // box matrix (columns = box vectors) and inverse
+  double as[3];              // column norms of Ai: the image search extent
+  double aCut[kMaxLevel + 1];
+  double tau[kMaxNu];        // softening series tau_0 .. tau_{nu-1}
+  double taylor[7];          // top-level kernel series coefficients
+  double detA;
+  int nu, L;
+  int M[kMaxLevel + 1][3];   // level grids
+  int S[kMaxLevel + 1][3];   // level stencil extents
+};
+// Level l's stencil into khat (S_l^3): image sum, then omega' along z (w2),
+// y (w1) and x (w0); tmp_a, tmp_b hold S_l^3 doubles each.
+void launch_stencil_level(const StencilBox &b, int l, const double *w0, const double *w1,
+                          const double *w2, double *tmp_a, double *tmp_b, double *khat,
+                          hipStream_t s);
+// Top-level Fourier kernel into khat (M_L^3); cL = cL_x, cL_y, cL_z packed.
+void launch_stencil_top(const StencilBox &b, const double *cL, double *khat, hipStream_t s);
+// Toeplitz B operands of a level stencil (device khat) in MFMA lane order,
+// the same values g2g_bfrag_fill writes on the host.
+void launch_g2g_bfrag_fill(int prec, const G2GLevel &L, const double *khat, void *out,
+                           hipStream_t s);
 
 }  // namespace msm

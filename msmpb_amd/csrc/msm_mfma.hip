@@ -325,6 +325,44 @@ void g2g_bfrag_fill(int prec, const G2GLevel &L, const double *khat, void *host_
       }
 }
 
+namespace {
+// Device twin of g2g_bfrag_fill: one thread per (fragment, lane).
+template <int PREC>
+__global__ __launch_bounds__(256) void k_g2g_bfrag_fill(G2GLevel L, const double *__restrict__ khat,
+                                                        void *__restrict__ out) {
+  constexpr int KC = Cfg<PREC>::KC, KPL = Cfg<PREC>::KPL;
+  const long t = (long)blockIdx.x * 256 + threadIdx.x;
+  const long nfrag = (long)L.s.x * L.s.y * L.nch;
+  if (t >= nfrag * 64) return;
+  const long f = t / 64;
+  const int lane = (int)(t % 64), j = lane % 16;
+  const int ch = (int)(f % L.nch), ky = (int)((f / L.nch) % L.s.y), kx = (int)(f / ((long)L.nch * L.s.y));
+  for (int e = 0; e < KPL; ++e) {
+    const int k = ch * KC + (lane / 16) * KPL + e, kz = k - j;
+    const double v = (kz >= 0 && kz < L.s.z) ? khat[((long)kx * L.s.y + ky) * L.s.z + kz] : 0.0;
+    if constexpr (PREC == MSM_G2G_FP64) {
+      static_cast<double *>(out)[f * 64 + lane] = v;
+    } else {
+      uint16_t *o = static_cast<uint16_t *>(out);
+      const float fv = (float)v;
+      const uint16_t hi = bf16_bits(fv), lo = bf16_bits(fv - bf16_float(hi));
+      o[(f * 128 + lane) * 8 + e] = hi;
+      o[(f * 128 + 64 + lane) * 8 + e] = lo;
+    }
+  }
+}
+}  // namespace
+
+void launch_g2g_bfrag_fill(int prec, const G2GLevel &L, const double *khat, void *out,
+                           hipStream_t s) {
+  const long n = (long)L.s.x * L.s.y * L.nch * 64;
+  const unsigned nb = (unsigned)((n + 255) / 256);
+  if (prec == MSM_G2G_FP64)
+    k_g2g_bfrag_fill<MSM_G2G_FP64><<<nb, 256, 0, s>>>(L, khat, out);
+  else
+    k_g2g_bfrag_fill<MSM_G2G_BF16X3><<<nb, 256, 0, s>>>(L, khat, out);
+}
+
 size_t g2g_lds_bytes(int prec, const G2GLevel &L) {
   int R = 16 + L.s.y - 1;
   if (prec == MSM_G2G_FP64) return sizeof(double) * R * (L.nch * 4 + Cfg<MSM_G2G_FP64>::PAD);

@@ -263,7 +263,7 @@ void build_level_kernel(Plan &p, int l, const double as[3]) {
   p.khat[l] = std::move(out);
 }
 
-bool box_dependent(Plan &p, std::string &err) {
+bool box_dependent(Plan &p, std::string &err, bool stencils = true) {
   double as[3] = {col_norm(p.Ai, 0), col_norm(p.Ai, 1), col_norm(p.Ai, 2)};
   // constant-energy coefficients (forcefield0.c:612-643): own images of
   // each charge inside the cutoff, and the neutralising background
@@ -286,8 +286,10 @@ bool box_dependent(Plan &p, std::string &err) {
   double aL = p.aCut[p.L];
   p.coeff2 = kPi * aL * aL / ((4 * p.nu + 2) * p.detA);
   build_omegap(p);
-  build_top_kernel(p);
-  for (int l = p.L - 1; l >= 1; --l) build_level_kernel(p, l, as);
+  if (stencils) {
+    build_top_kernel(p);
+    for (int l = p.L - 1; l >= 1; --l) build_level_kernel(p, l, as);
+  }
   (void)err;
   return true;
 }
@@ -427,7 +429,7 @@ bool build_plan(const BuildRequest &r, Plan &p, std::string &err) {
   return box_dependent(p, err);
 }
 
-bool rebuild_plan(Plan &p, const double edges[3][3], std::string &err) {
+bool rebuild_plan(Plan &p, const double edges[3][3], std::string &err, bool stencils) {
   std::memcpy(p.A, edges, sizeof p.A);
   p.detA = invert3(p.A, p.Ai);
   if (!(p.detA > 0)) { err = "box matrix is singular"; return false; }
@@ -438,7 +440,9 @@ bool rebuild_plan(Plan &p, const double edges[3][3], std::string &err) {
     for (int a = 0; a < 3; ++a) p.omegap[l][a].clear();
     p.khat[l].clear();
   }
-  return box_dependent(p, err);
+  return box_dependent(p, err, stencils);
 }
+
+void top_series(int nu, double b[7]) { top_taylor(nu, b); }
 
 }  // namespace msm

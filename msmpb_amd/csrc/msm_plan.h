@@ -49,8 +49,13 @@ struct BuildRequest {
 bool build_plan(const BuildRequest &req, Plan &plan, std::string &err);
 
 // Recomputes the box-dependent parts (constants and stencils) for a new box
-// with the parameters of an existing plan (FF_rebuild).
-bool rebuild_plan(Plan &plan, const double edges[3][3], std::string &err);
+// with the parameters of an existing plan (FF_rebuild).  With stencils =
+// false the level and top-level stencils (khat) are left empty for the GPU
+// to generate; omega', cL and the constants are still computed.
+bool rebuild_plan(Plan &plan, const double edges[3][3], std::string &err, bool stencils = true);
+
+// Series coefficients b_0..b_6 of t^2 psi(t) of the top-level kernel.
+void top_series(int nu, double b[7]);
 
 // A-priori relative force error estimates (forcefield0.c:521-587).
 double err_estimate(const Plan &plan, const double *q, int nu);
