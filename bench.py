@@ -27,6 +27,10 @@ sys.path.insert(0, ROOT)
 METRIC = "MSM energy+force evals/s at 96k atoms, mixed vs FP64 GPU; rms rel force err"
 PEAK = {"hbm": 8000.0, "fp64": 78.6, "fp32": 157.3, "bf16": 2500.0}  # GB/s, TFLOP/s
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "water96k_hbm_traffic.txt")
+SQ_FILE = os.path.join(ROOT, "profiles", "water96k_kernel_stats.md")
+# VALU issue rate of gfx950: a wave64 VALU instruction occupies one of a
+# CU's four SIMDs for 4 cycles; 256 CUs at 2.4 GHz
+VALU_WAVE_INSTS_PER_S = 256 * 4 * 2.4e9 / 4
 
 
 def parse():
@@ -84,6 +88,43 @@ def traffic_of(kernel):
         if name.strip().startswith(kernel):
             return float(b)
     return None
+
+
+def valu_insts_of(kernel):
+    """SQ_INSTS_VALU per dispatch of `kernel` from the committed rocprofv3
+    SQ-counter summary (None if absent)."""
+    if not os.path.exists(SQ_FILE):
+        return None
+    cols = None
+    for line in open(SQ_FILE):
+        if line.startswith("| kernel | WAVES"):
+            cols = [c.strip() for c in line.strip().strip("|").split("|")]
+            continue
+        if cols and line.startswith("| `" + kernel + "`"):
+            vals = [c.strip() for c in line.strip().strip("|").split("|")]
+            return float(vals[cols.index("INSTS_VALU")])
+    return None
+
+
+def rebuild_ms(sysm, precision="mixed", reps=5):
+    """FF_rebuild (box change) time: stencils and MFMA operands regenerated
+    on the GPU for a box scaled by 1 + 1e-4 k, median of reps calls."""
+    import torch
+    from msmpb_amd import MSM, _lib
+    p = sysm.params
+    m = MSM(sysm.pos, sysm.q, sysm.edges, precision=precision, nu=p["nu"], levels=p["levels"],
+            top_grid=p["top_grid"], cutoff=p["cutoff"])
+    dp = ctypes.POINTER(ctypes.c_double)
+    ts = []
+    for k in range(reps + 1):
+        e = np.ascontiguousarray(sysm.edges * (1.0 + 1e-4 * (k + 1)))
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        _lib.lib().FF_rebuild(m._ff, e.ctypes.data_as(dp), None, ctypes.c_double(0.0))
+        torch.cuda.synchronize()
+        ts.append(time.perf_counter() - t0)
+    m.close()
+    return float(np.median(ts[1:]) * 1e3)
 
 
 def device_free():
@@ -193,9 +234,15 @@ def roofline(stages, m, pairs, nu):
     peak = {"partcl2partcl": PEAK["fp32"] if m.precision == "mixed" else PEAK["fp64"],
             "grid2grid": PEAK["bf16"] if m.precision == "mixed" else PEAK["fp64"]}.get(
                 dom, PEAK["fp64"])
-    return dict(bound="mfma", achieved=ach, peak=peak, unit="TFLOP/s", frac=ach / peak,
-                traffic=traffic_of(kernel), kernel=kernel, stage=dom,
-                work_per_launch=float(amount), launch_ms=stages[dom] * 1e3)
+    out = dict(bound="mfma", achieved=ach, peak=peak, unit="TFLOP/s", frac=ach / peak,
+               traffic=traffic_of(kernel), kernel=kernel, stage=dom,
+               work_per_launch=float(amount), launch_ms=stages[dom] * 1e3)
+    vi = valu_insts_of(kernel)
+    if vi is not None:  # the pair kernel is bound by VALU issue, not by FLOP
+        out["valu"] = {"insts_per_launch": vi,
+                       "issue_frac": vi / VALU_WAVE_INSTS_PER_S / stages[dom],
+                       "source": "SQ_INSTS_VALU, profiles/water96k_kernel_stats.md"}
+    return out
 
 
 def main_water96k(args):
@@ -230,6 +277,7 @@ def main_water96k(args):
         "pairs_within_cutoff": pairs,
         "roofline": roofline(stages, mmx, pairs, mmx.nu),
     }
+    line["rebuild_ms"] = {"mixed": rebuild_ms(sysm, "mixed"), "fp64": rebuild_ms(sysm, "fp64")}
     # the same workload at the other B-spline order (nu = 4)
     if args.nu == 6:
         s4 = inputs.cfg3_water96k(4)
@@ -276,6 +324,7 @@ def main_water1m(args):
         "hbm_footprint_bytes": {"fp64": r["fp64"]["footprint"], "mixed": r["mixed"]["footprint"]},
         "stage_ms_mixed": {k: v * 1e3 for k, v in r["mixed"]["stages"].items()},
         "pairs_within_cutoff": m64.pair_count,
+        "rebuild_ms": {"mixed": rebuild_ms(sysm, "mixed"), "fp64": rebuild_ms(sysm, "fp64")},
     }
     print(json.dumps(line))
 
