@@ -713,13 +713,18 @@ __global__ __launch_bounds__(kPairBS) void k_pair_count(
 // a 16 x 16 output tile accumulated by v_mfma_f64_16x16x4_f64 over the atoms
 // four at a time.  The workgroup lists the pair-search cell columns whose
 // atoms can reach the brick (atoms are sorted by cell, so a column is one or
-// two contiguous index ranges), stages the folded weights of the reaching
-// atoms in LDS 256 candidates at a time, and its four wavefronts split the
-// atoms of each round; their tiles are summed in a fixed order and every grid
-// point is stored once.  No atomics: the result is deterministic.
+// two contiguous index ranges); its four wavefronts split the candidates,
+// queue each atom that reaches the brick and, 32 at a time, fold the queued
+// atoms' weights into LDS records (two lanes per atom) and issue the MFMAs.
+// Their tiles are summed in a fixed order and every grid point is stored
+// once.  No atomics: the result is deterministic.  Records of 32 atoms per
+// wavefront keep a workgroup at about 45 KB of LDS, so it fits beside the
+// pair kernel's wavefronts.
 constexpr int kSX = 16, kSY = 4, kSZ = 4;
 constexpr int kSRanges = 512;       // index ranges per batch of cell columns
 constexpr int kWXs = 17, kWYs = 5;  // padded record strides (doubles)
+constexpr int kSFlush = 32;         // atoms per flush (records per wavefront)
+constexpr int kSQueue = kSFlush + 64;  // queued atoms: < kSFlush + one batch of 64
 
 typedef double sp_f64x4 __attribute__((ext_vector_type(4)));
 
@@ -734,59 +739,89 @@ __device__ __forceinline__ void cell_span(int g0, int B, int half, int M, int nc
   if (len >= nc) lo = 0, len = nc;
 }
 
-// Locates candidate k in the range table (roff[lo] <= k < roff[lo + 1]),
-// advancing from a previous position lo (k only grows) and loads its atom.
-__device__ __forceinline__ double4 spread_fetch(const int *roff, const int *rbeg, int nr, int k,
-                                                int total, int &lo,
-                                                const double4 *__restrict__ spos) {
-  if (k >= total) return make_double4(0.0, 0.0, 0.0, 0.0);
+// Sorted index of candidate k in the range table (roff[lo] <= k < roff[lo + 1]),
+// advancing from a previous position lo (k only grows); -1 past the end.
+__device__ __forceinline__ int spread_index(const int *roff, const int *rbeg, int nr, int k,
+                                            int total, int &lo) {
+  if (k >= total) return -1;
   while (lo + 1 < nr && roff[lo + 1] <= k) ++lo;
-  return spos[rbeg[lo] + (k - roff[lo])];
+  return rbeg[lo] + (k - roff[lo]);
 }
 
-// An atom that reaches the brick, queued until 64 of them fill a wavefront:
-// fractional offsets t and charge, and the brick-relative index of the
-// lowest support point per axis.
-struct SpreadItem {
-  double t[3];
-  double q;
-  short rlo[3];  // grid dims stay below 2^15
-  short pad;
-};
-
-// Folded weights of one atom (the wavefront's 64 queued atoms, one per lane)
-// into its records, then Q += WX^T (WY x WZ) by 16 MFMAs.  `valid` lanes
-// hold an atom; the others leave zero records.
+// Fractional offsets t and the brick-relative index of the lowest support
+// point m - NU/2 + 1 per axis (m lies in [0, M] and org in [0, M), so one
+// wrap suffices, NU >= 4); reach: the support meets the brick.
 template <int NU>
-__device__ __forceinline__ void spread_flush(const DevParams &P, const SpreadItem &it, bool valid,
-                                             int natoms, const int dims[3], double *RX,
-                                             double *RY, double *RZ, int lane,
+__device__ __forceinline__ bool spread_place(const DevParams &P, const double4 &at,
+                                             const int dims[3], const int org[3],
+                                             const int ext[3], double t[3], int rlo[3]) {
+  const double r[3] = {at.x, at.y, at.z};
+  bool reach = true;
+#pragma unroll
+  for (int ax = 0; ax < 3; ++ax) {
+    double sv = r[ax] * P.ibox[ax];
+    sv = sv - floor(sv);
+    const double tt = dims[ax] * sv, m = floor(tt);
+    t[ax] = tt - m;
+    int v = (int)m - NU / 2 + 1 - org[ax];
+    v += v < 0 ? dims[ax] : 0;
+    rlo[ax] = v;
+    reach = reach && (v < ext[ax] || v > dims[ax] - NU);
+  }
+  return reach;
+}
+
+// One axis of an atom's record: weight i of the support sits at brick index
+// rlo + NU - 1 - i (periodic), kept where it falls inside the brick (EXT).
+template <int NU, int EXT>
+__device__ __forceinline__ void spread_fold(const DevParams &P, double t, double sc, int rlo,
+                                            int M, double *rec) {
+  double w[NU];
+  bspline<NU, false>(P.Q, t, w, nullptr);
+  if (M >= NU) {  // the support points are distinct grid points
+#pragma unroll
+    for (int i = 0; i < NU; ++i) {
+      int rr = rlo + (NU - 1 - i);
+      rr -= rr >= M ? M : 0;
+      if (rr < EXT) rec[rr] = sc * w[i];
+    }
+  } else {  // grids narrower than the support: points repeat, accumulate
+    for (int i = 0; i < NU; ++i) {
+      const int rr = (rlo + (NU - 1 - i)) % M;
+      if (rr < EXT) rec[rr] += sc * w[i];
+    }
+  }
+}
+
+// The queued atoms (`at`: atom lane % kSFlush, valid when `ok`): folded
+// weights into the records -- lanes < kSFlush build the x record of their
+// atom, the other half the y (times q) and z records -- then
+// Q += WX^T (WY x WZ) by natoms / 4 MFMAs.
+template <int NU>
+__device__ __forceinline__ void spread_flush(const DevParams &P, const double4 &at, bool ok,
+                                             int natoms, const int dims[3], const int org[3],
+                                             double *RX, double *RY, double *RZ, int lane,
                                              sp_f64x4 &acc) {
   constexpr int ext[3] = {kSX, kSY, kSZ};
-  double *rec[3] = {RX + lane * kWXs, RY + lane * kWYs, RZ + lane * kWYs};
+  {
+    const int a = lane % kSFlush;
+    double *const rx = RX + a * kWXs, *const ry = RY + a * kWYs, *const rz = RZ + a * kWYs;
+    if (lane < kSFlush) {
 #pragma unroll
-  for (int u = 0; u < kSX; ++u) rec[0][u] = 0.0;
+      for (int u = 0; u < kSX; ++u) rx[u] = 0.0;
+    } else {
 #pragma unroll
-  for (int u = 0; u < kSY; ++u) rec[1][u] = 0.0, rec[2][u] = 0.0;
-  if (valid) {
-    double w[NU];
-#pragma unroll
-    for (int ax = 0; ax < 3; ++ax) {
-      bspline<NU, false>(P.Q, it.t[ax], w, nullptr);
-      const double sc = ax == 1 ? it.q : 1.0;
-      const int M = dims[ax];
-      if (M >= NU) {  // the support points are distinct grid points
-#pragma unroll
-        for (int i = 0; i < NU; ++i) {  // weight i sits at point m + NU/2 - i
-          int rr = it.rlo[ax] + (NU - 1 - i);
-          rr -= rr >= M ? M : 0;
-          if (rr < ext[ax]) rec[ax][rr] = sc * w[i];
-        }
-      } else {  // grids narrower than the support: points repeat, accumulate
-        for (int i = 0; i < NU; ++i) {
-          int rr = (it.rlo[ax] + (NU - 1 - i)) % M;
-          if (rr < ext[ax]) rec[ax][rr] += sc * w[i];
-        }
+      for (int u = 0; u < kSY; ++u) ry[u] = 0.0, rz[u] = 0.0;
+    }
+    if (ok) {
+      double t[3];
+      int rlo[3];
+      spread_place<NU>(P, at, dims, org, ext, t, rlo);
+      if (lane < kSFlush) {
+        spread_fold<NU, kSX>(P, t[0], 1.0, rlo[0], dims[0], rx);
+      } else {
+        spread_fold<NU, kSY>(P, t[1], at.w, rlo[1], dims[1], ry);
+        spread_fold<NU, kSZ>(P, t[2], 1.0, rlo[2], dims[2], rz);
       }
     }
   }
@@ -808,10 +843,10 @@ __global__ __launch_bounds__(256) void k_spread_mfma(DevParams P, GridDims g,
                                                      const double4 *__restrict__ spos,
                                                      const int *__restrict__ start,
                                                      double *__restrict__ q1) {
-  // per wavefront: records of 64 atoms and a queue of up to 127 reaching atoms
-  __shared__ double WX[256 * kWXs];
-  __shared__ double WY[256 * kWYs], WZ[256 * kWYs];
-  __shared__ SpreadItem queue[4 * 128];
+  // per wavefront: records of kSFlush atoms and a queue of sorted indices
+  __shared__ double WX[4 * kSFlush * kWXs];
+  __shared__ double WY[4 * kSFlush * kWYs], WZ[4 * kSFlush * kWYs];
+  __shared__ double4 queue[4 * kSQueue];
   __shared__ int rbeg[kSRanges], roff[kSRanges + 1];
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int nbz = (g.z + kSZ - 1) / kSZ, nby = (g.y + kSY - 1) / kSY;
@@ -826,9 +861,9 @@ __global__ __launch_bounds__(256) void k_spread_mfma(DevParams P, GridDims g,
   const int ncol = lenx * leny;
   const int dims[3] = {g.x, g.y, g.z}, org[3] = {gx0, gy0, gz0}, ext[3] = {kSX, kSY, kSZ};
   const unsigned long long below = (1ull << lane) - 1ull;
-  double *const RX = WX + wv * 64 * kWXs, *const RY = WY + wv * 64 * kWYs,
-               *const RZ = WZ + wv * 64 * kWYs;
-  SpreadItem *const Qw = queue + wv * 128;
+  double *const RX = WX + wv * kSFlush * kWXs, *const RY = WY + wv * kSFlush * kWYs,
+               *const RZ = WZ + wv * kSFlush * kWYs;
+  double4 *const Qw = queue + wv * kSQueue;
   int npend = 0;  // queued atoms of this wavefront (uniform)
   sp_f64x4 acc = {0.0, 0.0, 0.0, 0.0};
   for (int c0 = 0; c0 < ncol; c0 += kSRanges / 2) {
@@ -873,49 +908,37 @@ __global__ __launch_bounds__(256) void k_spread_mfma(DevParams P, GridDims g,
     // wavefront wv takes candidates 64 (4 r + wv) + lane, r = 0, 1, ...; the
     // next candidate's atom is fetched while the current one is processed
     int k = wv * 64 + lane, lo = 0;
-    double4 nxt = spread_fetch(roff, rbeg, nr, k, total, lo, spos);
+    int inext = spread_index(roff, rbeg, nr, k, total, lo);
+    double4 nxt = inext >= 0 ? spos[inext] : make_double4(0.0, 0.0, 0.0, 0.0);
     for (; k - lane < total; k += 256) {
       const double4 at = nxt;
-      nxt = spread_fetch(roff, rbeg, nr, k + 256, total, lo, spos);
-      SpreadItem it;
-      bool reach = k < total;
-      const double r[3] = {at.x, at.y, at.z};
-#pragma unroll
-      for (int ax = 0; ax < 3; ++ax) {
-        double sv = r[ax] * P.ibox[ax];
-        sv = sv - floor(sv);
-        const double t = dims[ax] * sv, m = floor(t);
-        it.t[ax] = t - m;
-        // brick-relative index of the lowest support point m - NU/2 + 1:
-        // m lies in [0, M] and org in [0, M), so one wrap suffices (NU >= 4)
-        int v = (int)m - NU / 2 + 1 - org[ax];
-        v += v < 0 ? dims[ax] : 0;
-        it.rlo[ax] = (short)v;
-        reach = reach && (v < ext[ax] || v > dims[ax] - NU);
-      }
-      it.q = at.w;
+      const int icur = inext;
+      inext = spread_index(roff, rbeg, nr, k + 256, total, lo);
+      nxt = inext >= 0 ? spos[inext] : make_double4(0.0, 0.0, 0.0, 0.0);
+      double t[3];
+      int rlo[3];
+      const bool reach = icur >= 0 && spread_place<NU>(P, at, dims, org, ext, t, rlo);
       const unsigned long long bm = __ballot(reach);
-      const int nrel = __popcll(bm);
-      if (reach) Qw[npend + __popcll(bm & below)] = it;
-      npend += nrel;
-      if (npend >= 64) {
+      if (reach) Qw[npend + __popcll(bm & below)] = at;
+      npend += __popcll(bm);
+      while (npend >= kSFlush) {
         __builtin_amdgcn_wave_barrier();
-        const SpreadItem mine = Qw[lane];
-        spread_flush<NU>(P, mine, true, 64, dims, RX, RY, RZ, lane, acc);
-        // move the leftover queue entries [64, npend) to the front
-        if (64 + lane < npend) {
-          const SpreadItem rest = Qw[64 + lane];
-          Qw[lane] = rest;
-        }
-        npend -= 64;
+        const double4 mine = Qw[lane % kSFlush];
+        const double4 rest = Qw[min(kSFlush + lane, kSQueue - 1)];
+        __builtin_amdgcn_wave_barrier();
+        spread_flush<NU>(P, mine, true, kSFlush, dims, org, RX, RY, RZ, lane, acc);
+        // move the leftover queue entries [kSFlush, npend) to the front
+        if (kSFlush + lane < npend) Qw[lane] = rest;
+        npend -= kSFlush;
         __builtin_amdgcn_wave_barrier();
       }
     }
   }
-  if (npend > 0) {  // the last, partial wavefront of atoms
+  if (npend > 0) {  // the last, partial group of atoms
     __builtin_amdgcn_wave_barrier();
-    const SpreadItem mine = Qw[lane];
-    spread_flush<NU>(P, mine, lane < npend, (npend + 3) & ~3, dims, RX, RY, RZ, lane, acc);
+    const double4 mine = Qw[lane % kSFlush];
+    spread_flush<NU>(P, mine, lane % kSFlush < npend, (npend + 3) & ~3, dims, org, RX, RY, RZ,
+                     lane, acc);
   }
   // ---- sum the four wavefronts' tiles in a fixed order and store
   __syncthreads();
