@@ -72,11 +72,19 @@ int msm_eval_device(FF *ff, const double *d_pos, const double *d_q, double *d_en
                     double *d_force, double *d_phi, void *stream);
 
 /* Split evaluation for automatic differentiation.
- * msm_forward_device computes the energy only; keep != 0 keeps the
- * level-1 potential grid and the atom ordering for msm_adjoint_device.
- * msm_adjoint_device writes d_force = -dE/dr and d_phi = dE/dq; when the
- * forward state was not kept (or recompute != 0) it recomputes the grid
- * levels from d_pos and d_q first. */
+ * msm_forward_device computes the energy and keeps state for
+ * msm_adjoint_device: keep = MSM_KEEP_ALL keeps the level-1 potential grid,
+ * the atom ordering and the short-range parts of -dE/dr and dE/dq (the pair
+ * kernel writes them during the forward at no extra pass);
+ * MSM_KEEP_PAIRS keeps the short-range parts only; MSM_KEEP_NONE nothing.
+ * msm_adjoint_device writes d_force = -dE/dr and d_phi = dE/dq: the
+ * long-range parts by interpolation from the level-1 grid, recomputed from
+ * d_pos and d_q when it was not kept (or recompute != 0), plus the kept
+ * short-range parts or, without them, a pair pass.  The kept state belongs
+ * to the latest forward on this FF and to its d_pos and d_q pointers. */
+#define MSM_KEEP_NONE 0
+#define MSM_KEEP_ALL 1
+#define MSM_KEEP_PAIRS 2
 int msm_forward_device(FF *ff, const double *d_pos, const double *d_q, double *d_energy,
                        int keep, void *stream);
 int msm_adjoint_device(FF *ff, const double *d_pos, const double *d_q, double *d_force,

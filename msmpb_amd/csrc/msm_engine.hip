@@ -119,7 +119,8 @@ struct Engine {
   Timing T;
   int timed_evals = 0;
   // forward state kept for a following adjoint
-  bool kept = false;
+  bool kept = false;        // level-1 grid and atom ordering of the last forward
+  bool kept_pairs = false;  // short-range F, phi of the last forward in d_Fp, d_phip
   const double *kept_pos = nullptr, *kept_q = nullptr;
   double last_terms[3] = {0, 0, 0};
 
@@ -295,7 +296,6 @@ void fill_params(const msm::Plan &p, int N, msm::DevParams &P) {
   P.fa02 = (float)P.a02;
   P.fia02 = (float)P.inv_a02;
   P.fcut2 = (float)(P.a02 * 1.0001);
-  P.fr2min = 1e-4f;
 }
 
 // Uploads the box-dependent device data (stencils, twiddles, parameters).
@@ -419,7 +419,7 @@ int rebuild_box(Engine &E) {
   }
   HIP_TRY(hipStreamSynchronize(s));
   HIP_TRY(hipGetLastError());
-  E.kept = false;
+  E.kept = E.kept_pairs = false;
   return MSM_OK;
 }
 
@@ -713,7 +713,7 @@ int eval_chain(FFX *x, const double *pos, const double *q, double *out, double *
   T.close(t, "energy", s);
   HIP_TRY(hipGetLastError());
   if (T.on) ++E.timed_evals;
-  E.kept = false;
+  E.kept = E.kept_pairs = false;
   return MSM_OK;
 }
 
@@ -1199,15 +1199,21 @@ int msm_forward_device(FF *ff, const double *d_pos, const double *d_q, double *d
   FFX *x = X(ff);
   Engine &E = *x->eng;
   hipStream_t s = (hipStream_t)stream;  // NULL: the default stream, as in HIP
+  if (keep != MSM_KEEP_NONE && keep != MSM_KEEP_ALL && keep != MSM_KEEP_PAIRS)
+    return fail(MSM_ERR_ARG, "msm_forward_device: keep must be MSM_KEEP_NONE, _ALL or _PAIRS");
   Timing T;
+  // with state kept for the adjoint the pair kernel also writes its
+  // per-atom sums (the short-range parts of -dE/dr and dE/dq)
+  const bool kp = keep != MSM_KEEP_NONE;
   stage_prepare(E, d_pos, d_q, s);
-  TRY(fork_pairs(E, x->precision, nullptr, nullptr, s, T));
+  TRY(fork_pairs(E, x->precision, kp ? E.d_Fp : nullptr, kp ? E.d_phip : nullptr, s, T));
   TRY(stage_grids(E, x->precision, x->layout, T, s));
   stage_grid_energy(E, x->layout, s);
   HIP_TRY(hipStreamWaitEvent(s, E.ev_join, 0));
   stage_energy(E, nullptr, nullptr, d_energy, s);
   HIP_TRY(hipGetLastError());
-  E.kept = keep != 0;
+  E.kept = keep == MSM_KEEP_ALL;
+  E.kept_pairs = kp;
   E.kept_pos = d_pos;
   E.kept_q = d_q;
   return MSM_OK;
@@ -1221,16 +1227,30 @@ int msm_adjoint_device(FF *ff, const double *d_pos, const double *d_q, double *d
   Engine &E = *x->eng;
   hipStream_t s = (hipStream_t)stream;  // NULL: the default stream, as in HIP
   Timing T;
-  if (recompute || !E.kept || E.kept_pos != d_pos || E.kept_q != d_q) {
+  const bool same = E.kept_pos == d_pos && E.kept_q == d_q;
+  if (recompute || !E.kept || !same) {
     // grid levels are not stored: rebuild the ordering and e^1 from scratch
     stage_prepare(E, d_pos, d_q, s);
     msm::launch_fill_int(E.d_flags, 1, 0, s);  // no energy reduction follows to clear it
     TRY(stage_grids(E, x->precision, x->layout, T, s));
+    E.kept = false;
   }
-  stage_pairs(E, x->precision, d_force, d_phi, s);  // the pair kernel writes, gather adds
-  if (d_force || d_phi)
-    msm::launch_gather(E.P, E.gd[1], E.d_spos, E.d_sidx, E.d_start, E.d_el[1], E.d_scal, d_force, d_phi,
-                       true, s);
+  if (E.kept_pairs && same) {
+    // the forward's pair kernel left the short-range parts (indexed by atom,
+    // independent of the ordering): interpolate the long-range parts, then
+    // add them (the reduction rides along and rewrites E.d_out only)
+    if (d_force || d_phi) {
+      msm::launch_gather(E.P, E.gd[1], E.d_spos, E.d_sidx, E.d_start, E.d_el[1], E.d_scal,
+                         d_force, d_phi, false, s);
+      msm::launch_combine(E.P, d_force ? E.d_Fp : nullptr, d_phi ? E.d_phip : nullptr, d_force,
+                          d_phi, E.d_ep2, finalize_args(E, E.d_out), s);
+    }
+  } else {
+    stage_pairs(E, x->precision, d_force, d_phi, s);  // the pair kernel writes, gather adds
+    if (d_force || d_phi)
+      msm::launch_gather(E.P, E.gd[1], E.d_spos, E.d_sidx, E.d_start, E.d_el[1], E.d_scal,
+                         d_force, d_phi, true, s);
+  }
   HIP_TRY(hipGetLastError());
   return MSM_OK;
 }

@@ -25,7 +25,6 @@ struct DevParams {
   double tau_s[kMaxNu], dtau_s[kMaxNu];  // tau_k / a0 and (k + 1) tau_{k+1} 2 / a0^3
   float ftau[kMaxNu], fdtau[kMaxNu];     // the same in FP32
   float fa02, fia02, fcut2;              // fcut2: a0^2 enlarged by 1e-4
-  float fr2min;                          // FP32 pair term: lower clamp of r^2 (1e-4 A^2)
   int nu;
   int N;
   double tau[kMaxNu];

@@ -331,20 +331,18 @@ __device__ __forceinline__ double rsqrt_f64(double r2) {
 // with d = r_j - r_i; the pre-scaled coefficients fold 1/a0 into tau and
 // 2/a0^3 into tau'.
 //
-// FP32: r^2 is clamped into [r2min, a0^2] by one med3 and no pair is masked.
-// g and g' both vanish at r = a0 (the smoothed kernel joins 1/r with a
-// continuous first derivative), so a neighbour beyond the cutoff adds only
-// the rounding residue of g(a0) and g'(a0), about 1e-7 of 1/a0.  The self
-// pair (d = 0 exactly) adds zero force and q_i * pair_self_term() to sa,
-// which the caller takes off.  Distinct atoms closer than sqrt(r2min) =
-// 0.01 A are evaluated at 0.01 A.
+// FP32: r^2 is clamped to at most a0^2 (one med3) instead of masking the
+// neighbours beyond the cutoff: g and g' both vanish at r = a0 (the smoothed
+// kernel joins 1/r with a continuous first derivative), so such a neighbour
+// adds only the rounding residue of g(a0) and g'(a0), about 1e-7 of 1/a0.
+// The self pair (d = 0 exactly) is masked through its charge.
 template <int NU>
 __device__ __forceinline__ void pair_term(const DevParams &P, float4 aj, float ix, float iy,
                                           float iz, float &sa, float &sx, float &sy, float &sz) {
   const float dx = aj.x - ix, dy = aj.y - iy, dz = aj.z - iz;
   const float r2 = dx * dx + dy * dy + dz * dz;
-  const float r2s = __builtin_amdgcn_fmed3f(r2, P.fr2min, P.fa02);
-  const float qj = aj.w;
+  const float r2s = __builtin_amdgcn_fmed3f(r2, 1e-20f, P.fa02);
+  const float qj = r2 > 0.f ? aj.w : 0.f;
   const float sv = r2s * P.fia02 - 1.f;
   float ts = P.ftau[NU - 1], t1 = P.fdtau[NU - 2];
 #pragma unroll
@@ -355,15 +353,6 @@ __device__ __forceinline__ void pair_term(const DevParams &P, float4 aj, float i
   sa += qj * (rinv - ts);
   const float qc = qj * (-rinv * rinv * rinv - t1);
   sx += qc * dx, sy += qc * dy, sz += qc * dz;
-}
-
-// g at the clamped self distance, as the FP32 pair term computes it: the
-// self pair's contribution to sa is q_i times this value.
-template <int NU>
-__device__ __forceinline__ double pair_self_term(const DevParams &P) {
-  float sa = 0.f, sx = 0.f, sy = 0.f, sz = 0.f;
-  pair_term<NU>(P, make_float4(0.f, 0.f, 0.f, 1.f), 0.f, 0.f, 0.f, sa, sx, sy, sz);
-  return (double)sa;
 }
 
 // Cell of workgroup b.  Consecutive workgroups are dispatched to the eight
@@ -508,8 +497,6 @@ __global__ __launch_bounds__(64) void k_pair_cell(DevParams P, const double4 *__
   const float mrg = 1e-3f * (float)fmin(P.cw[0], fmin(P.cw[1], P.cw[2]));
   const unsigned long long lanes_below = (1ull << lane) - 1ull;
   const R far = (R)1e18;  // padding atoms sit far from everything
-  // FP32: the self pair's g (see pair_term); FP64 masks the self pair instead
-  const double gself = MIXED ? pair_self_term<NU>(P) : 0.0;
   double esum = 0.0;
   for (int ip = 0; ip < n; ip += kPassAtoms) {
     const int ni = min(kPassAtoms, n - ip);
@@ -655,7 +642,7 @@ __global__ __launch_bounds__(64) void k_pair_cell(DevParams P, const double4 *__
       const double qi = spos[si].w;
       const int oi = sidx[si];
       if (d == 0) {
-        const double pv = tot - (double)(float)qi * gself - qi * P.self_img;
+        const double pv = tot - qi * P.self_img;
         if (phi) phi[oi] = pv;
         esum += 0.5 * qi * pv;
       } else if (F) {

@@ -186,12 +186,14 @@ class MSM:
         return e, F, phi
 
     def forward_device(self, pos, q, keep=True):
+        """keep: True/1 (grid and short-range parts), 2 (short-range parts
+        only: the adjoint recomputes the grid levels), False/0 (nothing)."""
         p = self._dev(pos, (self.N, 3))
         qq = self._dev(q, (self.N,))
         e = torch.empty(4, dtype=torch.float64, device=pos.device)
         s = ctypes.c_void_p(torch.cuda.current_stream(pos.device).cuda_stream)
         _check(_lib.lib().msm_forward_device(self._ff, p, qq, ctypes.c_void_p(e.data_ptr()),
-                                             1 if keep else 0, s), "msm_forward_device")
+                                             int(keep), s), "msm_forward_device")
         return e
 
     def adjoint_device(self, pos, q, recompute=False):
@@ -212,7 +214,9 @@ class _MSMEnergy(torch.autograd.Function):
 
     @staticmethod
     def forward(ctx, pos, q, msm, recompute):
-        e = msm.forward_device(pos, q, keep=not recompute)
+        # the short-range parts are per-atom (kept even when the grid levels
+        # are recomputed in backward)
+        e = msm.forward_device(pos, q, keep=2 if recompute else 1)
         msm._generation = getattr(msm, "_generation", 0) + 1
         ctx.generation = msm._generation
         ctx.msm = msm

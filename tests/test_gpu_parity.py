@@ -142,6 +142,37 @@ def test_autograd_adjoint(recompute, oracle_mod, torch_cuda):
     assert np.max(np.abs(q.grad.cpu().numpy() - phio)) / np.sqrt(np.mean(phio ** 2)) < TOL
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_forward_adjoint_keep_modes(precision, oracle_mod, torch_cuda):
+    """msm_forward_device / msm_adjoint_device for every keep mode (all state,
+    short-range parts only, nothing) and recompute flag: the adjoint's F and
+    phi against the oracle (FP64 to 1e-10, mixed to the mixed bar), and a
+    full evaluation in between invalidates the kept state."""
+    torch = torch_cuda
+    from msmpb_amd.inputs import cfg2_random3000
+    s = cfg2_random3000()
+    m, o = _pair(s, oracle_mod, precision=precision)
+    Eo, Fo, phio, _ = o.energy(s.pos)
+    pos = torch.tensor(s.pos, device="cuda")
+    q = torch.tensor(s.q, device="cuda")
+    tol = TOL if precision == "fp64" else 1e-5
+    for keep, recompute, between in ((1, False, False), (1, True, False), (2, True, False),
+                                     (2, False, False), (0, False, False), (1, False, True)):
+        print(f"[parity] keep={keep} recompute={recompute} eval_between={between} {precision}",
+              flush=True)
+        e = m.forward_device(pos, q, keep=keep)
+        if between:  # overwrites the kept pair parts and grid
+            m.eval_device(pos * 1.0, q * 1.0)
+        F, phi = m.adjoint_device(pos, q, recompute=recompute)
+        assert rel_err(e[0].item(), Eo) < tol
+        F, phi = F.cpu().numpy(), phi.cpu().numpy()
+        if precision == "fp64":
+            assert force_err(F, Fo) < tol, (keep, recompute)
+        else:
+            assert rms_rel_force(F, Fo) < tol, (keep, recompute)
+        assert np.max(np.abs(phi - phio)) / np.sqrt(np.mean(phio ** 2)) < tol, (keep, recompute)
+
+
 def test_adjoint_central_differences(oracle_mod, torch_cuda):
     """dE/dr and dE/dq from the adjoint chain against central differences of
     the forward energy, FP64 (3000 charges in 30 x 35 x 40 A; steps 1e-5 A
