@@ -298,7 +298,7 @@ template <>
 struct PairPrec<true> {
   using T = float4;
   using R = float;
-  static constexpr int CAP = 512;  // staged neighbours per interaction run
+  static constexpr int CAP = 384;  // staged neighbours per interaction run (6 KB of LDS)
 };
 template <>
 struct PairPrec<false> {
