@@ -7,8 +7,9 @@
  * index layout.  All pointers named d_* are device pointers (HBM) of the
  * current HIP device; `stream` is a hipStream_t on which the call is
  * ordered (NULL = the default stream, as everywhere in HIP; PyTorch's
- * default stream is that stream).  Internally the short-range pairs run on
- * a second library stream forked from and joined back into `stream`.
+ * default stream is that stream).  Internally the short-range pairs and the
+ * level-1 stencil pass run on two library streams forked from and joined
+ * back into `stream`.
  * Every int-returning call returns 0 on success and a negative code on
  * failure, with a message available from FF_last_error().
  */

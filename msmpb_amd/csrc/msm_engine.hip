@@ -675,9 +675,9 @@ int check_built(FF *ff) {
   return MSM_OK;
 }
 
-// Pairs on the second stream, concurrently with the grid chain; the join
-// precedes the energy reduction and the interpolation, which add to the
-// pair results.
+// Pairs on the pair stream, concurrently with the grid chain; the caller
+// joins before the reduction that adds the pair results (k_combine or
+// k_finalize).
 int fork_pairs(Engine &E, int precision, double *F, double *phi, hipStream_t s, Timing &T) {
   HIP_TRY(hipEventRecord(E.ev_fork, s));
   HIP_TRY(hipStreamWaitEvent(E.stream2, E.ev_fork, 0));
