@@ -15,16 +15,28 @@ import ctypes
 import numpy as np
 import pytest
 
-from helpers import rel_err, force_err, rms_rel_force
+from helpers import load_dataset, rel_err, force_err, rms_rel_force
 from msmpb_amd import inputs
 
 pytestmark = pytest.mark.gpu
 
 TOL = 1e-10
+
+
+def _spce300(nu, L, M, a0):
+    def make():
+        edges, q, pos, mass = load_dataset("spceN300")
+        return inputs.System("spceN300", pos, q, edges, mass,
+                             dict(nu=nu, levels=L, top_grid=M, cutoff=a0))
+    return make
+
+
 CASES = {
     "water300": (inputs.cfg0_water300, 1.003),
     "crystal8192": (inputs.cfg1_crystal8192, 0.998),
     "random3000": (inputs.cfg2_random3000, 1.002),
+    "spce300_L1_nu4": (_spce300(4, 1, 24, 9.0), 1.001),  # top level only
+    "spce300_L2_nu8": (_spce300(8, 2, 8, 11.0), 0.999),
 }
 
 
