@@ -17,6 +17,7 @@
 #define MSMPB_HIP_EXT_H
 
 #include "forcefield.h"
+#include "msm_bridge.h"
 
 #ifdef __cplusplus
 extern "C" {
@@ -54,6 +55,16 @@ const char *FF_last_error(void);
 int FF_set_precision(FF *ff, int precision);
 int FF_get_precision(FF *ff);
 int FF_set_layout_compat(FF *ff, int layout);
+/* Summation method, set before FF_build.  MSM_METHOD_PME (msm_bridge.h)
+ * evaluates the reference's smooth particle-mesh Ewald comparator
+ * (code/pme0.c, code/pme1.c) with the same kernels: one grid of topGridDim
+ * points (used as given; default from the reference's spacing table), the
+ * Ewald pair term erfc(beta r)/r inside the cutoff with erfc(beta cutoff)
+ * = tolDir * cutoff (tolDir <= 0: 1e-5) and a Gaussian Fourier kernel.
+ * Defaults: order 6, cutoff 8 A; maxLevel is ignored (one level). */
+int FF_set_method(FF *ff, int method, double tolDir);
+/* Ewald splitting parameter beta of a built PME FF (0 for MSM). */
+double FF_get_beta(FF *ff);
 /* Record per-stage GPU time into the FF time_* fields on FF_energy. */
 int FF_set_stage_timing(FF *ff, int enable);
 

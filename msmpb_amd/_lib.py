@@ -17,6 +17,7 @@ _ip = ctypes.POINTER(ctypes.c_int)
 
 MSM_PREC_FP64, MSM_PREC_MIXED = 0, 1
 MSM_LAYOUT_FIXED, MSM_LAYOUT_ASIS = 0, 1
+MSM_METHOD_MSM, MSM_METHOD_PME = 0, 1
 
 
 class FFStruct(ctypes.Structure):
@@ -81,9 +82,92 @@ EXPORTS = [
     "FF_set_stage_timing", "FF_get_energy_terms", "FF_energy_potential", "msm_eval_device",
     "msm_forward_device", "msm_adjoint_device", "msm_grid_dims", "msm_stencil_dims",
     "msm_copy_grid", "msm_copy_stencil", "msm_count_pairs", "msm_stage_times", "msm_stage_name",
+    "FF_set_method", "FF_get_beta",
+    # include/msm_bridge.h
+    "msmb_create", "msmb_build", "msmb_rebuild", "msmb_energy", "msmb_configure", "msmb_info",
+    "msmb_tables", "msmb_times", "msmb_delete",
 ]
 
+# every symbol include/pme.h and include/pme_hip.h declare (libpme_hip.so)
+PME_EXPORTS = [
+    "FF_new", "FF_set_cutoff", "FF_get_cutoff", "FF_set_orderAcc", "FF_set_topGridDim",
+    "FF_set_tolDir", "FF_build", "FF_get_orderAcc", "FF_get_topGridDim", "FF_get_tolDir",
+    "FF_get_errEst", "FF_rebuild", "FF_energy", "FF_delete", "msm4g_tictocmanager", "msm4g_tic",
+    "msm4g_toc", "prod", "prodT", "pme_set_precision",
+]
+
+class PMEStruct(ctypes.Structure):
+    """FF of include/pme.h (the reference's code/pme.h:15-44)."""
+    _fields_ = [
+        ("N", ctypes.c_int),
+        ("q", _dp),
+        ("A", (ctypes.c_double * 3) * 3),
+        ("Ai", (ctypes.c_double * 3) * 3),
+        ("detA", ctypes.c_double),
+        ("cutoff", ctypes.c_double),
+        ("orderAcc", ctypes.c_int),
+        ("topGridDim", ctypes.c_int * 3),
+        ("nlist", _ip),
+        ("nlist_len", ctypes.c_int),
+        ("nlist_interaction_count", ctypes.c_int),
+        ("tolDir", ctypes.c_double),
+        ("fftw_in", ctypes.c_void_p),
+        ("forward", ctypes.c_void_p),
+        ("backward", ctypes.c_void_p),
+        ("beta", ctypes.c_double),
+        ("Q", _dp),
+        ("cL", _dp * 3),
+        ("khat", _dp),
+        ("coeff1", ctypes.c_double),
+        ("coeff2", ctypes.c_double),
+        ("time_partcl2partcl", ctypes.c_double),
+        ("time_nlist", ctypes.c_double),
+        ("time_grid2grid", ctypes.c_double),
+        ("time_stencil", ctypes.c_double),
+        ("time_anterpolation", ctypes.c_double),
+        ("time_interpolation", ctypes.c_double),
+    ]
+
+
+PMEP = ctypes.POINTER(PMEStruct)
+PME_LIB_PATH = os.path.join(os.path.dirname(LIB_PATH), "libpme_hip.so")
+
 _lib = None
+_pme = None
+
+
+def pme_lib():
+    """libpme_hip.so (the reference's pme.h API); fails loudly when absent."""
+    global _pme
+    if _pme is not None:
+        return _pme
+    lib()  # torch's HIP runtime first, then the MSM library it depends on
+    if not os.path.exists(PME_LIB_PATH):
+        raise RuntimeError(f"{PME_LIB_PATH} is missing: build it with `python -m msmpb_amd.build`")
+    P = ctypes.CDLL(PME_LIB_PATH)
+    M3 = (ctypes.c_double * 3) * 3
+    P.FF_new.restype = PMEP
+    P.FF_new.argtypes = [ctypes.c_int, _dp, ctypes.POINTER(M3)]
+    P.FF_set_cutoff.argtypes = [PMEP, ctypes.c_double]
+    P.FF_set_orderAcc.argtypes = [PMEP, ctypes.c_int]
+    P.FF_set_topGridDim.argtypes = [PMEP, _ip]
+    P.FF_set_tolDir.argtypes = [PMEP, ctypes.c_double]
+    P.FF_build.argtypes = [PMEP, _dp]
+    P.FF_rebuild.argtypes = [PMEP, ctypes.POINTER(M3), _dp]
+    P.FF_energy.restype = ctypes.c_double
+    P.FF_energy.argtypes = [PMEP, _dp, _dp, _dp]
+    P.FF_get_errEst.restype = ctypes.c_double
+    P.FF_get_errEst.argtypes = [PMEP, ctypes.c_int, _dp]
+    P.FF_get_cutoff.restype = ctypes.c_double
+    P.FF_get_cutoff.argtypes = [PMEP]
+    P.FF_get_tolDir.restype = ctypes.c_double
+    P.FF_get_tolDir.argtypes = [PMEP]
+    P.FF_get_orderAcc.argtypes = [PMEP]
+    P.FF_get_topGridDim.argtypes = [PMEP, _ip]
+    P.FF_delete.argtypes = [PMEP]
+    P.pme_set_precision.argtypes = [PMEP, ctypes.c_int]
+    _pme = P
+    return P
 
 
 def lib():
@@ -154,6 +238,10 @@ def lib():
     L.msm4g_toc.restype = ctypes.c_double
     L.invert.restype = ctypes.c_double
     L.invert.argtypes = [_dp]
+    L.FF_set_method.restype = ctypes.c_int
+    L.FF_set_method.argtypes = [FFP, ctypes.c_int, ctypes.c_double]
+    L.FF_get_beta.restype = ctypes.c_double
+    L.FF_get_beta.argtypes = [FFP]
     _lib = L
     return L
 

@@ -2,7 +2,9 @@
 
 Outputs (in-tree, git-ignored):
   msmpb_amd/libmsmpb_hip.so  -- HIP kernels + forcefield.h C ABI (hipcc, gfx950)
+  msmpb_amd/libpme_hip.so    -- the PME comparator's pme.h C ABI on those kernels
   msmpb_amd/bin/msmpb        -- CLI with the reference msmpb's argv and JSON
+  msmpb_amd/bin/pme          -- CLI with the reference pme's argv and JSON
 """
 import os
 import subprocess
@@ -13,6 +15,8 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libmsmpb_hip.so")
 CLI = os.path.join(HERE, "bin", "msmpb")
+PME_LIB = os.path.join(HERE, "libpme_hip.so")
+PME_CLI = os.path.join(HERE, "bin", "pme")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("MSMPB_OFFLOAD_ARCH", "gfx950")
@@ -25,6 +29,9 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-value", "-Wno-unused-result",
     "-I", os.path.join(ROOT, "include"),
+    # calls between the library's own entry points stay inside it, also when
+    # libpme_hip.so (same FF_* names, other signatures) is loaded first
+    "-Wl,-Bsymbolic-functions",
 ]
 
 
@@ -55,12 +62,27 @@ def build_cli(verbose=False):
     return CLI
 
 
+def build_pme(verbose=False):
+    inc = ["-I", os.path.join(ROOT, "include")]
+    tmp = PME_LIB + ".tmp"
+    _run(["gcc", "-O2", "-std=gnu11", "-Wall", "-fPIC", "-shared", *inc, "-o", tmp,
+          os.path.join(CSRC, "pme_capi.c"), "-L", HERE, "-lmsmpb_hip", "-Wl,-rpath,$ORIGIN", "-Wl,-Bsymbolic-functions",
+          "-lm"], verbose)
+    os.replace(tmp, PME_LIB)
+    _run(["gcc", "-O2", "-std=c11", "-Wall", *inc, "-o", PME_CLI, os.path.join(CSRC, "pme_cli.c"),
+          "-L", HERE, "-lpme_hip", "-lmsmpb_hip", "-Wl,-rpath,$ORIGIN/..", "-lm"], verbose)
+    return PME_LIB
+
+
 def build(verbose=False):
     build_library(verbose)
     build_cli(verbose)
+    build_pme(verbose)
 
 
 if __name__ == "__main__":
     build(verbose="-v" in sys.argv)
     print(LIB)
     print(CLI)
+    print(PME_LIB)
+    print(PME_CLI)

@@ -18,6 +18,7 @@
 #include <string>
 #include <vector>
 
+#include "../../include/msm_bridge.h"
 #include "../../include/msmpb_hip.h"
 // forcefield.h defines GNU max/min macros for C callers; not wanted here
 #undef max
@@ -174,6 +175,8 @@ struct FFX {
   int precision = MSM_PREC_FP64;
   int layout = MSM_LAYOUT_FIXED;
   int timing = 0;
+  int method = MSM_METHOD_MSM;
+  double tolDir = -1;  // PME direct-sum tolerance (<= 0: the default)
   bool built = false;
   // host copies behind the public FF pointers
   std::vector<double> h_tau, h_Q, h_J, h_aCut, h_cL[3];
@@ -249,6 +252,10 @@ void pair_geometry(const msm::Plan &p, msm::DevParams &P, std::vector<int4> &col
         const double x = kx * p.A[0][0], y = ky * p.A[1][1], z = kz * p.A[2][2];
         const double r2 = x * x + y * y + z * z;
         if (r2 >= a0 * a0) continue;
+        if (p.method == msm::kPme) {
+          C += std::erfc(p.beta * std::sqrt(r2)) / std::sqrt(r2);
+          continue;
+        }
         const double sv = r2 / (a0 * a0) - 1.0;
         double ts = p.tau[p.nu - 1];
         for (int k = p.nu - 2; k >= 0; --k) ts = p.tau[k] + sv * ts;
@@ -296,6 +303,13 @@ void fill_params(const msm::Plan &p, int N, msm::DevParams &P) {
   P.fa02 = (float)P.a02;
   P.fia02 = (float)P.inv_a02;
   P.fcut2 = (float)(P.a02 * 1.0001);
+  P.ewald = p.method == msm::kPme;
+  P.beta = p.beta;
+  P.beta2 = p.beta * p.beta;
+  P.ewc = 2.0 * p.beta / std::sqrt(M_PI);
+  P.fbeta = (float)P.beta;
+  P.fbeta2 = (float)P.beta2;
+  P.fewc = (float)P.ewc;
 }
 
 // Uploads the box-dependent device data (stencils, twiddles, parameters).
@@ -406,7 +420,11 @@ int rebuild_box(Engine &E) {
   for (int l = 1; l < p.L; ++l)
     msm::launch_stencil_level(b, l, E.d_stab + woff[l][0], E.d_stab + woff[l][1],
                               E.d_stab + woff[l][2], E.d_sscr, E.d_sscr + smax, E.d_khat[l], s);
-  msm::launch_stencil_top(b, E.d_stab + cloff, E.d_khat[p.L], s);
+  b.beta = p.beta;
+  if (p.method == msm::kPme)
+    msm::launch_stencil_top_pme(b, E.d_stab + cloff, E.d_khat[1], s);
+  else
+    msm::launch_stencil_top(b, E.d_stab + cloff, E.d_khat[p.L], s);
   for (int pr = 0; pr < 2; ++pr)
     for (int l = 1; l < p.L; ++l)
       msm::launch_g2g_bfrag_fill(pr, E.g2g[pr].lv[l - 1], E.d_khat[l], E.d_bfrag[pr][l], s);
@@ -845,6 +863,8 @@ int build_impl(FF *ff, int N, double (*position)[3], double edges[3][3]) {
   for (int a = 0; a < 3; ++a) r.Mtop[a] = ff->topGridDim[0] > 0 ? ff->topGridDim[a] : -1;
   r.a0 = ff->cutoff;
   r.estErr = ff->estErr;
+  r.method = x->method == MSM_METHOD_PME ? msm::kPme : msm::kMsm;
+  r.tolDir = x->tolDir;
   msm::Plan plan;
   std::string err;
   if (!ff->FFT) return fail(MSM_ERR_ARG, "FF_build: only the FFT top level is supported");
@@ -1129,6 +1149,20 @@ int FF_set_precision(FF *ff, int precision) {
 }
 int FF_get_precision(FF *ff) { return ff ? X(ff)->precision : -1; }
 
+int FF_set_method(FF *ff, int method, double tolDir) {
+  if (!ff) return fail(MSM_ERR_ARG, "FF is NULL");
+  if (method != MSM_METHOD_MSM && method != MSM_METHOD_PME)
+    return fail(MSM_ERR_ARG, "method must be MSM_METHOD_MSM or MSM_METHOD_PME");
+  X(ff)->method = method;
+  X(ff)->tolDir = tolDir;
+  return MSM_OK;
+}
+
+double FF_get_beta(FF *ff) {
+  if (!ff || !X(ff)->eng) return 0.0;
+  return X(ff)->eng->plan.beta;
+}
+
 int FF_set_layout_compat(FF *ff, int layout) {
   if (!ff) return fail(MSM_ERR_ARG, "FF is NULL");
   if (layout != MSM_LAYOUT_FIXED && layout != MSM_LAYOUT_ASIS)
@@ -1298,5 +1332,90 @@ int msm_count_pairs(FF *ff, double (*position)[3], long long *count) {
   TRY(stage_host_inputs(x, position));
   return count_pairs_dev(x, x->eng->d_pos, x->eng->d_q, count);
 }
+
+
+// ============================================================ msm_bridge.h
+void *msmb_create(int N, double *q, int method, int orderAcc, int maxLevel,
+                  const int *topGridDim, double cutoff, double tolDir) {
+  FF *ff = FF_new(N, q);
+  if (orderAcc > 0) ff->orderAcc = orderAcc;
+  if (maxLevel > 0) ff->maxLevel = maxLevel;
+  if (topGridDim && topGridDim[0] > 0)
+    for (int a = 0; a < 3; ++a) ff->topGridDim[a] = topGridDim[a];
+  if (cutoff > 0) ff->cutoff = cutoff;
+  if (FF_set_method(ff, method, tolDir) != MSM_OK) {
+    FF_delete(ff);
+    return nullptr;
+  }
+  return ff;
+}
+
+int msmb_build(void *h, double (*position)[3], double edges[3][3]) {
+  if (!h) return fail(MSM_ERR_ARG, "handle is NULL");
+  FF *ff = (FF *)h;
+  return build_impl(ff, ff->N, position, edges);
+}
+
+int msmb_rebuild(void *h, double edges[3][3], double (*position)[3]) {
+  if (!h) return fail(MSM_ERR_ARG, "handle is NULL");
+  FF *ff = (FF *)h;
+  g_err.clear();
+  FF_rebuild(ff, edges, position, 0.0);
+  return X(ff)->built ? MSM_OK : MSM_ERR_ARG;
+}
+
+double msmb_energy(void *h, double (*force)[3], double (*position)[3], double *potential) {
+  double E = NAN;
+  if (energy_host((FF *)h, force, position, potential, &E) != MSM_OK) return NAN;
+  return E;
+}
+
+int msmb_configure(void *h, int precision, int timing) {
+  if (!h) return fail(MSM_ERR_ARG, "handle is NULL");
+  TRY(FF_set_precision((FF *)h, precision));
+  return FF_set_stage_timing((FF *)h, timing);
+}
+
+int msmb_info(void *h, int ints[5], double dbl[5]) {
+  TRY(check_built((FF *)h));
+  const FF *ff = (const FF *)h;
+  const msm::Plan &p = X((FF *)h)->eng->plan;
+  ints[0] = p.nu;
+  for (int a = 0; a < 3; ++a) ints[1 + a] = p.M[p.L][a];
+  ints[4] = ff->nlist_interaction_count;
+  dbl[0] = p.aCut[0];
+  dbl[1] = p.beta;
+  dbl[2] = p.coeff1;
+  dbl[3] = p.coeff2;
+  dbl[4] = p.detA;
+  return MSM_OK;
+}
+
+int msmb_tables(void *h, const double **Q, const double *cL[3], const double **khat) {
+  TRY(check_built((FF *)h));
+  const FF *ff = (const FF *)h;
+  if (Q) *Q = ff->Q;
+  if (cL)
+    for (int a = 0; a < 3; ++a) cL[a] = ff->cL[a];
+  if (khat) *khat = ff->khat[ff->maxLevel];
+  return MSM_OK;
+}
+
+int msmb_times(void *h, double t[6]) {
+  if (!h) return fail(MSM_ERR_ARG, "handle is NULL");
+  const FF *ff = (const FF *)h;
+  const int L = ff->maxLevel > 0 && ff->maxLevel <= 9 ? ff->maxLevel : 1;
+  double st = 0;
+  for (int l = 1; l <= L; ++l) st += ff->time_stencil[l];
+  t[0] = ff->time_nlist;
+  t[1] = st;
+  t[2] = ff->time_partcl2partcl;
+  t[3] = ff->time_anterpolation;
+  t[4] = ff->time_grid2grid[L];
+  t[5] = ff->time_interpolation;
+  return MSM_OK;
+}
+
+void msmb_delete(void *h) { FF_delete((FF *)h); }
 
 }  // extern "C"

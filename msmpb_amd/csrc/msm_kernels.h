@@ -32,6 +32,10 @@ struct DevParams {
   double J[kMaxNu + 1];
   double coeff_self;   // coeff1 - gamma(0)/a0 (per-charge constant potential)
   double coeff1, coeff2, gamma0;
+  // PME: the pair term is erfc(beta r)/r (ewald != 0); ewc = 2 beta/sqrt(pi)
+  int ewald;
+  double beta, beta2, ewc;
+  float fbeta, fbeta2, fewc;
 };
 
 struct GridDims {
@@ -172,6 +176,7 @@ struct StencilBox {
   int nu, L;
   int M[kMaxLevel + 1][3];   // level grids
   int S[kMaxLevel + 1][3];   // level stencil extents
+  double beta;               // PME: Ewald splitting parameter
 };
 // Level l's stencil into khat (S_l^3): image sum, then omega' along z (w2),
 // y (w1) and x (w0); tmp_a, tmp_b hold S_l^3 doubles each.
@@ -180,6 +185,8 @@ void launch_stencil_level(const StencilBox &b, int l, const double *w0, const do
                           hipStream_t s);
 // Top-level Fourier kernel into khat (M_L^3); cL = cL_x, cL_y, cL_z packed.
 void launch_stencil_top(const StencilBox &b, const double *cL, double *khat, hipStream_t s);
+// PME's Gaussian Fourier kernel into khat (M_1^3), same cL packing.
+void launch_stencil_top_pme(const StencilBox &b, const double *cL, double *khat, hipStream_t s);
 // Toeplitz B operands of a level stencil (device khat) in MFMA lane order,
 // the same values g2g_bfrag_fill writes on the host.
 void launch_g2g_bfrag_fill(int prec, const G2GLevel &L, const double *khat, void *out,

@@ -336,23 +336,38 @@ __device__ __forceinline__ double rsqrt_f64(double r2) {
 // kernel joins 1/r with a continuous first derivative), so such a neighbour
 // adds only the rounding residue of g(a0) and g'(a0), about 1e-7 of 1/a0.
 // The self pair (d = 0 exactly) is masked through its charge.
+//
+// NU = 0 selects the Ewald term of the PME comparator (pme1.c:160-170):
+// g(r) = erfc(beta r)/r, g'(r)/r = -(g(r) + 2 beta/sqrt(pi) exp(-beta^2 r^2))/r^2.
+// It does not vanish at the cutoff, so neighbours beyond it are masked.
 template <int NU>
 __device__ __forceinline__ void pair_term(const DevParams &P, float4 aj, float ix, float iy,
                                           float iz, float &sa, float &sx, float &sy, float &sz) {
   const float dx = aj.x - ix, dy = aj.y - iy, dz = aj.z - iz;
   const float r2 = dx * dx + dy * dy + dz * dz;
-  const float r2s = __builtin_amdgcn_fmed3f(r2, 1e-20f, P.fa02);
-  const float qj = r2 > 0.f ? aj.w : 0.f;
-  const float sv = r2s * P.fia02 - 1.f;
-  float ts = P.ftau[NU - 1], t1 = P.fdtau[NU - 2];
+  if constexpr (NU == 0) {
+    const bool ok = r2 < P.fa02 && r2 > 0.f;
+    const float r2s = ok ? r2 : P.fa02;
+    const float qj = ok ? aj.w : 0.f;
+    const float rinv = __builtin_amdgcn_rsqf(r2s);
+    const float v = erfcf(P.fbeta * (r2s * rinv)) * rinv;
+    sa += qj * v;
+    const float qc = -qj * (v + P.fewc * __expf(-P.fbeta2 * r2s)) * (rinv * rinv);
+    sx += qc * dx, sy += qc * dy, sz += qc * dz;
+  } else {
+    const float r2s = __builtin_amdgcn_fmed3f(r2, 1e-20f, P.fa02);
+    const float qj = r2 > 0.f ? aj.w : 0.f;
+    const float sv = r2s * P.fia02 - 1.f;
+    float ts = P.ftau[NU - 1], t1 = P.fdtau[NU - 2];
 #pragma unroll
-  for (int k = NU - 2; k >= 0; --k) ts = P.ftau[k] + sv * ts;
+    for (int k = NU - 2; k >= 0; --k) ts = P.ftau[k] + sv * ts;
 #pragma unroll
-  for (int k = NU - 3; k >= 0; --k) t1 = P.fdtau[k] + sv * t1;
-  const float rinv = __builtin_amdgcn_rsqf(r2s);
-  sa += qj * (rinv - ts);
-  const float qc = qj * (-rinv * rinv * rinv - t1);
-  sx += qc * dx, sy += qc * dy, sz += qc * dz;
+    for (int k = NU - 3; k >= 0; --k) t1 = P.fdtau[k] + sv * t1;
+    const float rinv = __builtin_amdgcn_rsqf(r2s);
+    sa += qj * (rinv - ts);
+    const float qc = qj * (-rinv * rinv * rinv - t1);
+    sx += qc * dx, sy += qc * dy, sz += qc * dz;
+  }
 }
 
 // Cell of workgroup b.  Consecutive workgroups are dispatched to the eight
@@ -373,16 +388,23 @@ __device__ __forceinline__ void pair_term(const DevParams &P, double4 aj, double
   const bool ok = r2 < P.a02 && r2 > 0.0;
   const double r2s = ok ? r2 : P.a02;
   const double qj = ok ? aj.w : 0.0;
-  const double sv = r2s * P.inv_a02 - 1.0;
-  double ts = P.tau_s[NU - 1], t1 = P.dtau_s[NU - 2];
-#pragma unroll
-  for (int k = NU - 2; k >= 0; --k) ts = P.tau_s[k] + sv * ts;
-#pragma unroll
-  for (int k = NU - 3; k >= 0; --k) t1 = P.dtau_s[k] + sv * t1;
   const double rinv = rsqrt_f64(r2s);
-  sa += qj * (rinv - ts);
-  const double qc = qj * (-rinv * rinv * rinv - t1);
-  sx += qc * dx, sy += qc * dy, sz += qc * dz;
+  if constexpr (NU == 0) {  // Ewald (PME)
+    const double v = erfc(P.beta * (r2s * rinv)) * rinv;
+    sa += qj * v;
+    const double qc = -qj * (v + P.ewc * exp(-P.beta2 * r2s)) * (rinv * rinv);
+    sx += qc * dx, sy += qc * dy, sz += qc * dz;
+  } else {
+    const double sv = r2s * P.inv_a02 - 1.0;
+    double ts = P.tau_s[NU - 1], t1 = P.dtau_s[NU - 2];
+#pragma unroll
+    for (int k = NU - 2; k >= 0; --k) ts = P.tau_s[k] + sv * ts;
+#pragma unroll
+    for (int k = NU - 3; k >= 0; --k) t1 = P.dtau_s[k] + sv * t1;
+    sa += qj * (rinv - ts);
+    const double qc = qj * (-rinv * rinv * rinv - t1);
+    sx += qc * dx, sy += qc * dy, sz += qc * dz;
+  }
 }
 
 // The staged atoms [0, npad) against the pass's G home atoms: lane l takes
@@ -678,9 +700,12 @@ __global__ __launch_bounds__(kPairBS) void k_pair_count(
           if (j <= i) continue;
           double4 aj = spos[j];
           double dx = aj.x + bx, dy = aj.y + by, dz2 = aj.z + bz;
-          // count each pair once, at its nearest image
-          if (rint(dx * P.ibox[0]) != 0.0 || rint(dy * P.ibox[1]) != 0.0 ||
-              rint(dz2 * P.ibox[2]) != 0.0)
+          // count each pair once, at its nearest image: the one with every
+          // component in [-L/2, L/2), r - floor(r/L + 1/2) L as in the
+          // reference (forcefield0.c:140-148), so that of two images tied at
+          // exactly half a box only one is counted
+          if (floor(dx * P.ibox[0] + 0.5) != 0.0 || floor(dy * P.ibox[1] + 0.5) != 0.0 ||
+              floor(dz2 * P.ibox[2] + 0.5) != 0.0)
             continue;
           if (dx * dx + dy * dy + dz2 * dz2 < P.a02) n += 1.0;
         }
@@ -1681,7 +1706,11 @@ void launch_pair_fp64(const DevParams &P, const double4 *spos, const int *sidx, 
                       hipStream_t s) {
 #define CALL(NU) \
   k_pair_cell<false, NU><<<P.ncell, 64, 0, s>>>(P, spos, sidx, start, cols, ncol, F, phi, epart)
-  MSM_NU_SWITCH(P.nu, CALL)
+  if (P.ewald) {
+    CALL(0);
+  } else {
+    MSM_NU_SWITCH(P.nu, CALL)
+  }
 #undef CALL
 }
 
@@ -1690,7 +1719,11 @@ void launch_pair_fp32(const DevParams &P, const double4 *spos, const int *sidx, 
                       hipStream_t s) {
 #define CALL(NU) \
   k_pair_cell<true, NU><<<P.ncell, 64, 0, s>>>(P, spos, sidx, start, cols, ncol, F, phi, epart)
-  MSM_NU_SWITCH(P.nu, CALL)
+  if (P.ewald) {
+    CALL(0);
+  } else {
+    MSM_NU_SWITCH(P.nu, CALL)
+  }
 #undef CALL
 }
 

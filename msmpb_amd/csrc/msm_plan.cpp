@@ -294,6 +294,116 @@ bool box_dependent(Plan &p, std::string &err, bool stencils = true) {
   return true;
 }
 
+// ---- PME (code/pme0.c) ----------------------------------------------------
+// Ewald splitting: erfc(b) = c for b = beta a0, by Newton's method from
+// b = 0 until the residual stops shrinking (pme0.c:209-224).
+double ewald_root(double c) {
+  double b = 0.0, res = std::erfc(b) - c, old = 2.0 * res;
+  while (std::fabs(res) < std::fabs(old)) {
+    b -= res / (-2.0 / std::sqrt(kPi) * std::exp(-b * b));
+    old = res;
+    res = std::erfc(b) - c;
+  }
+  return b;
+}
+
+// Constants and the Fourier kernel for the current box (FF_rebuild and
+// dALp1, pme0.c:365-412 and :473-503):
+//   coeff1 = sum over the own images |Ap| < a0 of erfc(beta r)/r - 2 beta/sqrt(pi),
+//   coeff2 = pi / (beta^2 V),
+//   khat(k) = exp(-pi^2 |m|^2 / beta^2) / (pi V |m|^2) cL_x cL_y cL_z M^3
+// with m = Ai^T k over the centred frequencies and khat(0) = 0.
+void pme_box_dependent(Plan &p, bool stencils) {
+  const double a0 = p.aCut[0], b = p.beta;
+  int lim[3];
+  for (int a = 0; a < 3; ++a) lim[a] = (int)std::ceil(a0 * col_norm(p.Ai, a) - 1.0);
+  p.coeff1 = -2.0 * b / std::sqrt(kPi);
+  for (int x = -lim[0]; x <= lim[0]; ++x)
+    for (int y = -lim[1]; y <= lim[1]; ++y)
+      for (int z = -lim[2]; z <= lim[2]; ++z) {
+        if (!x && !y && !z) continue;
+        double r2 = 0;
+        for (int a = 0; a < 3; ++a) {
+          double v = p.A[a][0] * x + p.A[a][1] * y + p.A[a][2] * z;
+          r2 += v * v;
+        }
+        if (r2 >= a0 * a0) continue;
+        p.coeff1 += std::erfc(b * std::sqrt(r2)) / std::sqrt(r2);
+      }
+  p.coeff2 = kPi / (b * b * p.detA);
+  p.gamma0 = 0.0;
+  build_omegap(p);  // cL of the single level
+  if (!stencils) return;
+  const int *g = p.M[1];
+  const double n3 = (double)g[0] * g[1] * g[2];
+  std::vector<double> kh((size_t)n3, 0.0);
+  for (int i = 0; i < g[0]; ++i)
+    for (int j = 0; j < g[1]; ++j)
+      for (int k = 0; k < g[2]; ++k) {
+        const int kv[3] = {i <= (g[0] - 1) / 2 ? i : i - g[0], j <= (g[1] - 1) / 2 ? j : j - g[1],
+                           k <= (g[2] - 1) / 2 ? k : k - g[2]};
+        double m2 = 0;
+        for (int a = 0; a < 3; ++a) {
+          const double v = p.Ai[0][a] * kv[0] + p.Ai[1][a] * kv[1] + p.Ai[2][a] * kv[2];
+          m2 += v * v;
+        }
+        if (m2 == 0.0) continue;
+        kh[((size_t)i * g[1] + j) * g[2] + k] = std::exp(-kPi * kPi * m2 / (b * b)) /
+                                                (kPi * p.detA * m2) * p.cL[0][i] * p.cL[1][j] *
+                                                p.cL[2][k] * n3;
+      }
+  p.khat[1] = std::move(kh);
+}
+
+// FF_build of the PME comparator (pme0.c:192-322): order 6, cutoff 8 A and
+// direct tolerance 1e-5 by default; the grid from the reference's spacing
+// table when none is given (pme0.c:226-263, orders 4 and 6, cutoffs 6-10 A).
+bool build_pme(const BuildRequest &r, Plan &p, std::string &err) {
+  char buf[256];
+  p.method = kPme;
+  p.L = 1;
+  const int nu = r.nu > 0 ? r.nu : 6;
+  if (nu != 4 && nu != 6 && nu != 8 && nu != 10) {
+    std::snprintf(buf, sizeof buf, "orderAcc must be 4, 6, 8 or 10 (got %d)", nu);
+    err = buf;
+    return false;
+  }
+  p.nu = nu;
+  const double a0 = r.a0 > 0 ? r.a0 : 8.0;
+  if (!std::isfinite(a0)) { err = "cutoff must be finite"; return false; }
+  p.aCut[0] = a0;
+  p.aCut[1] = 2.0 * a0;
+  p.tolDir = r.tolDir > 0 ? r.tolDir : 1e-5;
+  if (!(p.tolDir * a0 < 1.0)) { err = "tolDir * cutoff must be below 1"; return false; }
+  p.beta = ewald_root(p.tolDir * a0) / a0;
+  if (r.Mtop[0] > 0) {
+    for (int a = 0; a < 3; ++a) p.M[1][a] = r.Mtop[a];
+  } else {
+    static const double href[2][9] = {{0.70, 0.77, 0.83, 0.91, 1.00, 1.11, 1.21, 1.33, 1.48},
+                                      {1.05, 1.18, 1.29, 1.43, 1.54, 1.67, 1.82, 2.00, 2.22}};
+    const int i = (nu - 4) / 2, j = (int)std::lround((a0 - 6.0) / 0.5);
+    if (i > 1 || j < 0 || j > 8) {
+      std::snprintf(buf, sizeof buf,
+                    "no default PME grid for order %d and cutoff %g (set topGridDim)", nu, a0);
+      err = buf;
+      return false;
+    }
+    const double hstar = std::cbrt(p.detA / r.N);
+    const double beta_ref = ewald_root(1e-5 * a0) / a0;
+    const double h = std::pow(p.tolDir / (1e-5 * hstar), 1.0 / nu) * (beta_ref / p.beta) * href[i][j];
+    for (int a = 0; a < 3; ++a) p.M[1][a] = (int)std::ceil(row_norm(p.A, a) / h);
+  }
+  for (int a = 0; a < 3; ++a)
+    if (p.M[1][a] < nu) {
+      std::snprintf(buf, sizeof buf, "PME grid dimension %d is below the order %d", p.M[1][a], nu);
+      err = buf;
+      return false;
+    }
+  bspline_pieces(nu, p.Q);
+  pme_box_dependent(p, true);
+  return true;
+}
+
 }  // namespace
 
 double invert3(const double a[3][3], double ai[3][3]) {
@@ -349,6 +459,7 @@ bool build_plan(const BuildRequest &r, Plan &p, std::string &err) {
     err = "only orthorhombic boxes (diagonal edge matrix) are supported";
     return false;
   }
+  if (r.method == kPme) return build_pme(r, p, err);
   p.estErr = r.estErr > 0 ? r.estErr : 1e-3;
   p.L = r.L > 0 ? r.L : 1;
   if (p.L > kMaxLevel) {
@@ -439,6 +550,10 @@ bool rebuild_plan(Plan &p, const double edges[3][3], std::string &err, bool sten
   for (int l = 1; l <= p.L; ++l) {
     for (int a = 0; a < 3; ++a) p.omegap[l][a].clear();
     p.khat[l].clear();
+  }
+  if (p.method == kPme) {
+    pme_box_dependent(p, stencils);
+    return true;
   }
   return box_dependent(p, err, stencils);
 }

@@ -14,7 +14,14 @@ namespace msm {
 
 constexpr int kMaxLevel = 9;   // the reference's time_* arrays hold 10 levels
 
+// Summation method of a plan.  kPme is the reference's smooth particle-mesh
+// Ewald comparator (code/pme0.c, code/pme1.c): one grid level (L = 1) with
+// the Ewald real-space term erfc(beta r)/r and a Gaussian Fourier kernel.
+enum Method { kMsm = 0, kPme = 1 };
+
 struct Plan {
+  int method = kMsm;
+  double beta = 0, tolDir = 0; // PME: Ewald splitting, erfc(beta a0) = tolDir a0
   int N = 0;
   double A[3][3] = {}, Ai[3][3] = {}, detA = 0;
   bool ortho = true;           // box matrix diagonal
@@ -41,9 +48,12 @@ struct BuildRequest {
   double edges[3][3] = {};
   int nu = -1, L = -1, Mtop[3] = {-1, -1, -1};
   double a0 = -1, estErr = -1;
+  int method = kMsm;
+  double tolDir = -1;          // PME only
 };
 
-// Resolves parameters with the reference's rules and builds every table and
+// Resolves parameters with the reference's rules (FF_build of
+// forcefield0.c, or of pme0.c for method kPme) and builds every table and
 // stencil.  Returns false with a message for inputs outside the domain the
 // reference computes correctly.
 bool build_plan(const BuildRequest &req, Plan &plan, std::string &err);

@@ -13,7 +13,8 @@
 //                    forcefield0.c:995-1031);
 //   k_stencil_top    the top-level kernel khat_L[k] = psi(pi |A^-T k| a_L)
 //                    cL_x cL_y cL_z M^3 (forcefield0.c:806-884): a 7-term
-//                    series for t <= pi, the sin/cos closed form above.
+//                    series for t <= pi, the sin/cos closed form above;
+//   k_stencil_top_pme  the PME comparator's Gaussian kernel (pme0.c:473-503).
 // The anti-blurring weights omega' and cL (a few dozen values per axis) and
 // the scalar constants stay on the host.  msm_plan.cpp computes the same
 // stencils on the host for FF_build; both paths agree to rounding.
@@ -153,6 +154,30 @@ __global__ __launch_bounds__(kStBS) void k_stencil_top(StencilBox b, const doubl
   kh[idx] = psi * c * g0 * g1 * g2;
 }
 
+// PME's Fourier kernel (dALp1, pme0.c:473-503):
+// khat(k) = exp(-pi^2 |m|^2 / beta^2) / (pi V |m|^2) cL_x cL_y cL_z M^3,
+// m = Ai^T k over the centred frequencies, khat(0) = 0.
+__global__ __launch_bounds__(kStBS) void k_stencil_top_pme(StencilBox b,
+                                                           const double *__restrict__ cL,
+                                                           double *__restrict__ kh) {
+  const int g0 = b.M[1][0], g1 = b.M[1][1], g2 = b.M[1][2];
+  const long idx = (long)blockIdx.x * kStBS + threadIdx.x;
+  if (idx >= (long)g0 * g1 * g2) return;
+  const int i = (int)(idx / ((long)g1 * g2)), j = (int)((idx / g2) % g1), k = (int)(idx % g2);
+  const int kv[3] = {i <= (g0 - 1) / 2 ? i : i - g0, j <= (g1 - 1) / 2 ? j : j - g1,
+                     k <= (g2 - 1) / 2 ? k : k - g2};
+  double m2 = 0.0;
+  for (int a = 0; a < 3; ++a) {
+    const double v = b.Ai[0][a] * kv[0] + b.Ai[1][a] * kv[1] + b.Ai[2][a] * kv[2];
+    m2 += v * v;
+  }
+  const double pi = 3.14159265358979323846;
+  const double c = cL[i] * cL[g0 + j] * cL[g0 + g1 + k];
+  kh[idx] = m2 == 0.0 ? 0.0
+                      : exp(-pi * pi * m2 / (b.beta * b.beta)) / (pi * b.detA * m2) * c *
+                            ((double)g0 * g1 * g2);
+}
+
 unsigned st_blocks(long n) { return (unsigned)((n + kStBS - 1) / kStBS); }
 
 }  // namespace
@@ -172,6 +197,11 @@ void launch_stencil_level(const StencilBox &b, int l, const double *w0, const do
 void launch_stencil_top(const StencilBox &b, const double *cL, double *khat, hipStream_t s) {
   const long n = (long)b.M[b.L][0] * b.M[b.L][1] * b.M[b.L][2];
   k_stencil_top<<<st_blocks(n), kStBS, 0, s>>>(b, cL, khat);
+}
+
+void launch_stencil_top_pme(const StencilBox &b, const double *cL, double *khat, hipStream_t s) {
+  const long n = (long)b.M[1][0] * b.M[1][1] * b.M[1][2];
+  k_stencil_top_pme<<<st_blocks(n), kStBS, 0, s>>>(b, cL, khat);
 }
 
 }  // namespace msm

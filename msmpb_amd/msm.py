@@ -15,6 +15,7 @@ _dp = ctypes.POINTER(ctypes.c_double)
 
 PRECISIONS = {"fp64": _lib.MSM_PREC_FP64, "mixed": _lib.MSM_PREC_MIXED}
 LAYOUTS = {"fixed": _lib.MSM_LAYOUT_FIXED, "asis": _lib.MSM_LAYOUT_ASIS}
+METHODS = {"msm": _lib.MSM_METHOD_MSM, "pme": _lib.MSM_METHOD_PME}
 
 
 class MSMError(RuntimeError):
@@ -38,10 +39,15 @@ class MSM:
     nu, levels, top_grid, cutoff, est_err = -1/None select the reference's
     automatic parameter rules (forcefield0.c:286-348).  precision: "fp64" or
     "mixed"; layout: "fixed" or "asis" (reference HEAD index layout).
+    method="pme" evaluates the reference's smooth PME comparator (pme0.c,
+    pme1.c) with the same kernels: one grid of top_grid points, the Ewald
+    pair term with erfc(beta cutoff) = tol_dir * cutoff (tol_dir <= 0:
+    1e-5); levels and est_err do not apply, nu defaults to 6, cutoff to 8.
     """
 
     def __init__(self, positions, charges, edges, nu=-1, levels=-1, top_grid=None, cutoff=-1.0,
-                 est_err=-1.0, precision="fp64", layout="fixed", stage_timing=False):
+                 est_err=-1.0, precision="fp64", layout="fixed", stage_timing=False,
+                 method="msm", tol_dir=-1.0):
         L = _lib.lib()
         self._q = np.ascontiguousarray(charges, dtype=np.float64).copy()
         self.N = int(self._q.shape[0])
@@ -63,7 +69,9 @@ class MSM:
         _check(L.FF_set_precision(self._ff, PRECISIONS[precision]), "precision")
         _check(L.FF_set_layout_compat(self._ff, LAYOUTS[layout]), "layout")
         _check(L.FF_set_stage_timing(self._ff, 1 if stage_timing else 0), "stage timing")
+        _check(L.FF_set_method(self._ff, METHODS[method], float(tol_dir)), "method")
         self.precision = precision
+        self.method = method
         L.FF_build(self._ff, self.N, _p(pos), _p(self.edges), 0.0)
         if L.FF_get_maxLevel(self._ff) <= 0 or not self._built():
             err = _lib.last_error()
@@ -76,6 +84,7 @@ class MSM:
         self.cutoff = ff.cutoff
         self.nlim = ff.nLim
         self.pair_count = ff.nlist_interaction_count
+        self.beta = L.FF_get_beta(self._ff)
 
     def _built(self):
         d = (ctypes.c_int * 3)()

@@ -13,8 +13,10 @@ binary.  Differences, all deliberate:
 * ``runMSM`` accepts ``precision`` ("fp64" or "mixed") and ``layout``
   ("fixed" or "asis") keywords, passed to the CLI as ``--precision`` and
   ``--layout``; without them the command line is the reference's;
-* ``runPME`` raises ``NotImplementedError``: the PME comparator is not part
-  of this build;
+* ``runPME`` runs the PME CLI built by ``msmpb_amd.build`` (``PME_BINARY``,
+  environment variable of the same name, or ``binary=``); the reference
+  hard-codes ``'../pme'``.  Like the reference's pme it ignores ``nu`` (the
+  order is 6 unless ``order=`` is given) and accepts ``precision=``;
 * ``getFactors`` rejects M < 1 with ``ValueError`` (the reference loops
   forever on 0 and negatives).
 """
@@ -26,6 +28,7 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 MSMPB_BINARY = os.environ.get("MSMPB_BINARY", os.path.join(HERE, "bin", "msmpb"))
+PME_BINARY = os.environ.get("PME_BINARY", os.path.join(HERE, "bin", "pme"))
 DATA_DIR = os.environ.get("DATA_DIR", "../../data/")
 
 
@@ -106,10 +109,24 @@ def runMSM(benchmark, M, a0, nu, L=1, bincoor=None, potfile=None, accfile=None, 
     return json.loads(out.stdout.decode("utf-8"))
 
 
-def runPME(benchmark, M, a0, nu, bincoor=None, potfile=None, accfile=None):
-    """The reference's PME comparator (msmpb_wrapper.py:68-81) is not part of
-    this build."""
-    raise NotImplementedError("runPME: the PME comparator is not provided by msmpb_amd")
+def runPME(benchmark, M, a0, nu, bincoor=None, potfile=None, accfile=None, *,
+           order=None, precision=None, binary=None, data_dir=None):
+    """Run the PME CLI on ``data_dir + benchmark['name']`` and return its JSON
+    record (msmpb_wrapper.py:68-81).  ``nu`` is passed as ``--nu``, which
+    the PME program ignores as the reference's does; ``order`` sets it."""
+    binary = binary or PME_BINARY
+    data_dir = DATA_DIR if data_dir is None else data_dir
+    extra = ["--a0", str(a0), "--nu", str(nu), "-M", str(M)]
+    for flag, val in (("--bincoor", bincoor), ("--potfile", potfile), ("--accfile", accfile)):
+        if val is not None:
+            extra += [flag, val]
+    if order is not None:
+        extra += ["--order", str(order)]
+    if precision is not None:
+        extra += ["--precision", precision]
+    out = subprocess.run(_cli_args(binary, data_dir, benchmark["name"], extra),
+                         stdout=subprocess.PIPE)
+    return json.loads(out.stdout.decode("utf-8"))
 
 
 def fudgeFactor(results):

@@ -3,7 +3,8 @@
 TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() and
 by bench.py's cpu_baseline leg, always as the checker, never as the product.
 The oracle restates code/forcefield0.c and code/forcefield1.c of
-hkayabilisim/msmpb (see msm_oracle.c for per-function citations).
+hkayabilisim/msmpb (see msm_oracle.c for per-function citations) and, for
+the PME comparator, code/pme0.c and code/pme1.c (pme_oracle.c).
 """
 import ctypes
 import os
@@ -57,6 +58,16 @@ def lib():
         L.orc_free.argtypes = [ctypes.c_void_p]
         L.orc_build_pairs.restype = ctypes.c_int
         L.orc_build_pairs.argtypes = [ctypes.c_void_p, _dp, ctypes.c_double]
+        L.pmo_build.restype = ctypes.c_void_p
+        L.pmo_build.argtypes = [ctypes.c_int, _dp, _dp, _dp, ctypes.c_int, _ip, ctypes.c_double,
+                                ctypes.c_double, ctypes.c_char_p]
+        L.pmo_energy.restype = ctypes.c_double
+        L.pmo_energy.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp, _dp]
+        L.pmo_info.argtypes = [ctypes.c_void_p, _ip, _dp]
+        L.pmo_copy_khat.argtypes = [ctypes.c_void_p, _dp]
+        L.pmo_copy_grid.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp]
+        L.pmo_bspline.argtypes = [ctypes.c_int, ctypes.c_double, _dp, _dp]
+        L.pmo_free.argtypes = [ctypes.c_void_p]
         _lib = L
     return _lib
 
@@ -185,3 +196,74 @@ class Oracle:
 
     def deltaF(self):
         return lib().orc_deltaF(self._h)
+
+
+class PmeOracle:
+    """FP64 CPU smooth PME with the reference pme's parameter rules.
+
+    nu, a0, tolDir <= 0 and M = None select the reference's defaults
+    (order 6, cutoff 8 A, direct tolerance 1e-5, grid from its spacing
+    table).
+    """
+
+    def __init__(self, pos, q, edges, nu=-1, M=None, a0=-1.0, tolDir=-1.0):
+        self.q = np.ascontiguousarray(q, dtype=np.float64)
+        self.N = self.q.shape[0]
+        pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(self.N, 3)
+        self.edges = np.ascontiguousarray(edges, dtype=np.float64).reshape(3, 3)
+        err = ctypes.create_string_buffer(512)
+        mtp = None
+        if M is not None:
+            if np.isscalar(M):
+                M = [int(M)] * 3
+            mt = (ctypes.c_int * 3)(*[int(m) for m in M])
+            mtp = ctypes.cast(mt, _ip)
+        self._h = lib().pmo_build(self.N, _p(self.q), _p(pos), _p(self.edges), int(nu), mtp,
+                                  float(a0), float(tolDir), err)
+        if not self._h:
+            raise ValueError(err.value.decode())
+        info = (ctypes.c_int * 5)()
+        dinfo = (ctypes.c_double * 6)()
+        lib().pmo_info(self._h, info, dinfo)
+        self.nu = info[0]
+        self.M = (info[1], info[2], info[3])
+        self.pair_count = info[4]
+        self.a0, self.beta, self.coeff1, self.coeff2, self.detA, self.tolDir = list(dinfo)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and _lib is not None:
+            try:
+                _lib.pmo_free(h)
+            except Exception:  # interpreter shutdown
+                pass
+            self._h = None
+
+    def energy(self, pos, q=None, want_phi=True):
+        """Returns (E, F[N,3], phi[N] or None, terms[3])."""
+        pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(self.N, 3)
+        qq = self.q if q is None else np.ascontiguousarray(q, dtype=np.float64)
+        F = np.zeros((self.N, 3))
+        phi = np.zeros(self.N) if want_phi else None
+        terms = np.zeros(3)
+        E = lib().pmo_energy(self._h, _p(pos), _p(qq), _p(F), _p(phi) if want_phi else None,
+                             _p(terms))
+        return E, F, phi, terms
+
+    def khat(self):
+        out = np.zeros(self.M)
+        lib().pmo_copy_khat(self._h, _p(out))
+        return out
+
+    def grid(self, which):
+        out = np.zeros(self.M)
+        lib().pmo_copy_grid(self._h, which, _p(out))
+        return out
+
+
+def pme_bspline(nu, t):
+    """(w, dw): M_nu(i + t) and its derivative for i = 0 .. nu-1."""
+    w = np.zeros(nu)
+    dw = np.zeros(nu)
+    lib().pmo_bspline(int(nu), float(t), _p(w), _p(dw))
+    return w, dw

@@ -8,7 +8,10 @@ Outputs (all data, no code):
       data/LICENSE); Box44.ini is stored gzip-compressed.
   tests/golden/published.json         -- values the reference's own runs
       published in results/: total energies of TableII, Figure4 and Figure6
-      (Box44), and the 4-digit force errors of Figure1 and Figure2.
+      (Box44), and the 4-digit force errors of Figure1 and Figure2; and
+      the PME comparator's runs of Figure4 ("Figure4_pme": energy, beta,
+      pair count; the reference's pme CLI ignores --nu, so every run has
+      the order "nu" = 6 it reports, whatever "nu_arg" asked for).
 """
 import gzip
 import json
@@ -45,6 +48,11 @@ def main():
         pub[fig] = [{"L": t.get("L", t["msmout"]["NumberOfLevels"]), "nu": t["nu"], "M": t["M"],
                      "a0": t["a0"], **{k: t["msmout"][k] for k in keep if k in t["msmout"]}}
                     for t in d["trial"]]
+        if fig == "Figure4":
+            pk = ["nu", "cutoff", "TopLevelMx", "beta", "tol_dir", "utotal",
+                  "nlist_interaction_count", "poterror", "deltaF/Fref", "time_energy", "time_nlist"]
+            pub["Figure4_pme"] = [{"M": t["M"], "a0": t["a0"], "nu_arg": t["nu"],
+                                   **{k: t["pmeout"][k] for k in pk}} for t in d["trial"]]
     rows = []
     with open(os.path.join(REF, "results", "Figure1.txt")) as f:
         f.readline()

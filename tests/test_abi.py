@@ -10,12 +10,13 @@ import pytest
 
 from helpers import ROOT
 
-HEADERS = [os.path.join(ROOT, "include", h) for h in ("forcefield.h", "msmpb_hip.h")]
+HEADERS = [os.path.join(ROOT, "include", h) for h in ("forcefield.h", "msmpb_hip.h", "msm_bridge.h")]
+PME_HEADERS = [os.path.join(ROOT, "include", h) for h in ("pme.h", "pme_hip.h")]
 
 
-def declared_functions():
+def declared_functions(headers=HEADERS):
     names = set()
-    for h in HEADERS:
+    for h in headers:
         text = open(h).read()
         text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
         text = re.sub(r"#define[^\n]*(\\\n[^\n]*)*", "", text)
@@ -79,3 +80,56 @@ def test_cli_built_and_links():
         pytest.skip("CLI not built")
     r = subprocess.run([cli], capture_output=True, text=True)
     assert r.returncode == 1 and "Usage: msmpb" in r.stderr
+
+
+def test_pme_library_exports_every_declared_symbol():
+    """libpme_hip.so implements include/pme.h (the reference's code/pme.h)
+    and pme_hip.h; FF_last_error comes from libmsmpb_hip.so."""
+    from msmpb_amd import _lib
+    P = _lib.pme_lib()
+    names = declared_functions(PME_HEADERS)
+    assert len(names) >= 19
+    missing = [n for n in names if not hasattr(P, n)]
+    assert not missing, missing
+    assert set(names) - {"FF_last_error"} <= set(_lib.PME_EXPORTS)
+    out = subprocess.check_output(["nm", "-D", "--defined-only", _lib.PME_LIB_PATH], text=True)
+    defined = {l.split()[-1] for l in out.splitlines() if " T " in l}
+    assert set(_lib.PME_EXPORTS) <= defined
+
+
+def test_pme_struct_layout_matches_c(tmp_path):
+    from msmpb_amd._lib import PMEStruct
+    fields = [f[0] for f in PMEStruct._fields_]
+    src = tmp_path / "off.c"
+    body = "\n".join(f'  printf("{f} %zu\\n", offsetof(FF, {f}));' for f in fields)
+    src.write_text('#include <stddef.h>\n#include <stdio.h>\n#include "pme.h"\n'
+                   f"int main(void) {{\n{body}\n  printf(\"sizeof %zu\\n\", sizeof(FF));\n}}\n")
+    exe = tmp_path / "off"
+    subprocess.check_call(["gcc", "-std=gnu11", "-I", os.path.join(ROOT, "include"), str(src),
+                           "-o", str(exe)])
+    out = dict(l.split() for l in subprocess.check_output([str(exe)], text=True).splitlines())
+    for f in fields:
+        assert int(out[f]) == getattr(PMEStruct, f).offset, f
+    assert int(out["sizeof"]) == ctypes.sizeof(PMEStruct)
+
+
+def test_pme_reference_header_layout_offsets():
+    """Offsets fixed by code/pme.h:15-44 on LP64 (pointers 8 bytes, the
+    three FFTW members pointer-sized)."""
+    from msmpb_amd._lib import PMEStruct
+    assert PMEStruct.A.offset == 16 and PMEStruct.detA.offset == 160
+    assert PMEStruct.cutoff.offset == 168 and PMEStruct.orderAcc.offset == 176
+    assert PMEStruct.topGridDim.offset == 180 and PMEStruct.nlist.offset == 192
+    assert PMEStruct.nlist_interaction_count.offset == 204 and PMEStruct.tolDir.offset == 208
+    assert PMEStruct.fftw_in.offset == 216 and PMEStruct.beta.offset == 240
+    assert PMEStruct.khat.offset == 280 and PMEStruct.coeff1.offset == 288
+    assert PMEStruct.time_partcl2partcl.offset == 304
+    assert ctypes.sizeof(PMEStruct) == 352
+
+
+def test_pme_cli_built_and_links():
+    cli = os.path.join(ROOT, "msmpb_amd", "bin", "pme")
+    if not os.path.exists(cli):
+        pytest.skip("CLI not built")
+    r = subprocess.run([cli], capture_output=True, text=True)
+    assert r.returncode == 1 and "Usage: pme" in r.stderr

@@ -73,9 +73,19 @@ def test_default_data_dir_matches_reference():
     assert W.DATA_DIR == os.environ.get("DATA_DIR", "../../data/")
 
 
-def test_run_pme_not_provided():
-    with pytest.raises(NotImplementedError):
-        W.runPME({"name": "x"}, 8, 9.0, 4)
+def test_run_pme_command_line(tmp_path):
+    """runPME passes the reference's arguments (msmpb_wrapper.py:68-81)."""
+    stub = _stub_binary(tmp_path)
+    bench = {"name": "Box44"}
+    out = W.runPME(bench, 48, 9, 4, binary=stub, data_dir="/data/")
+    assert out["argv"] == ["/data/Box44", "--a0", "9", "--nu", "4", "-M", "48"]
+    out = W.runPME(bench, 30, 12.0, 6, bincoor="c.coor", potfile="p.pot", accfile="a.acc",
+                   order=8, precision="mixed", binary=stub, data_dir="")
+    assert out["argv"] == ["Box44", "--a0", "12.0", "--nu", "6", "-M", "30", "--bincoor", "c.coor",
+                           "--potfile", "p.pot", "--accfile", "a.acc", "--order", "8",
+                           "--precision", "mixed"]
+    assert W.PME_BINARY == os.environ.get(
+        "PME_BINARY", os.path.join(ROOT, "msmpb_amd", "bin", "pme"))
 
 
 def test_fudge_factor_is_geometric_mean_ratio():
@@ -161,18 +171,22 @@ def test_journal_figure2_through_cli(row, tmp_path, monkeypatch):
     assert rel_err(out["deltaFest/Fref"], t["deltaFest/Fref"]) < 2e-3, (out["deltaFest/Fref"], t)
 
 
-@pytest.mark.gpu
-def test_journal_table2_through_cli(tmp_path, monkeypatch):
+def _box44_plain(data, tmp_path):
     import gzip
     import shutil
-    binary, data = _cli(monkeypatch, tmp_path)
-    for ext in (".ini", ".pot"):  # the CLI reads plain files; the fixture is compressed
+    for ext in (".ini", ".pot"):  # the CLIs read plain files; the fixture is compressed
         src = os.path.join(data, "Box44" + ext)
         if os.path.exists(src):
             shutil.copy(src, tmp_path / ("Box44" + ext))
         else:
             with gzip.open(src + ".gz", "rb") as f, open(tmp_path / ("Box44" + ext), "wb") as g:
                 shutil.copyfileobj(f, g)
+
+
+@pytest.mark.gpu
+def test_journal_table2_through_cli(tmp_path, monkeypatch):
+    binary, data = _cli(monkeypatch, tmp_path)
+    _box44_plain(data, tmp_path)
     t = PUB["TableII"][0]
     print(f"[journal] TableII Box44 L={t['L']} nu={t['nu']} M={t['M']} a0={t['a0']}", flush=True)
     out = W.runMSM({"name": "Box44"}, t["M"] // 2 ** (t["L"] - 1), t["a0"], t["nu"], L=t["L"],
@@ -181,3 +195,23 @@ def test_journal_table2_through_cli(tmp_path, monkeypatch):
     assert rel_err(out["utotal"], t["utotal"]) < 1e-10, (out["utotal"], t["utotal"])
     assert out["nlist_interaction_count"] == t["nlist_interaction_count"]
     assert rel_err(out["poterror"], t["poterror"]) < 1e-6
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("row", [0, 13])
+def test_journal_figure4_pme_through_cli(row, tmp_path, monkeypatch):
+    """code/journal/Figure4_run.py calls runPME(Box44, M, a0, nu); the GPU
+    PME CLI reproduces the published record (order 6 whatever nu asks)."""
+    _, data = _cli(monkeypatch, tmp_path)
+    _box44_plain(data, tmp_path)
+    t = PUB["Figure4_pme"][row]
+    print(f"[journal] Figure4 PME Box44 M={t['M']} a0={t['a0']} nu={t['nu_arg']}", flush=True)
+    out = W.runPME({"name": "Box44"}, t["M"], t["a0"], t["nu_arg"],
+                   binary=os.path.join(ROOT, "msmpb_amd", "bin", "pme"),
+                   data_dir=str(tmp_path) + "/")
+    assert out["nu"] == 6 and out["TopLevelMx"] == t["TopLevelMx"]
+    assert rel_err(out["utotal"], t["utotal"]) < 1e-10, (out["utotal"], t["utotal"])
+    assert out["beta"] == t["beta"] and out["tol_dir"] == t["tol_dir"]
+    assert out["nlist_interaction_count"] == t["nlist_interaction_count"]
+    assert rel_err(out["poterror"], t["poterror"]) < 1e-6
+    assert os.path.exists(tmp_path / "pme.acc") and os.path.exists(tmp_path / "pme.pot")
