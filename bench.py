@@ -26,6 +26,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "MSM energy+force evals/s at 96k atoms, mixed vs FP64 GPU; rms rel force err"
 PEAK = {"hbm": 8000.0, "fp64": 78.6, "fp32": 157.3, "bf16": 2500.0}  # GB/s, TFLOP/s
+PME_GRID = 96  # PME comparator grid for water96k: spacing 1.03 A, order 6
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "water96k_hbm_traffic.txt")
 SQ_FILE = os.path.join(ROOT, "profiles", "water96k_kernel_stats.md")
 # VALU issue rate of gfx950: a wave64 VALU instruction occupies one of a
@@ -133,7 +134,7 @@ def device_free():
     return torch.cuda.mem_get_info(0)[0]
 
 
-def run_precisions(args, sysm, precisions=("fp64", "mixed"), steps=None, warmup=None):
+def run_precisions(args, sysm, precisions=("fp64", "mixed"), steps=None, warmup=None, kw=None):
     """Timed evaluations per precision through msm_eval_device: a plain timed
     region of K steps, then K more steps with per-stage HIP events on the
     streams the kernels run on (kept out of the plain timed region)."""
@@ -144,7 +145,8 @@ def run_precisions(args, sysm, precisions=("fp64", "mixed"), steps=None, warmup=
     warmup = args.warmup if warmup is None else warmup
     dev = torch.device("cuda:0")
     p = dict(sysm.params)
-    kw = dict(nu=p["nu"], levels=p["levels"], top_grid=p["top_grid"], cutoff=p["cutoff"])
+    if kw is None:
+        kw = dict(nu=p["nu"], levels=p["levels"], top_grid=p["top_grid"], cutoff=p["cutoff"])
     pos = torch.tensor(sysm.pos, dtype=torch.float64, device=dev)
     q = torch.tensor(sysm.q, dtype=torch.float64, device=dev)
     out = torch.empty(4, dtype=torch.float64, device=dev)
@@ -289,6 +291,18 @@ def main_water96k(args):
                        / abs(r4["fp64"]["E"])}
         for t in ("fp64", "mixed"):
             r4[t]["m"].close()
+    # the reference's PME comparator (pme0.c, pme1.c) on the same workload
+    if args.nu == 6:
+        kp = dict(nu=6, top_grid=PME_GRID, cutoff=10.0, method="pme")
+        rp = run_precisions(args, sysm, kw=kp)
+        line["pme"] = {
+            "config": f"smooth PME, order 6, grid {PME_GRID}^3, cutoff 10 A, tolDir 1e-5",
+            "fp64_evals_per_s": 1e3 / rp["fp64"]["ms"], "mixed_evals_per_s": 1e3 / rp["mixed"]["ms"],
+            "rms_rel_force_err": rms_rel(rp["mixed"]["F"], rp["fp64"]["F"]),
+            "stage_ms_fp64": {k: v * 1e3 for k, v in rp["fp64"]["stages"].items()},
+            "rms_rel_force_diff_msm_fp64": rms_rel(F64, rp["fp64"]["F"])}
+        for t in ("fp64", "mixed"):
+            rp[t]["m"].close()
     if not args.no_cpu:
         cb, Ec, Fc = cpu_baseline(sysm)
         line["cpu_baseline"] = cb
