@@ -43,6 +43,17 @@ struct GridDims {
   __host__ __device__ long size() const { return (long)x * y * z; }
 };
 
+// One axis of the top-level FFT: length, radices of its Stockham stages and
+// lines per workgroup.
+constexpr int kFftMaxStages = 13;
+struct FftAxis {
+  int M, nst, B;
+  int rad[kFftMaxStages];
+};
+// Plans an axis; false when M has a prime factor above 7 or is too long
+// (the top level then falls back to the direct separable DFT).
+bool fft_axis_plan(int M, FftAxis &A);
+
 // ---- launchers (stream-ordered, no synchronisation) ----
 void launch_fill_int(int *p, long n, int v, hipStream_t s);
 void launch_fill_dbl(double *p, long n, double v, hipStream_t s);

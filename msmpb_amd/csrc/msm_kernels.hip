@@ -9,6 +9,7 @@
 // level stencil pass.  Wave size is 64.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -1149,6 +1150,132 @@ __global__ __launch_bounds__(256) void k_top_out(long n, const double *__restric
   if (e < n) eL[e] = re[e] / (double)n;
 }
 
+// Grids above kTopSmall points whose axes factor into 2, 3, 4, 5 and 7 (every
+// MSM top grid; PME grids in practice): per axis, each workgroup loads B
+// whole lines into LDS and transforms them with a mixed-radix Stockham FFT
+// (natural order in and out).  Stage s of radix R, with Ns the product of
+// the radices before it: butterfly j of a line reads x[j + r M/R],
+// multiplies by W^(r (j mod Ns) M/(Ns R)), takes the R-point DFT and writes
+// y[(j - j mod Ns) R + j mod Ns + q Ns]; W^k = exp(i sign 2 pi k / M) comes
+// from the axis' table of M roots.  A workgroup reads and writes only its
+// own lines, so the passes run in place.  Five passes: forward z (reading
+// the real grid), forward y, forward x + product with khat + inverse x in
+// one LDS tile, inverse y, inverse z (writing Re / M^3).
+constexpr int kFftBS = 256;
+constexpr int kFftLds = 65536;  // LDS per workgroup: roots + two line buffers
+
+template <int R>
+__device__ __forceinline__ void fft_stage(const double2 *__restrict__ X, double2 *__restrict__ Y,
+                                          const double2 *__restrict__ W, int M, int nb, int Ns,
+                                          double sign) {
+  const int mr = M / R, span = M / (Ns * R);
+  double2 root[R];  // exp(i sign 2 pi k / R)
+#pragma unroll
+  for (int k = 0; k < R; ++k) root[k] = make_double2(W[k * mr].x, sign * W[k * mr].y);
+  for (int t = threadIdx.x; t < nb * mr; t += kFftBS) {
+    const int b = t / mr, j = t - b * mr, jm = j % Ns;
+    const double2 *x = X + b * M;
+    double2 v[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const double2 a = x[j + r * mr];
+      const double2 w = W[r * jm * span];  // r jm span < R Ns span = M
+      const double ws = sign * w.y;
+      v[r] = make_double2(a.x * w.x - a.y * ws, a.x * ws + a.y * w.x);
+    }
+    double2 *y = Y + b * M + (j - jm) * R + jm;
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      double sr = 0.0, si = 0.0;
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const double2 w = root[(r * q) % R];
+        sr += v[r].x * w.x - v[r].y * w.y;
+        si += v[r].x * w.y + v[r].y * w.x;
+      }
+      y[q * Ns] = make_double2(sr, si);
+    }
+  }
+}
+
+// All stages of the nb lines in X (result left in X; Y is scratch).
+__device__ void fft_lines(double2 *&X, double2 *&Y, const double2 *W, int M, int nb,
+                          const FftAxis &A, double sign) {
+  int Ns = 1;
+  for (int s = 0; s < A.nst; ++s) {
+    const int R = A.rad[s];
+    switch (R) {
+      case 2: fft_stage<2>(X, Y, W, M, nb, Ns, sign); break;
+      case 3: fft_stage<3>(X, Y, W, M, nb, Ns, sign); break;
+      case 4: fft_stage<4>(X, Y, W, M, nb, Ns, sign); break;
+      case 5: fft_stage<5>(X, Y, W, M, nb, Ns, sign); break;
+      default: fft_stage<7>(X, Y, W, M, nb, Ns, sign); break;
+    }
+    __syncthreads();
+    double2 *t = X;
+    X = Y;
+    Y = t;
+    Ns *= R;
+  }
+}
+
+// Grid index of element k of line l along an axis with `inner` points after it.
+__device__ __forceinline__ long fft_index(long l, int k, int M, long inner) {
+  return (l / inner) * (M * inner) + (l % inner) + (long)k * inner;
+}
+
+// MODE 0: forward z from the real grid qL; 1: one direction of y (sign);
+// 2: forward x, times khat, inverse x; 3: inverse z into eL (real part / n).
+template <int MODE>
+__global__ __launch_bounds__(kFftBS) void k_fft_pass(DevParams P, GridDims g, FftAxis A, int axis,
+                                                     double sign, double2 *__restrict__ buf,
+                                                     const double *__restrict__ tw,
+                                                     const double *__restrict__ khat,
+                                                     const double *__restrict__ qL,
+                                                     double *__restrict__ eL, int asis) {
+  extern __shared__ double2 fft_lds[];
+  const int M = A.M, B = A.B;
+  const long inner = axis == 2 ? 1 : axis == 1 ? (long)g.z : (long)g.y * g.z;
+  const long lines = g.size() / M;
+  const long l0 = (long)blockIdx.x * B;
+  const int nb = (int)min((long)B, lines - l0);
+  double2 *W = fft_lds, *X = fft_lds + M, *Y = X + B * M;
+  const double *c = tw + 2 * (axis == 0 ? 0 : axis == 1 ? g.x : g.x + g.y);
+  for (int k = threadIdx.x; k < M; k += kFftBS) W[k] = make_double2(c[2 * k], c[2 * k + 1]);
+  // consecutive threads take consecutive addresses: along the line where it
+  // is contiguous (z), across lines otherwise
+  for (int e = threadIdx.x; e < nb * M; e += kFftBS) {
+    const int b = inner == 1 ? e / M : e % nb, k = inner == 1 ? e % M : e / nb;
+    const long l = l0 + b;
+    double2 v;
+    if constexpr (MODE == 0)
+      v = make_double2(top_input(g, qL, (int)(l / g.y), (int)(l % g.y), k, asis, P.nu), 0.0);
+    else
+      v = buf[fft_index(l, k, M, inner)];
+    X[b * M + k] = v;
+  }
+  __syncthreads();
+  fft_lines(X, Y, W, M, nb, A, sign);
+  if constexpr (MODE == 2) {
+    for (int e = threadIdx.x; e < nb * M; e += kFftBS) {
+      const int b = e / M, k = e % M;
+      const double kh = khat[fft_index(l0 + b, k, M, inner)];
+      X[e] = make_double2(X[e].x * kh, X[e].y * kh);
+    }
+    __syncthreads();
+    fft_lines(X, Y, W, M, nb, A, -sign);
+  }
+  const double inv_n = 1.0 / (double)g.size();
+  for (int e = threadIdx.x; e < nb * M; e += kFftBS) {
+    const int b = inner == 1 ? e / M : e % nb, k = inner == 1 ? e % M : e / nb;
+    const long idx = fft_index(l0 + b, k, M, inner);
+    if constexpr (MODE == 3)
+      eL[idx] = X[b * M + k].x * inv_n;
+    else
+      buf[idx] = X[b * M + k];
+  }
+}
+
 // ---------------------------------------------------------------- levels
 // e^l = P e^{l+1} + K_l * q^l.  The stencil pass is grid2grid
 // (forcefield0.c:898-931): e_m = sum_k khat_k q_{m+k-s/2}, periodic.
@@ -1749,11 +1876,41 @@ void launch_restrict(const DevParams &P, GridDims fine, GridDims coarse, const d
 #undef CALL
 }
 
+// Radices of an FFT of length M (4 first, then 2, 3, 5, 7) and the lines per
+// workgroup that fit kFftLds; false if M has another prime factor or one
+// line does not fit.
+bool fft_axis_plan(int M, FftAxis &A) {
+  A.M = M;
+  A.nst = 0;
+  int m = M;
+  for (int p : {4, 2, 3, 5, 7})
+    while (m % p == 0 && A.nst < kFftMaxStages) {
+      A.rad[A.nst++] = p;
+      m /= p;
+    }
+  const int per_line = 2 * M * (int)sizeof(double2);
+  A.B = std::min(32, (kFftLds - M * (int)sizeof(double2)) / per_line);
+  return m == 1 && A.B >= 1;
+}
+
 void launch_top(const DevParams &P, GridDims g, const double *qL, const double *khat,
                 const double *tw, double *work, double *eL, int asis, hipStream_t s) {
   long n = g.size();
   if (n <= kTopSmall) {
     k_top_small<<<1, kTopBS, 4 * n * sizeof(double), s>>>(P, g, qL, khat, tw, eL, asis);
+    return;
+  }
+  FftAxis ax[3];
+  if (fft_axis_plan(g.x, ax[0]) && fft_axis_plan(g.y, ax[1]) && fft_axis_plan(g.z, ax[2])) {
+    double2 *buf = reinterpret_cast<double2 *>(work);
+    const int dims[3] = {g.x, g.y, g.z};
+    auto grid = [&](int a) { return (unsigned)((n / dims[a] + ax[a].B - 1) / ax[a].B); };
+    auto lds = [&](int a) { return (size_t)(dims[a] + 2 * ax[a].B * dims[a]) * sizeof(double2); };
+    k_fft_pass<0><<<grid(2), kFftBS, lds(2), s>>>(P, g, ax[2], 2, -1.0, buf, tw, khat, qL, eL, asis);
+    k_fft_pass<1><<<grid(1), kFftBS, lds(1), s>>>(P, g, ax[1], 1, -1.0, buf, tw, khat, qL, eL, asis);
+    k_fft_pass<2><<<grid(0), kFftBS, lds(0), s>>>(P, g, ax[0], 0, -1.0, buf, tw, khat, qL, eL, asis);
+    k_fft_pass<1><<<grid(1), kFftBS, lds(1), s>>>(P, g, ax[1], 1, 1.0, buf, tw, khat, qL, eL, asis);
+    k_fft_pass<3><<<grid(2), kFftBS, lds(2), s>>>(P, g, ax[2], 2, 1.0, buf, tw, khat, qL, eL, asis);
     return;
   }
   double *r0 = work, *i0 = work + n, *r1 = work + 2 * n, *i1 = work + 3 * n;

@@ -58,6 +58,8 @@ CASES = {
     # sit exactly on it, where erfc(beta r)/r does not vanish); > box/2
     "nacl512": (_dataset("NaClN512"), 8, 16, 4.1, 1e-8),
     "random3000": (_random3000, 10, (30, 36, 40), 9.0, -1.0),
+    # 22 = 2 x 11 and 26 = 2 x 13: no mixed-radix FFT, the direct DFT path
+    "spce300_grid_11_13": (_dataset("spceN300"), 6, (22, 25, 26), 9.0, -1.0),
 }
 
 
