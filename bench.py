@@ -41,7 +41,10 @@ def parse():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--nu", type=int, default=6)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--workload", default="water96k", choices=["water96k", "water1m"])
+    ap.add_argument("--workload", default="water96k", choices=["water96k", "water1m", "pme96k"],
+                    help="pme96k: only the PME comparator on the water96k system")
+    ap.add_argument("--no-pme", action="store_true",
+                    help="skip the PME comparator sub-measurement of water96k")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB five times (the reference "
                          "kernel for scaling FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -247,6 +250,22 @@ def roofline(stages, m, pairs, nu):
     return out
 
 
+def pme_comparator(args, sysm, F64=None):
+    """Smooth PME (order 6, 96^3 grid, cutoff 10 A) on the same system."""
+    kp = dict(nu=6, top_grid=PME_GRID, cutoff=10.0, method="pme")
+    rp = run_precisions(args, sysm, kw=kp)
+    out = {"config": f"smooth PME, order 6, grid {PME_GRID}^3, cutoff 10 A, tolDir 1e-5",
+           "fp64_evals_per_s": 1e3 / rp["fp64"]["ms"], "mixed_evals_per_s": 1e3 / rp["mixed"]["ms"],
+           "rms_rel_force_err": rms_rel(rp["mixed"]["F"], rp["fp64"]["F"]),
+           "stage_ms_fp64": {k: v * 1e3 for k, v in rp["fp64"]["stages"].items()},
+           "stage_ms_mixed": {k: v * 1e3 for k, v in rp["mixed"]["stages"].items()}}
+    if F64 is not None:
+        out["rms_rel_force_diff_msm_fp64"] = rms_rel(F64, rp["fp64"]["F"])
+    for t in ("fp64", "mixed"):
+        rp[t]["m"].close()
+    return out
+
+
 def main_water96k(args):
     from msmpb_amd import inputs
     sysm = inputs.cfg3_water96k(args.nu)
@@ -292,17 +311,8 @@ def main_water96k(args):
         for t in ("fp64", "mixed"):
             r4[t]["m"].close()
     # the reference's PME comparator (pme0.c, pme1.c) on the same workload
-    if args.nu == 6:
-        kp = dict(nu=6, top_grid=PME_GRID, cutoff=10.0, method="pme")
-        rp = run_precisions(args, sysm, kw=kp)
-        line["pme"] = {
-            "config": f"smooth PME, order 6, grid {PME_GRID}^3, cutoff 10 A, tolDir 1e-5",
-            "fp64_evals_per_s": 1e3 / rp["fp64"]["ms"], "mixed_evals_per_s": 1e3 / rp["mixed"]["ms"],
-            "rms_rel_force_err": rms_rel(rp["mixed"]["F"], rp["fp64"]["F"]),
-            "stage_ms_fp64": {k: v * 1e3 for k, v in rp["fp64"]["stages"].items()},
-            "rms_rel_force_diff_msm_fp64": rms_rel(F64, rp["fp64"]["F"])}
-        for t in ("fp64", "mixed"):
-            rp[t]["m"].close()
+    if args.nu == 6 and not args.no_pme:
+        line["pme"] = pme_comparator(args, sysm, F64)
     if not args.no_cpu:
         cb, Ec, Fc = cpu_baseline(sysm)
         line["cpu_baseline"] = cb
@@ -354,6 +364,14 @@ def main():
         sys.exit(2)
     if args.workload == "water1m":
         main_water1m(args)
+    elif args.workload == "pme96k":
+        from msmpb_amd import inputs
+        r = pme_comparator(args, inputs.cfg3_water96k(6))
+        print(json.dumps({"metric": "PME comparator evals/s at 96k atoms (not the headline)",
+                          "value": r["mixed_evals_per_s"], "unit": "evals/s", "n_gpus": 1,
+                          "steps": args.steps, "warmup": args.warmup, "higher_is_better": True,
+                          "data": "synthetic", "config": {"workload": "water96k " + r["config"]},
+                          **r}))
     else:
         main_water96k(args)
 
