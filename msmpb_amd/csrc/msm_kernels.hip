@@ -738,6 +738,7 @@ constexpr int kSRanges = 512;       // index ranges per batch of cell columns
 constexpr int kWXs = 17, kWYs = 5;  // padded record strides (doubles)
 constexpr int kSFlush = 32;         // atoms per flush (records per wavefront)
 constexpr int kSQueue = kSFlush + 64;  // queued atoms: < kSFlush + one batch of 64
+constexpr unsigned kSpreadFine = 2048;  // bricks from which one wavefront takes a brick
 
 typedef double sp_f64x4 __attribute__((ext_vector_type(4)));
 
@@ -851,15 +852,19 @@ __device__ __forceinline__ void spread_flush(const DevParams &P, const double4 &
   __builtin_amdgcn_wave_barrier();
 }
 
-template <int NU>
-__global__ __launch_bounds__(256) void k_spread_mfma(DevParams P, GridDims g,
-                                                     const double4 *__restrict__ spos,
-                                                     const int *__restrict__ start,
-                                                     double *__restrict__ q1) {
+// WPB wavefronts per brick: 4 where the grid has few bricks (they split the
+// brick's candidates), 1 on fine grids, where the bricks alone fill the GPU
+// and a wavefront needs no barrier partners and about 11 KB of LDS.
+template <int NU, int WPB>
+__global__ __launch_bounds__(64 * WPB) void k_spread_mfma(DevParams P, GridDims g,
+                                                          const double4 *__restrict__ spos,
+                                                          const int *__restrict__ start,
+                                                          double *__restrict__ q1) {
+  constexpr int BS = 64 * WPB;
   // per wavefront: records of kSFlush atoms and a queue of sorted indices
-  __shared__ double WX[4 * kSFlush * kWXs];
-  __shared__ double WY[4 * kSFlush * kWYs], WZ[4 * kSFlush * kWYs];
-  __shared__ double4 queue[4 * kSQueue];
+  __shared__ double WX[WPB * kSFlush * kWXs];
+  __shared__ double WY[WPB * kSFlush * kWYs], WZ[WPB * kSFlush * kWYs];
+  __shared__ double4 queue[WPB * kSQueue];
   __shared__ int rbeg[kSRanges], roff[kSRanges + 1];
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int nbz = (g.z + kSZ - 1) / kSZ, nby = (g.y + kSY - 1) / kSY;
@@ -882,7 +887,7 @@ __global__ __launch_bounds__(256) void k_spread_mfma(DevParams P, GridDims g,
   for (int c0 = 0; c0 < ncol; c0 += kSRanges / 2) {
     const int nc_b = min(ncol - c0, kSRanges / 2);
     __syncthreads();  // previous batch done with the table
-    for (int cc = tid; cc < nc_b; cc += 256) {
+    for (int cc = tid; cc < nc_b; cc += BS) {
       int col = c0 + cc;
       int x = wrapi(lox + col / leny, P.nc[0]), y = wrapi(loy + col % leny, P.nc[1]);
       int base = (x * P.nc[1] + y) * P.nc[2];
@@ -918,15 +923,15 @@ __global__ __launch_bounds__(256) void k_spread_mfma(DevParams P, GridDims g,
     }
     __syncthreads();
     const int total = roff[nr];
-    // wavefront wv takes candidates 64 (4 r + wv) + lane, r = 0, 1, ...; the
-    // next candidate's atom is fetched while the current one is processed
+    // wavefront wv takes candidates 64 (WPB r + wv) + lane, r = 0, 1, ...;
+    // the next candidate's atom is fetched while the current one is processed
     int k = wv * 64 + lane, lo = 0;
     int inext = spread_index(roff, rbeg, nr, k, total, lo);
     double4 nxt = inext >= 0 ? spos[inext] : make_double4(0.0, 0.0, 0.0, 0.0);
-    for (; k - lane < total; k += 256) {
+    for (; k - lane < total; k += BS) {
       const double4 at = nxt;
       const int icur = inext;
-      inext = spread_index(roff, rbeg, nr, k + 256, total, lo);
+      inext = spread_index(roff, rbeg, nr, k + BS, total, lo);
       nxt = inext >= 0 ? spos[inext] : make_double4(0.0, 0.0, 0.0, 0.0);
       double t[3];
       int rlo[3];
@@ -953,18 +958,20 @@ __global__ __launch_bounds__(256) void k_spread_mfma(DevParams P, GridDims g,
     spread_flush<NU>(P, mine, lane % kSFlush < npend, (npend + 3) & ~3, dims, org, RX, RY, RZ,
                      lane, acc);
   }
-  // ---- sum the four wavefronts' tiles in a fixed order and store
-  __syncthreads();
+  // ---- sum the wavefronts' tiles in a fixed order and store
+  if constexpr (WPB > 1) {
+    __syncthreads();
 #pragma unroll
-  for (int i = 0; i < 4; ++i) WX[(wv * 4 + i) * 64 + lane] = acc[i];
-  __syncthreads();
+    for (int i = 0; i < 4; ++i) WX[(wv * 4 + i) * 64 + lane] = acc[i];
+    __syncthreads();
+  }
   if (wv == 0) {
     // f64 16x16x4 result layout: register i of lane l holds row 4 i + l / 16,
     // column l % 16
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-      double v = WX[i * 64 + lane];
-      for (int w = 1; w < 4; ++w) v += WX[(w * 4 + i) * 64 + lane];
+      double v = acc[i];
+      for (int w = 1; w < WPB; ++w) v += WX[(w * 4 + i) * 64 + lane];
       const int x = gx0 + 4 * i + (lane >> 4), col = lane & 15;
       const int y = gy0 + (col >> 2), z = gz0 + (col & 3);
       if (x < g.x && y < g.y && z < g.z) q1[((long)x * g.y + y) * g.z + z] = v;
@@ -1862,9 +1869,15 @@ void launch_pair_count(const DevParams &P, const double4 *spos, const int *scell
 void launch_spread(const DevParams &P, GridDims g, const double4 *spos, const int *start,
                    double *q1, hipStream_t s) {
   unsigned nb = (unsigned)(((g.x + kSX - 1) / kSX) * ((g.y + kSY - 1) / kSY) * ((g.z + kSZ - 1) / kSZ));
-#define CALL(NU) k_spread_mfma<NU><<<nb, 256, 0, s>>>(P, g, spos, start, q1)
-  MSM_NU_SWITCH(P.nu, CALL)
+  if (nb >= kSpreadFine) {
+#define CALL(NU) k_spread_mfma<NU, 1><<<nb, 64, 0, s>>>(P, g, spos, start, q1)
+    MSM_NU_SWITCH(P.nu, CALL)
 #undef CALL
+  } else {
+#define CALL(NU) k_spread_mfma<NU, 4><<<nb, 256, 0, s>>>(P, g, spos, start, q1)
+    MSM_NU_SWITCH(P.nu, CALL)
+#undef CALL
+  }
 }
 
 void launch_restrict(const DevParams &P, GridDims fine, GridDims coarse, const double *qf,
