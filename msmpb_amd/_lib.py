@@ -18,6 +18,8 @@ _ip = ctypes.POINTER(ctypes.c_int)
 MSM_PREC_FP64, MSM_PREC_MIXED = 0, 1
 MSM_LAYOUT_FIXED, MSM_LAYOUT_ASIS = 0, 1
 MSM_METHOD_MSM, MSM_METHOD_PME = 0, 1
+# return codes of include/msmpb_hip.h
+MSM_OK, MSM_ERR_ARG, MSM_ERR_STATE, MSM_ERR_HIP, MSM_ERR_ALLOC = 0, -1, -2, -3, -4
 
 
 class FFStruct(ctypes.Structure):

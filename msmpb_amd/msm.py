@@ -48,6 +48,10 @@ class MSM:
     def __init__(self, positions, charges, edges, nu=-1, levels=-1, top_grid=None, cutoff=-1.0,
                  est_err=-1.0, precision="fp64", layout="fixed", stage_timing=False,
                  method="msm", tol_dir=-1.0):
+        for name, val, table in (("precision", precision, PRECISIONS), ("layout", layout, LAYOUTS),
+                                 ("method", method, METHODS)):
+            if val not in table:
+                raise ValueError(f"{name} must be one of {sorted(table)} (got {val!r})")
         L = _lib.lib()
         self._q = np.ascontiguousarray(charges, dtype=np.float64).copy()
         self.N = int(self._q.shape[0])
@@ -73,8 +77,8 @@ class MSM:
         self.precision = precision
         self.method = method
         L.FF_build(self._ff, self.N, _p(pos), _p(self.edges), 0.0)
+        err = _lib.last_error()  # the build's own message, before any other call
         if L.FF_get_maxLevel(self._ff) <= 0 or not self._built():
-            err = _lib.last_error()
             self.close()
             raise MSMError(f"FF_build failed: {err}")
         ff = self._ff.contents
