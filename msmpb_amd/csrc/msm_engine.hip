@@ -868,6 +868,9 @@ int build_impl(FF *ff, int N, double (*position)[3], double edges[3][3]) {
   msm::Plan plan;
   std::string err;
   if (!ff->FFT) return fail(MSM_ERR_ARG, "FF_build: only the FFT top level is supported");
+  if (x->method == MSM_METHOD_PME && x->layout == MSM_LAYOUT_ASIS)
+    return fail(MSM_ERR_ARG, "FF_build: the as-is index layout reproduces the MSM reference's "
+                             "quirks and does not apply to the PME method");
   auto t0 = std::chrono::steady_clock::now();
   if (!msm::build_plan(r, plan, err)) return fail(MSM_ERR_ARG, "FF_build: " + err);
   double tstencil =
@@ -1167,6 +1170,8 @@ int FF_set_layout_compat(FF *ff, int layout) {
   if (!ff) return fail(MSM_ERR_ARG, "FF is NULL");
   if (layout != MSM_LAYOUT_FIXED && layout != MSM_LAYOUT_ASIS)
     return fail(MSM_ERR_ARG, "layout must be MSM_LAYOUT_FIXED or MSM_LAYOUT_ASIS");
+  if (layout == MSM_LAYOUT_ASIS && X(ff)->method == MSM_METHOD_PME)
+    return fail(MSM_ERR_ARG, "the as-is index layout does not apply to the PME method");
   X(ff)->layout = layout;
   return MSM_OK;
 }

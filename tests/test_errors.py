@@ -31,6 +31,7 @@ def water():
     (dict(method="pme", nu=6, top_grid=4, cutoff=8.0), "PME grid dimension 4 is below the order 6"),
     (dict(method="pme", nu=6, top_grid=16, cutoff=8.0, tol_dir=0.2), "tolDir * cutoff must be below 1"),
     (dict(method="pme", nu=7, top_grid=16, cutoff=8.0), "orderAcc must be 4, 6, 8 or 10 (got 7)"),
+    (dict(method="pme", nu=6, top_grid=16, cutoff=6.0, layout="asis"), "does not apply to the PME method"),
 ])
 def test_build_rejects_parameters_with_reason(kw, msg, water):
     s = water
@@ -69,6 +70,10 @@ def test_c_abi_failures_leave_the_ff_unbuilt(water):
         assert L.FF_set_method(ff, 9, 0.0) == _lib.MSM_ERR_ARG
         assert "method" in _lib.last_error()
         assert L.FF_set_layout_compat(ff, 3) == _lib.MSM_ERR_ARG
+        assert L.FF_set_method(ff, _lib.MSM_METHOD_PME, 0.0) == _lib.MSM_OK
+        assert L.FF_set_layout_compat(ff, _lib.MSM_LAYOUT_ASIS) == _lib.MSM_ERR_ARG
+        assert "PME" in _lib.last_error()
+        assert L.FF_set_method(ff, _lib.MSM_METHOD_MSM, 0.0) == _lib.MSM_OK
         L.FF_build(ff, len(q) + 1, pos.ctypes.data_as(dp), edges.ctypes.data_as(dp), 0.0)
         assert "N differs" in _lib.last_error()
         L.FF_build(ff, len(q), None, edges.ctypes.data_as(dp), 0.0)
