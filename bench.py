@@ -29,6 +29,7 @@ PEAK = {"hbm": 8000.0, "fp64": 78.6, "fp32": 157.3, "bf16": 2500.0}  # GB/s, TFL
 PME_GRID = 96  # PME comparator grid for water96k: spacing 1.03 A, order 6
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "water96k_hbm_traffic.txt")
 SQ_FILE = os.path.join(ROOT, "profiles", "water96k_kernel_stats.md")
+PME_SQ_FILE = os.path.join(ROOT, "profiles", "water96k_pme_kernel_stats.md")
 # VALU issue rate of gfx950: a wave64 VALU instruction occupies one of a
 # CU's four SIMDs for 4 cycles; 256 CUs at 2.4 GHz
 VALU_WAVE_INSTS_PER_S = 256 * 4 * 2.4e9 / 4
@@ -94,13 +95,13 @@ def traffic_of(kernel):
     return None
 
 
-def valu_insts_of(kernel):
-    """SQ_INSTS_VALU per dispatch of `kernel` from the committed rocprofv3
+def valu_insts_of(kernel, path=SQ_FILE):
+    """SQ_INSTS_VALU per dispatch of `kernel` from a committed rocprofv3
     SQ-counter summary (None if absent)."""
-    if not os.path.exists(SQ_FILE):
+    if not os.path.exists(path):
         return None
     cols = None
-    for line in open(SQ_FILE):
+    for line in open(path):
         if line.startswith("| kernel | WAVES"):
             cols = [c.strip() for c in line.strip().strip("|").split("|")]
             continue
@@ -250,6 +251,19 @@ def roofline(stages, m, pairs, nu):
     return out
 
 
+def pme_cpu_baseline(sysm):
+    """The PME oracle (single-threaded C restatement of pme0.c/pme1.c) on one
+    evaluation of the same workload, pair count excluded."""
+    from oracle.oracle import PmeOracle
+    o = PmeOracle(sysm.pos, sysm.q, sysm.edges, nu=6, M=PME_GRID, a0=10.0)
+    t0 = time.process_time()
+    E, F, _, _ = o.energy(sysm.pos, want_phi=False)
+    t = time.process_time() - t0
+    return dict(value=1.0 / t, unit="evals/s", cores=1, kind="port",
+                sample=f"1 full PME evaluation of the same {sysm.N}-atom workload (energy+forces), "
+                       f"{t:.2f} s CPU, one core"), E, F
+
+
 def pme_comparator(args, sysm, F64=None):
     """Smooth PME (order 6, 96^3 grid, cutoff 10 A) on the same system."""
     kp = dict(nu=6, top_grid=PME_GRID, cutoff=10.0, method="pme")
@@ -261,6 +275,19 @@ def pme_comparator(args, sysm, F64=None):
            "stage_ms_mixed": {k: v * 1e3 for k, v in rp["mixed"]["stages"].items()}}
     if F64 is not None:
         out["rms_rel_force_diff_msm_fp64"] = rms_rel(F64, rp["fp64"]["F"])
+    # the dominant kernel (the Ewald pairs, mixed): VALU issue against the
+    # machine's issue rate; no FLOP roofline (erfc and exp dominate the term)
+    st = rp["mixed"]["stages"].get("partcl2partcl", 0.0)
+    vi = valu_insts_of("k_pair_cell<true, 0>", PME_SQ_FILE)
+    if vi is not None and st > 0:
+        out["valu"] = {"kernel": "k_pair_cell<true, 0>", "insts_per_launch": vi,
+                       "issue_frac": vi / VALU_WAVE_INSTS_PER_S / st, "launch_ms": st * 1e3,
+                       "source": "SQ_INSTS_VALU, profiles/water96k_pme_kernel_stats.md"}
+    if not args.no_cpu:
+        cb, Ec, Fc = pme_cpu_baseline(sysm)
+        out["cpu_baseline"] = cb
+        out["fp64_vs_cpu_rel_energy"] = abs(rp["fp64"]["E"] - Ec) / abs(Ec)
+        out["fp64_vs_cpu_rms_rel_force"] = rms_rel(rp["fp64"]["F"], Fc)
     for t in ("fp64", "mixed"):
         rp[t]["m"].close()
     return out
