@@ -2,8 +2,9 @@
 // (hkayabilisim/msmpb code/forcefield.h:70-112) and the device-pointer
 // extensions of msmpb_hip.h, on top of the HIP kernel chain.
 //
-// One FF owns one Engine: the resolved plan, its device buffers (allocated
-// once in FF_build, sized for N atoms) and a HIP stream.  FF_energy stages
+// One FF owns one Engine: the resolved plan and its device buffers
+// (allocated once in FF_build, sized for N atoms); the two library streams
+// of the device are shared.  FF_energy stages
 // host positions and charges into HBM, runs the chain and copies forces
 // back; the *_device entry points work on caller-owned device memory and
 // are stream-ordered.
@@ -84,9 +85,9 @@ struct Engine {
   msm::DevParams P{};
   msm::GridDims gd[msm::kMaxLevel + 1]{}, sd[msm::kMaxLevel + 1]{};
   int device = 0;
-  hipStream_t stream = nullptr;    // the library's streams of this device (shared, not owned)
-  hipStream_t stream2 = nullptr;   // short-range pairs run here, beside the grid chain
-  hipStream_t stream3 = nullptr;   // the level-1 stencil pass, beside the coarser levels
+  hipStream_t stream = nullptr;    // the null stream: host-buffer entry points, FF_build
+  hipStream_t stream2 = nullptr;   // library stream (shared, not owned): the pairs, beside the chain
+  hipStream_t stream3 = nullptr;   // library stream: the level-1 stencil pass, beside the coarser levels
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   hipEvent_t ev_q1 = nullptr, ev_g1 = nullptr;  // q^1 ready; level-1 stencil pass done
   int nsten = 0;  // neighbour cell columns of the pair search
@@ -441,31 +442,35 @@ int rebuild_box(Engine &E) {
   return MSM_OK;
 }
 
-// Three streams per device, created once and shared by every FF of the
-// process: the library's own chain stream (host-buffer entry points), the
-// concurrent pair kernel and the level-1 stencil pass.  HIP maps streams onto
-// a few hardware queues (four by default); creating streams per FF would make
-// later FFs share a queue between concurrent stages, which serialises them.
-// FFs on one device are thereby ordered with each other.
-int device_streams(int dev, hipStream_t (&out)[3]) {
+// Two streams per device, created once and shared by every FF of the
+// process: the concurrent pair kernel and the level-1 stencil pass.  The
+// chain runs on the caller's stream (the device-pointer entry points) or on
+// HIP's null stream (the host-buffer entry points and FF_build's pair
+// count).  HIP maps streams onto a few hardware queues (four per process by
+// default): the caller's stream, the null stream that PyTorch also uses and
+// these two fit; a third library stream made a caller on its own stream
+// share a queue with a concurrent stage (0.36 instead of 0.30 ms per
+// water96k evaluation), and streams per FF would do the same.  FFs on one
+// device are thereby ordered with each other.
+int device_streams(int dev, hipStream_t (&out)[2]) {
   static std::mutex mu;
-  struct Trio {
-    hipStream_t s[3] = {nullptr, nullptr, nullptr};
+  struct Pair {
+    hipStream_t s[2] = {nullptr, nullptr};
   };
-  static std::vector<Trio> tab;
+  static std::vector<Pair> tab;
   std::lock_guard<std::mutex> lk(mu);
   if ((int)tab.size() <= dev) tab.resize(dev + 1);
-  Trio &t = tab[dev];
+  Pair &t = tab[dev];
   if (!t.s[0]) {
-    Trio n;
-    for (int k = 0; k < 3; ++k)
+    Pair n;
+    for (int k = 0; k < 2; ++k)
       if (hipStreamCreateWithFlags(&n.s[k], hipStreamNonBlocking) != hipSuccess) {
         for (int j = 0; j < k; ++j) hipStreamDestroy(n.s[j]);
         return fail(MSM_ERR_HIP, "hipStreamCreateWithFlags failed");
       }
     t = n;
   }
-  for (int k = 0; k < 3; ++k) out[k] = t.s[k];
+  for (int k = 0; k < 2; ++k) out[k] = t.s[k];
   return MSM_OK;
 }
 
@@ -509,9 +514,10 @@ int create_engine(FFX *x, const msm::Plan &plan) {
   x->eng = E;
   HIP_TRY(hipGetDevice(&E->device));
   {
-    hipStream_t st[3];
+    hipStream_t st[2];
     TRY(device_streams(E->device, st));
-    E->stream = st[0], E->stream2 = st[1], E->stream3 = st[2];
+    E->stream = nullptr;  // HIP's null stream: host-buffer entry points, FF_build
+    E->stream2 = st[0], E->stream3 = st[1];
   }
   for (hipEvent_t *e : {&E->ev_fork, &E->ev_join, &E->ev_q1, &E->ev_g1})
     HIP_TRY(hipEventCreateWithFlags(e, hipEventDisableTiming));

@@ -125,6 +125,40 @@ def test_device_path_matches_host(oracle_mod, torch_cuda):
     assert np.max(np.abs(phid.cpu().numpy() - phi)) <= 1e-13 * np.max(np.abs(phi))
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_caller_side_stream(precision, torch_cuda):
+    """A caller on its own (non-default) stream gets the same bits as on the
+    default stream, through msm_eval_device and the autograd pair, and the
+    results are ordered on its stream (read right after that stream's own
+    synchronisation only)."""
+    torch = torch_cuda
+    from msmpb_amd import MSM, msm_energy
+    from msmpb_amd.inputs import cfg1_crystal8192
+    s = cfg1_crystal8192()
+    p = s.params
+    m = MSM(s.pos, s.q, s.edges, nu=p["nu"], levels=p["levels"], top_grid=p["top_grid"],
+            cutoff=p["cutoff"], precision=precision)
+    try:
+        pos = torch.tensor(s.pos, device="cuda")
+        q = torch.tensor(s.q, device="cuda")
+        e0, F0, phi0 = m.eval_device(pos, q, forces=True, potential=True)
+        E0 = msm_energy(m, pos.clone().requires_grad_(True), q)
+        torch.cuda.synchronize()
+        side = torch.cuda.Stream()
+        with torch.cuda.stream(side):
+            for _ in range(3):  # several in flight on the side stream
+                e1, F1, phi1 = m.eval_device(pos, q, forces=True, potential=True)
+            pg = pos.clone().requires_grad_(True)
+            E1 = msm_energy(m, pg, q)
+            E1.backward()
+        side.synchronize()
+        assert torch.equal(e0, e1) and torch.equal(F0, F1) and torch.equal(phi0, phi1)
+        assert E1.item() == E0.item()
+        assert torch.equal(-pg.grad, F0)
+    finally:
+        m.close()
+
+
 @pytest.mark.parametrize("recompute", [False, True])
 def test_autograd_adjoint(recompute, oracle_mod, torch_cuda):
     torch = torch_cuda
