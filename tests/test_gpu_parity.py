@@ -159,6 +159,47 @@ def test_caller_side_stream(precision, torch_cuda):
         m.close()
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_graph_capture_replays_exact(precision, torch_cuda):
+    """msm_eval_device captured into a graph (torch.cuda.graph, stream
+    capture through the library's forked streams): every replay, with eager
+    evaluations and synchronisations in between, gives the eager result
+    bit for bit.  The evaluation records no memset, so nothing in the graph
+    depends on one."""
+    torch = torch_cuda
+    from msmpb_amd import MSM
+    from msmpb_amd.inputs import cfg0_water300
+    s = cfg0_water300()
+    p = s.params
+    m = MSM(s.pos, s.q, s.edges, nu=p["nu"], levels=p["levels"], top_grid=p["top_grid"],
+            cutoff=p["cutoff"], precision=precision)
+    try:
+        pos = torch.tensor(s.pos, device="cuda")
+        q = torch.tensor(s.q, device="cuda")
+        out = torch.empty(4, dtype=torch.float64, device="cuda")
+        _, F0, _ = m.eval_device(pos, q, out=out)
+        torch.cuda.synchronize()
+        E0, F0 = out.clone(), F0.clone()
+        side = torch.cuda.Stream()
+        with torch.cuda.stream(side):
+            for _ in range(3):
+                m.eval_device(pos, q, out=out)
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            _, Fg, _ = m.eval_device(pos, q, out=out)
+        for _ in range(5):
+            out.zero_()
+            Fg.zero_()
+            g.replay()
+            torch.cuda.synchronize()
+            assert torch.equal(out, E0) and torch.equal(Fg, F0)
+            m.eval_device(pos, q)  # eager work between replays
+            torch.cuda.synchronize()
+    finally:
+        m.close()
+
+
 @pytest.mark.parametrize("recompute", [False, True])
 def test_autograd_adjoint(recompute, oracle_mod, torch_cuda):
     torch = torch_cuda
