@@ -214,12 +214,19 @@ int dalloc(T **p, size_t n) {
 // k != 0 of an atom itself inside the cutoff.
 void pair_geometry(const msm::Plan &p, msm::DevParams &P, std::vector<int4> &cols) {
   const double a0 = p.aCut[0];
-  const double vol = p.A[0][0] * p.A[1][1] * p.A[2][2];
+  const bool tri = !p.ortho;
+  // face-to-face widths of the box: the diagonal, or 1 / |row a of A^-1|
+  double W[3];
+  for (int a = 0; a < 3; ++a)
+    W[a] = tri ? 1.0 / std::sqrt(p.Ai[a][0] * p.Ai[a][0] + p.Ai[a][1] * p.Ai[a][1] +
+                                 p.Ai[a][2] * p.Ai[a][2])
+               : p.A[a][a];
+  const double vol = tri ? p.detA : p.A[0][0] * p.A[1][1] * p.A[2][2];
   const double dens = p.N / vol, w3 = a0 / 3.0;
   const double div = dens * w3 * w3 * w3 >= 2.0 ? 3.0 : 2.0;
   int lim[3];
   for (int a = 0; a < 3; ++a) {
-    double L = p.A[a][a];
+    double L = W[a];
     int n = (int)std::floor(L / (a0 / div));
     if (n < 1) n = 1;
     if (n > 256) n = 256;
@@ -228,6 +235,58 @@ void pair_geometry(const msm::Plan &p, msm::DevParams &P, std::vector<int4> &col
     lim[a] = (int)std::ceil(a0 / P.cw[a]);
   }
   P.ncell = P.nc[0] * P.nc[1] * P.nc[2];
+  if (tri) {
+    // cells are parallelepipeds: offset d is kept where |A (d / nc)| minus
+    // the longest cell diagonal stays below a0 (the reference's neighbour-
+    // cell rule, forcefield0.c:61-107); for each (dx, dy) those dz form one
+    // interval (the distance is convex in dz)
+    auto vec = [&](double fx, double fy, double fz, double v[3]) {
+      for (int a = 0; a < 3; ++a) v[a] = p.A[a][0] * fx + p.A[a][1] * fy + p.A[a][2] * fz;
+    };
+    double diam = 0.0, v[3];
+    for (int sx = -1; sx <= 1; sx += 2)
+      for (int sy = -1; sy <= 1; sy += 2) {
+        vec((double)sx / P.nc[0], (double)sy / P.nc[1], 1.0 / P.nc[2], v);
+        diam = std::max(diam, std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]));
+      }
+    for (int a = 0; a < 3; ++a) lim[a] = (int)std::ceil((a0 + diam) / P.cw[a]);
+    cols.clear();
+    for (int dx = -lim[0]; dx <= lim[0]; ++dx)
+      for (int dy = -lim[1]; dy <= lim[1]; ++dy) {
+        int zlo = 1, zhi = 0;
+        for (int dz = -lim[2]; dz <= lim[2]; ++dz) {
+          vec((double)dx / P.nc[0], (double)dy / P.nc[1], (double)dz / P.nc[2], v);
+          if (std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]) - diam < a0) {
+            if (zlo > zhi) zlo = dz;
+            zhi = dz;
+          }
+        }
+        for (int z0 = zlo; z0 <= zhi; z0 += P.nc[2])
+          cols.push_back(make_int4(dx, dy, z0, std::min(zhi, z0 + P.nc[2] - 1)));
+      }
+    // g over the self images: lattice vectors A k inside the cutoff
+    double C = 0.0;
+    int K[3];
+    for (int a = 0; a < 3; ++a) K[a] = (int)std::ceil(a0 / W[a]);
+    for (int kx = -K[0]; kx <= K[0]; ++kx)
+      for (int ky = -K[1]; ky <= K[1]; ++ky)
+        for (int kz = -K[2]; kz <= K[2]; ++kz) {
+          if (!kx && !ky && !kz) continue;
+          vec(kx, ky, kz, v);
+          const double r2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+          if (r2 >= a0 * a0) continue;
+          if (p.method == msm::kPme) {
+            C += std::erfc(p.beta * std::sqrt(r2)) / std::sqrt(r2);
+            continue;
+          }
+          const double sv = r2 / (a0 * a0) - 1.0;
+          double ts = p.tau[p.nu - 1];
+          for (int k = p.nu - 2; k >= 0; --k) ts = p.tau[k] + sv * ts;
+          C += 1.0 / std::sqrt(r2) - ts / a0;
+        }
+    P.self_img = C;
+    return;
+  }
   auto gap2 = [&](int d, int a) {
     double g = (std::abs(d) - 1) * P.cw[a];
     return g > 0 ? g * g : 0.0;
@@ -270,6 +329,9 @@ void fill_params(const msm::Plan &p, int N, msm::DevParams &P) {
     P.box[a] = p.A[a][a];
     P.ibox[a] = p.Ai[a][a];
   }
+  P.tri = !p.ortho;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) P.A[3 * i + j] = p.A[i][j], P.Ai[3 * i + j] = p.Ai[i][j];
   double a0 = p.aCut[0];
   P.a0 = a0;
   P.a02 = a0 * a0;

@@ -36,6 +36,10 @@ struct DevParams {
   int ewald;
   double beta, beta2, ewc;
   float fbeta, fbeta2, fewc;
+  // triclinic box (tri != 0): A and A^-1 row-major; box/ibox/cw above then
+  // hold the face-to-face widths, and cells are slabs in fractional space
+  int tri;
+  double A[9], Ai[9];
 };
 
 struct GridDims {

@@ -93,6 +93,58 @@ __global__ void k_fill_dbl(double *p, long n, double v) {
 // ---------------------------------------------------------------- binning
 constexpr int kBinBS = 256;
 
+// Fractional coordinates A^-1 r: the diagonal for orthorhombic boxes, the
+// full inverse for triclinic ones (TRI: the box shape, fixed per kernel
+// instance so that the orthorhombic kernels carry no triclinic code).
+template <bool TRI>
+__device__ __forceinline__ void to_frac(const DevParams &P, const double r[3], double s[3]) {
+  if constexpr (TRI) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) s[a] = P.Ai[3 * a] * r[0] + P.Ai[3 * a + 1] * r[1] + P.Ai[3 * a + 2] * r[2];
+  } else {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) s[a] = r[a] * P.ibox[a];
+  }
+}
+
+// Field from the fractional gradient: E = -A^-T (M * grad_t).
+template <bool TRI>
+__device__ __forceinline__ void frac_field(const DevParams &P, GridDims g, double ex, double ey,
+                                           double ez, double &Ex, double &Ey, double &Ez) {
+  const double fx = ex * g.x, fy = ey * g.y, fz = ez * g.z;
+  if constexpr (TRI) {
+    Ex = -(P.Ai[0] * fx + P.Ai[3] * fy + P.Ai[6] * fz);
+    Ey = -(P.Ai[1] * fx + P.Ai[4] * fy + P.Ai[7] * fz);
+    Ez = -(P.Ai[2] * fx + P.Ai[5] * fy + P.Ai[8] * fz);
+  } else {
+    Ex = -(fx * P.ibox[0]), Ey = -(fy * P.ibox[1]), Ez = -(fz * P.ibox[2]);
+  }
+}
+
+// Lattice vector A k (Cartesian shift of the periodic image k).
+template <bool TRI>
+__device__ __forceinline__ void lattice(const DevParams &P, int kx, int ky, int kz, double &x,
+                                        double &y, double &z) {
+  if constexpr (TRI) {
+    x = P.A[0] * kx + P.A[1] * ky + P.A[2] * kz;
+    y = P.A[3] * kx + P.A[4] * ky + P.A[5] * kz;
+    z = P.A[6] * kx + P.A[7] * ky + P.A[8] * kz;
+  } else {
+    x = kx * P.box[0], y = ky * P.box[1], z = kz * P.box[2];
+  }
+}
+
+// The same decided at run time (binning and pair count: not hot).
+__device__ __forceinline__ void to_frac_rt(const DevParams &P, const double r[3], double s[3]) {
+  if (P.tri) to_frac<true>(P, r, s);
+  else to_frac<false>(P, r, s);
+}
+__device__ __forceinline__ void lattice_rt(const DevParams &P, int kx, int ky, int kz, double &x,
+                                           double &y, double &z) {
+  if (P.tri) lattice<true>(P, kx, ky, kz, x, y, z);
+  else lattice<false>(P, kx, ky, kz, x, y, z);
+}
+
 __global__ __launch_bounds__(kBinBS) void k_bin(DevParams P, const double *__restrict__ pos,
                                                 const double *__restrict__ q,
                                                 int *__restrict__ cell_of, int *__restrict__ rank,
@@ -104,22 +156,33 @@ __global__ __launch_bounds__(kBinBS) void k_bin(DevParams P, const double *__res
     double r[3] = {pos[3 * (long)i], pos[3 * (long)i + 1], pos[3 * (long)i + 2]};
     qi = q[i];
     int c[3];
-    double rw[3];
+    double rw[3], sf[3], fl[3];
     bool ok = isfinite(qi);
+    to_frac_rt(P, r, sf);
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-      double s = r[a] * P.ibox[a];
+      double s = sf[a];
       double f = floor(s);
       double sw = s - f;
+      fl[a] = f;
       ok = ok && isfinite(s);
       // s - floor(s) may round up to 1 for tiny negative s; keep in range
       int ci = (int)(sw * P.nc[a]);
       if (!(sw >= 0.0) || ci < 0) ci = 0;
       if (ci >= P.nc[a]) ci = P.nc[a] - 1;
       c[a] = ci;
-      rw[a] = r[a] - f * P.box[a];
-      if (!isfinite(rw[a])) rw[a] = 0.0;
     }
+    if (P.tri) {  // r - A floor(A^-1 r)
+      rw[0] = r[0] - (P.A[0] * fl[0] + P.A[1] * fl[1] + P.A[2] * fl[2]);
+      rw[1] = r[1] - (P.A[3] * fl[0] + P.A[4] * fl[1] + P.A[5] * fl[2]);
+      rw[2] = r[2] - (P.A[6] * fl[0] + P.A[7] * fl[1] + P.A[8] * fl[2]);
+    } else {
+#pragma unroll
+      for (int a = 0; a < 3; ++a) rw[a] = r[a] - fl[a] * P.box[a];
+    }
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+      if (!isfinite(rw[a])) rw[a] = 0.0;
     if (!ok) atomicAdd(flags, 1);
     int cell = (c[0] * P.nc[1] + c[1]) * P.nc[2] + c[2];
     cell_of[i] = cell;
@@ -495,7 +558,7 @@ __device__ __forceinline__ double wave_sum_scatter(typename PairPrec<MIXED>::R (
   return r;
 }
 
-template <bool MIXED, int NU>
+template <bool MIXED, int NU, bool TRI>
 __global__ __launch_bounds__(64) void k_pair_cell(DevParams P, const double4 *__restrict__ spos,
                                                   const int *__restrict__ sidx,
                                                   const int *__restrict__ start,
@@ -513,7 +576,15 @@ __global__ __launch_bounds__(64) void k_pair_cell(DevParams P, const double4 *__
   const int b0 = start[c], n = start[c + 1] - b0;
   const int cz = c % P.nc[2], cy = (c / P.nc[2]) % P.nc[1], cx = c / (P.nc[1] * P.nc[2]);
   // home-cell origin: all coordinates below are relative to it
-  const double ox = cx * P.cw[0], oy = cy * P.cw[1], oz = cz * P.cw[2];
+  double ox, oy, oz;
+  if constexpr (TRI) {
+    const double fx = (double)cx / P.nc[0], fy = (double)cy / P.nc[1], fz = (double)cz / P.nc[2];
+    ox = P.A[0] * fx + P.A[1] * fy + P.A[2] * fz;
+    oy = P.A[3] * fx + P.A[4] * fy + P.A[5] * fz;
+    oz = P.A[6] * fx + P.A[7] * fy + P.A[8] * fz;
+  } else {
+    ox = cx * P.cw[0], oy = cy * P.cw[1], oz = cz * P.cw[2];
+  }
   const R cut2 = MIXED ? (R)P.fcut2 : (R)P.cut2;
   const float fcw0 = (float)P.cw[0], fcw1 = (float)P.cw[1], ficw2 = (float)(1.0 / P.cw[2]);
   const float fcut2 = P.fcut2;
@@ -556,16 +627,23 @@ __global__ __launch_bounds__(64) void k_pair_cell(DevParams P, const double4 *__
           }
           // lane k: column c0 + k -> at most two contiguous atom ranges
           int beg0 = 0, cnt0 = 0, beg1 = 0, cnt1 = 0;
-          double sx = 0, sy = 0, sz0 = 0, sz1 = 0;
+          double sx = 0, sy = 0, sz0 = 0, sx1 = 0, sy1 = 0, sz1 = 0;
           if (c0 + lane < ncol) {
             const int4 col = cols[c0 + lane];  // dx, dy, dzlo, dzhi
-            const float gx = fmaxf(fmaxf(col.x * fcw0 - mrg - fhi0, flo0 - (col.x + 1) * fcw0 - mrg), 0.f);
-            const float gy = fmaxf(fmaxf(col.y * fcw1 - mrg - fhi1, flo1 - (col.y + 1) * fcw1 - mrg), 0.f);
-            const float rz2 = fcut2 - gx * gx - gy * gy;
+            // orthorhombic cells: trim the z range to the pass's bounding box
+            // (triclinic cells are not axis-aligned: the whole column is
+            // staged and the per-candidate bounding-box test decides)
+            int zl = col.z, zh = col.w;
+            float rz2 = 1.f;
+            if constexpr (!TRI) {
+              const float gx = fmaxf(fmaxf(col.x * fcw0 - mrg - fhi0, flo0 - (col.x + 1) * fcw0 - mrg), 0.f);
+              const float gy = fmaxf(fmaxf(col.y * fcw1 - mrg - fhi1, flo1 - (col.y + 1) * fcw1 - mrg), 0.f);
+              rz2 = fcut2 - gx * gx - gy * gy;
+              const float rz = sqrtf(fmaxf(rz2, 0.f)) + mrg;
+              zl = max(col.z, (int)floorf((flo2 - rz) * ficw2));
+              zh = min(col.w, (int)floorf((fhi2 + rz) * ficw2));
+            }
             if (rz2 > 0.f) {
-              const float rz = sqrtf(rz2) + mrg;
-              const int zl = max(col.z, (int)floorf((flo2 - rz) * ficw2));
-              const int zh = min(col.w, (int)floorf((fhi2 + rz) * ficw2));
               if (zl <= zh) {
                 int X = cx + col.x, Y = cy + col.y;
                 const int kx = floordiv(X, P.nc[0]), ky = floordiv(Y, P.nc[1]);
@@ -576,11 +654,13 @@ __global__ __launch_bounds__(64) void k_pair_cell(DevParams P, const double4 *__
                 const int e0 = min(Z1, (kz0 + 1) * P.nc[2] - 1);
                 beg0 = start[base + Z0 - kz0 * P.nc[2]];
                 cnt0 = start[base + e0 - kz0 * P.nc[2] + 1] - beg0;
-                sx = kx * P.box[0] - ox, sy = ky * P.box[1] - oy, sz0 = kz0 * P.box[2] - oz;
+                lattice<TRI>(P, kx, ky, kz0, sx, sy, sz0);
+                sx -= ox, sy -= oy, sz0 -= oz;
                 if (e0 < Z1) {  // the rest lies in the next periodic image along z
                   beg1 = start[base];
                   cnt1 = start[base + Z1 - (kz0 + 1) * P.nc[2] + 1] - beg1;
-                  sz1 = (kz0 + 1) * P.box[2] - oz;
+                  lattice<TRI>(P, kx, ky, kz0 + 1, sx1, sy1, sz1);
+                  sx1 -= ox, sy1 -= oy, sz1 -= oz;
                 }
               }
             }
@@ -602,7 +682,7 @@ __global__ __launch_bounds__(64) void k_pair_cell(DevParams P, const double4 *__
           }
           if (cnt1 > 0) {
             seg_off[k] = o + cnt0, seg_beg[k] = beg1;
-            seg_sx[k] = sx, seg_sy[k] = sy, seg_sz[k] = sz1;
+            seg_sx[k] = sx1, seg_sy[k] = sy1, seg_sz[k] = sz1;
           }
           total = last & 0xffffff;
           if (lane == 0) seg_off[last >> 24] = total;  // sentinel
@@ -696,7 +776,9 @@ __global__ __launch_bounds__(kPairBS) void k_pair_count(
         int ox = cx + col.x, oy = cy + col.y, oz = cz + dz;
         int kx = floordiv(ox, P.nc[0]), ky = floordiv(oy, P.nc[1]), kz = floordiv(oz, P.nc[2]);
         ox -= kx * P.nc[0], oy -= ky * P.nc[1], oz -= kz * P.nc[2];
-        double bx = kx * P.box[0] - ai.x, by = ky * P.box[1] - ai.y, bz = kz * P.box[2] - ai.z;
+        double bx, by, bz;
+        lattice_rt(P, kx, ky, kz, bx, by, bz);
+        bx -= ai.x, by -= ai.y, bz -= ai.z;
         int cn = (ox * P.nc[1] + oy) * P.nc[2] + oz;
         for (int j = start[cn]; j < start[cn + 1]; ++j) {
           if (j <= i) continue;
@@ -706,8 +788,10 @@ __global__ __launch_bounds__(kPairBS) void k_pair_count(
           // component in [-L/2, L/2), r - floor(r/L + 1/2) L as in the
           // reference (forcefield0.c:140-148), so that of two images tied at
           // exactly half a box only one is counted
-          if (floor(dx * P.ibox[0] + 0.5) != 0.0 || floor(dy * P.ibox[1] + 0.5) != 0.0 ||
-              floor(dz2 * P.ibox[2] + 0.5) != 0.0)
+          const double d[3] = {dx, dy, dz2};
+          double sd[3];
+          to_frac_rt(P, d, sd);
+          if (floor(sd[0] + 0.5) != 0.0 || floor(sd[1] + 0.5) != 0.0 || floor(sd[2] + 0.5) != 0.0)
             continue;
           if (dx * dx + dy * dy + dz2 * dz2 < P.a02) n += 1.0;
         }
@@ -766,15 +850,17 @@ __device__ __forceinline__ int spread_index(const int *roff, const int *rbeg, in
 // Fractional offsets t and the brick-relative index of the lowest support
 // point m - NU/2 + 1 per axis (m lies in [0, M] and org in [0, M), so one
 // wrap suffices, NU >= 4); reach: the support meets the brick.
-template <int NU>
+template <int NU, bool TRI>
 __device__ __forceinline__ bool spread_place(const DevParams &P, const double4 &at,
                                              const int dims[3], const int org[3],
                                              const int ext[3], double t[3], int rlo[3]) {
   const double r[3] = {at.x, at.y, at.z};
+  double sf[3];
+  to_frac<TRI>(P, r, sf);
   bool reach = true;
 #pragma unroll
   for (int ax = 0; ax < 3; ++ax) {
-    double sv = r[ax] * P.ibox[ax];
+    double sv = sf[ax];
     sv = sv - floor(sv);
     const double tt = dims[ax] * sv, m = floor(tt);
     t[ax] = tt - m;
@@ -812,7 +898,7 @@ __device__ __forceinline__ void spread_fold(const DevParams &P, double t, double
 // weights into the records -- lanes < kSFlush build the x record of their
 // atom, the other half the y (times q) and z records -- then
 // Q += WX^T (WY x WZ) by natoms / 4 MFMAs.
-template <int NU>
+template <int NU, bool TRI>
 __device__ __forceinline__ void spread_flush(const DevParams &P, const double4 &at, bool ok,
                                              int natoms, const int dims[3], const int org[3],
                                              double *RX, double *RY, double *RZ, int lane,
@@ -831,7 +917,7 @@ __device__ __forceinline__ void spread_flush(const DevParams &P, const double4 &
     if (ok) {
       double t[3];
       int rlo[3];
-      spread_place<NU>(P, at, dims, org, ext, t, rlo);
+      spread_place<NU, TRI>(P, at, dims, org, ext, t, rlo);
       if (lane < kSFlush) {
         spread_fold<NU, kSX>(P, t[0], 1.0, rlo[0], dims[0], rx);
       } else {
@@ -856,7 +942,7 @@ __device__ __forceinline__ void spread_flush(const DevParams &P, const double4 &
 // WPB wavefronts per brick: 4 where the grid has few bricks (they split the
 // brick's candidates), 1 on fine grids, where the bricks alone fill the GPU
 // and a wavefront needs no barrier partners and about 11 KB of LDS.
-template <int NU, int WPB>
+template <int NU, int WPB, bool TRI>
 __global__ __launch_bounds__(64 * WPB) void k_spread_mfma(DevParams P, GridDims g,
                                                           const double4 *__restrict__ spos,
                                                           const int *__restrict__ start,
@@ -936,7 +1022,7 @@ __global__ __launch_bounds__(64 * WPB) void k_spread_mfma(DevParams P, GridDims 
       nxt = inext >= 0 ? spos[inext] : make_double4(0.0, 0.0, 0.0, 0.0);
       double t[3];
       int rlo[3];
-      const bool reach = icur >= 0 && spread_place<NU>(P, at, dims, org, ext, t, rlo);
+      const bool reach = icur >= 0 && spread_place<NU, TRI>(P, at, dims, org, ext, t, rlo);
       const unsigned long long bm = __ballot(reach);
       if (reach) Qw[npend + __popcll(bm & below)] = at;
       npend += __popcll(bm);
@@ -945,7 +1031,7 @@ __global__ __launch_bounds__(64 * WPB) void k_spread_mfma(DevParams P, GridDims 
         const double4 mine = Qw[lane % kSFlush];
         const double4 rest = Qw[min(kSFlush + lane, kSQueue - 1)];
         __builtin_amdgcn_wave_barrier();
-        spread_flush<NU>(P, mine, true, kSFlush, dims, org, RX, RY, RZ, lane, acc);
+        spread_flush<NU, TRI>(P, mine, true, kSFlush, dims, org, RX, RY, RZ, lane, acc);
         // move the leftover queue entries [kSFlush, npend) to the front
         if (kSFlush + lane < npend) Qw[lane] = rest;
         npend -= kSFlush;
@@ -956,7 +1042,7 @@ __global__ __launch_bounds__(64 * WPB) void k_spread_mfma(DevParams P, GridDims 
   if (npend > 0) {  // the last, partial group of atoms
     __builtin_amdgcn_wave_barrier();
     const double4 mine = Qw[lane % kSFlush];
-    spread_flush<NU>(P, mine, lane % kSFlush < npend, (npend + 3) & ~3, dims, org, RX, RY, RZ,
+    spread_flush<NU, TRI>(P, mine, lane % kSFlush < npend, (npend + 3) & ~3, dims, org, RX, RY, RZ,
                      lane, acc);
   }
   // ---- sum the wavefronts' tiles in a fixed order and store
@@ -1489,7 +1575,7 @@ __global__ __launch_bounds__(kFinBS) void k_finalize(DevParams P, FinalizeArgs f
 
 // interpolate (forcefield1.c:570-607) + F += q E (:228-231), and the
 // potential phi_i = dE/dq_i: grid part plus the constant self terms.
-template <int NU, bool ADD>
+template <int NU, bool ADD, bool TRI>
 __device__ __forceinline__ void gather_atom(const DevParams &P, GridDims g,
                                             const double4 *__restrict__ spos,
                                             const int *__restrict__ sidx,
@@ -1500,11 +1586,12 @@ __device__ __forceinline__ void gather_atom(const DevParams &P, GridDims g,
   double4 a = spos[i];
   double r[3] = {a.x, a.y, a.z};
   int dims[3] = {g.x, g.y, g.z};
-  double t[3];
+  double t[3], sf[3];
   int m[3];
+  to_frac<TRI>(P, r, sf);
 #pragma unroll
   for (int ax = 0; ax < 3; ++ax) {
-    double tt = dims[ax] * (r[ax] * P.ibox[ax]), mm = floor(tt);
+    double tt = dims[ax] * sf[ax], mm = floor(tt);
     t[ax] = tt - mm;
     m[ax] = (int)mm;
   }
@@ -1541,8 +1628,8 @@ __device__ __forceinline__ void gather_atom(const DevParams &P, GridDims g,
     ey += wx * py;
     ez += wx * pz;
   }
-  // E = -A^{-T} (M * grad_t), orthorhombic A^{-T} = diag(1/box)
-  double Ex = -ex * g.x * P.ibox[0], Ey = -ey * g.y * P.ibox[1], Ez = -ez * g.z * P.ibox[2];
+  double Ex, Ey, Ez;  // E = -A^{-T} (M * grad_t)
+  frac_field<TRI>(P, g, ex, ey, ez, Ex, Ey, Ez);
   int oi = sidx[i];
   const double pv = ph + a.w * P.coeff_self - scal[0] * P.coeff2;
   if (F) {
@@ -1554,7 +1641,7 @@ __device__ __forceinline__ void gather_atom(const DevParams &P, GridDims g,
   if (phi) phi[oi] = (ADD ? phi[oi] : 0.0) + pv;
 }
 
-template <int NU, bool ADD>
+template <int NU, bool ADD, bool TRI>
 __global__ __launch_bounds__(256) void k_gather(DevParams P, GridDims g,
                                                 const double4 *__restrict__ spos,
                                                 const int *__restrict__ sidx,
@@ -1562,7 +1649,7 @@ __global__ __launch_bounds__(256) void k_gather(DevParams P, GridDims g,
                                                 const double *__restrict__ scal,
                                                 double *__restrict__ F, double *__restrict__ phi) {
   int i = blockIdx.x * 256 + threadIdx.x;
-  if (i < P.N) gather_atom<NU, ADD>(P, g, spos, sidx, e1, scal, F, phi, i);
+  if (i < P.N) gather_atom<NU, ADD, TRI>(P, g, spos, sidx, e1, scal, F, phi, i);
 }
 
 // The same interpolation from an LDS copy of the level-1 grid around a tile
@@ -1587,7 +1674,8 @@ TileGeom gather_tiles(const DevParams &P, GridDims g) {
     int R[3];
     long vol = 1;
     for (int a = 0; a < 3; ++a) {
-      R[a] = (int)std::ceil(tb * P.cw[a] * dims[a] / P.box[a]) + P.nu + 4;
+      R[a] = (P.tri ? (int)std::ceil((double)tb * dims[a] / P.nc[a])
+                    : (int)std::ceil(tb * P.cw[a] * dims[a] / P.box[a])) + P.nu + 4;
       vol *= R[a];
     }
     if (vol * (long)sizeof(double) > 48 * 1024 || R[2] > 64) continue;
@@ -1602,7 +1690,7 @@ TileGeom gather_tiles(const DevParams &P, GridDims g) {
   return tg;
 }
 
-template <int NU, bool ADD>
+template <int NU, bool ADD, bool TRI>
 __device__ __forceinline__ void gather_atom_tile(const DevParams &P, GridDims g,
                                                  const double *reg, int Ry, int Rz,
                                                  const int (&o)[3],
@@ -1614,11 +1702,12 @@ __device__ __forceinline__ void gather_atom_tile(const DevParams &P, GridDims g,
   const double4 a = spos[i];
   const double r[3] = {a.x, a.y, a.z};
   const int dims[3] = {g.x, g.y, g.z};
-  double t[3];
+  double t[3], sf[3];
   int m[3];
+  to_frac<TRI>(P, r, sf);
 #pragma unroll
   for (int ax = 0; ax < 3; ++ax) {
-    const double tt = dims[ax] * (r[ax] * P.ibox[ax]), mm = floor(tt);
+    const double tt = dims[ax] * sf[ax], mm = floor(tt);
     t[ax] = tt - mm;
     m[ax] = (int)mm - o[ax] + NU / 2;  // local index of weight 0's grid point
   }
@@ -1653,7 +1742,8 @@ __device__ __forceinline__ void gather_atom_tile(const DevParams &P, GridDims g,
 #pragma unroll
     for (int k = 0; k + 1 < NU; ++k) wx[k] = wx[k + 1], dwx[k] = dwx[k + 1];  // next x weight first
   }
-  const double Ex = -ex * g.x * P.ibox[0], Ey = -ey * g.y * P.ibox[1], Ez = -ez * g.z * P.ibox[2];
+  double Ex, Ey, Ez;  // E = -A^{-T} (M * grad_t)
+  frac_field<TRI>(P, g, ex, ey, ez, Ex, Ey, Ez);
   const int oi = sidx[i];
   const double pv = ph + a.w * P.coeff_self - scal[0] * P.coeff2;
   if (F) {
@@ -1665,7 +1755,7 @@ __device__ __forceinline__ void gather_atom_tile(const DevParams &P, GridDims g,
   if (phi) phi[oi] = (ADD ? phi[oi] : 0.0) + pv;
 }
 
-template <int NU, bool ADD>
+template <int NU, bool ADD, bool TRI>
 __global__ __launch_bounds__(256) void k_gather_tile(DevParams P, GridDims g, TileGeom tg,
                                                      const double4 *__restrict__ spos,
                                                      const int *__restrict__ sidx,
@@ -1686,7 +1776,8 @@ __global__ __launch_bounds__(256) void k_gather_tile(DevParams P, GridDims g, Ti
   int o[3];
 #pragma unroll
   for (int a = 0; a < 3; ++a)
-    o[a] = (int)floor(c0[a] * P.cw[a] * (dims[a] * P.ibox[a])) - NU / 2 - 1;
+    o[a] = (TRI ? (int)floor((double)c0[a] * dims[a] / P.nc[a])
+                : (int)floor(c0[a] * P.cw[a] * (dims[a] * P.ibox[a]))) - NU / 2 - 1;
   const int Ry = tg.R[1], Rz = tg.R[2], nreg = tg.R[0] * Ry * Rz;
   {  // eight loads in flight per thread (unconditional: clamped index); the
      // divisions and periodic wraps by FP32 reciprocals, exact for these
@@ -1734,7 +1825,7 @@ __global__ __launch_bounds__(256) void k_gather_tile(DevParams P, GridDims g, Ti
   int sl = 0;
   for (int k = t; k < total; k += 256) {
     while (seg_off[sl + 1] <= k) ++sl;
-    gather_atom_tile<NU, ADD>(P, g, reg, Ry, Rz, o, spos, sidx, scal, F, phi,
+    gather_atom_tile<NU, ADD, TRI>(P, g, reg, Ry, Rz, o, spos, sidx, scal, F, phi,
                               seg_beg[sl] + k - seg_off[sl]);
   }
 }
@@ -1839,8 +1930,11 @@ int pair_blocks(int N) { return (int)nblk(N, kPairBS); }
 void launch_pair_fp64(const DevParams &P, const double4 *spos, const int *sidx, const int *start,
                       const int4 *cols, int ncol, double *F, double *phi, double *epart,
                       hipStream_t s) {
-#define CALL(NU) \
-  k_pair_cell<false, NU><<<P.ncell, 64, 0, s>>>(P, spos, sidx, start, cols, ncol, F, phi, epart)
+#define CALL(NU)                                                                            \
+  if (P.tri)                                                                                \
+    k_pair_cell<false, NU, true><<<P.ncell, 64, 0, s>>>(P, spos, sidx, start, cols, ncol, F, phi, epart); \
+  else                                                                                      \
+    k_pair_cell<false, NU, false><<<P.ncell, 64, 0, s>>>(P, spos, sidx, start, cols, ncol, F, phi, epart)
   if (P.ewald) {
     CALL(0);
   } else {
@@ -1852,8 +1946,11 @@ void launch_pair_fp64(const DevParams &P, const double4 *spos, const int *sidx, 
 void launch_pair_fp32(const DevParams &P, const double4 *spos, const int *sidx, const int *start,
                       const int4 *cols, int ncol, double *F, double *phi, double *epart,
                       hipStream_t s) {
-#define CALL(NU) \
-  k_pair_cell<true, NU><<<P.ncell, 64, 0, s>>>(P, spos, sidx, start, cols, ncol, F, phi, epart)
+#define CALL(NU)                                                                            \
+  if (P.tri)                                                                                \
+    k_pair_cell<true, NU, true><<<P.ncell, 64, 0, s>>>(P, spos, sidx, start, cols, ncol, F, phi, epart); \
+  else                                                                                      \
+    k_pair_cell<true, NU, false><<<P.ncell, 64, 0, s>>>(P, spos, sidx, start, cols, ncol, F, phi, epart)
   if (P.ewald) {
     CALL(0);
   } else {
@@ -1871,11 +1968,15 @@ void launch_spread(const DevParams &P, GridDims g, const double4 *spos, const in
                    double *q1, hipStream_t s) {
   unsigned nb = (unsigned)(((g.x + kSX - 1) / kSX) * ((g.y + kSY - 1) / kSY) * ((g.z + kSZ - 1) / kSZ));
   if (nb >= kSpreadFine) {
-#define CALL(NU) k_spread_mfma<NU, 1><<<nb, 64, 0, s>>>(P, g, spos, start, q1)
+#define CALL(NU)                                                    \
+  if (P.tri) k_spread_mfma<NU, 1, true><<<nb, 64, 0, s>>>(P, g, spos, start, q1); \
+  else k_spread_mfma<NU, 1, false><<<nb, 64, 0, s>>>(P, g, spos, start, q1)
     MSM_NU_SWITCH(P.nu, CALL)
 #undef CALL
   } else {
-#define CALL(NU) k_spread_mfma<NU, 4><<<nb, 256, 0, s>>>(P, g, spos, start, q1)
+#define CALL(NU)                                                      \
+  if (P.tri) k_spread_mfma<NU, 4, true><<<nb, 256, 0, s>>>(P, g, spos, start, q1); \
+  else k_spread_mfma<NU, 4, false><<<nb, 256, 0, s>>>(P, g, spos, start, q1)
     MSM_NU_SWITCH(P.nu, CALL)
 #undef CALL
   }
@@ -1973,20 +2074,30 @@ void launch_gather(const DevParams &P, GridDims g, const double4 *spos, const in
   if (tg.tb > 0) {
     const unsigned nb = (unsigned)(tg.ntile[0] * tg.ntile[1] * tg.ntile[2]);
     const size_t lds = sizeof(double) * tg.R[0] * tg.R[1] * tg.R[2];
-#define CALL(NU)                                                                              \
-  if (add)                                                                                    \
-    k_gather_tile<NU, true><<<nb, 256, lds, s>>>(P, g, tg, spos, sidx, start, e1, scal, F, phi); \
-  else                                                                                        \
-    k_gather_tile<NU, false><<<nb, 256, lds, s>>>(P, g, tg, spos, sidx, start, e1, scal, F, phi)
+#define GT(NU, ADD, TRI) \
+  k_gather_tile<NU, ADD, TRI><<<nb, 256, lds, s>>>(P, g, tg, spos, sidx, start, e1, scal, F, phi)
+#define CALL(NU)                                         \
+  if (add) {                                             \
+    if (P.tri) GT(NU, true, true); else GT(NU, true, false);   \
+  } else {                                               \
+    if (P.tri) GT(NU, false, true); else GT(NU, false, false); \
+  }
     MSM_NU_SWITCH(P.nu, CALL)
 #undef CALL
+#undef GT
     return;
   }
-#define CALL(NU)                                                                        \
-  if (add) k_gather<NU, true><<<nblk(P.N, 256), 256, 0, s>>>(P, g, spos, sidx, e1, scal, F, phi); \
-  else k_gather<NU, false><<<nblk(P.N, 256), 256, 0, s>>>(P, g, spos, sidx, e1, scal, F, phi)
+#define GA(NU, ADD, TRI) \
+  k_gather<NU, ADD, TRI><<<nblk(P.N, 256), 256, 0, s>>>(P, g, spos, sidx, e1, scal, F, phi)
+#define CALL(NU)                                         \
+  if (add) {                                             \
+    if (P.tri) GA(NU, true, true); else GA(NU, true, false);   \
+  } else {                                               \
+    if (P.tri) GA(NU, false, true); else GA(NU, false, false); \
+  }
   MSM_NU_SWITCH(P.nu, CALL)
 #undef CALL
+#undef GA
 }
 
 int combine_blocks(int N) { return (int)nblk(N, 256); }

@@ -294,6 +294,30 @@ bool box_dependent(Plan &p, std::string &err, bool stencils = true) {
   return true;
 }
 
+// Box shape: orthorhombic (diagonal A) or triclinic.  A triclinic box needs
+// the cutoff below half its smallest face-to-face width 1 / |row a of A^-1|:
+// there the reference's pair rule (the image nearest in fractional
+// coordinates, neighborlist and partcl2partcl of forcefield0.c and
+// forcefield1.c) and the sum over every image inside the cutoff coincide.
+bool check_box(Plan &p, std::string &err) {
+  p.ortho = true;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j)
+      if (i != j && p.A[i][j] != 0.0) p.ortho = false;
+  if (p.ortho) return true;
+  double wmin = INFINITY;
+  for (int a = 0; a < 3; ++a) wmin = std::fmin(wmin, 1.0 / row_norm(p.Ai, a));
+  if (!(p.aCut[0] < 0.5 * wmin)) {
+    char buf[256];
+    std::snprintf(buf, sizeof buf,
+                  "triclinic box: the cutoff %g must be below half the smallest face-to-face "
+                  "width (%g)", p.aCut[0], wmin);
+    err = buf;
+    return false;
+  }
+  return true;
+}
+
 // ---- PME (code/pme0.c) ----------------------------------------------------
 // Ewald splitting: erfc(b) = c for b = beta a0, by Newton's method from
 // b = 0 until the residual stops shrinking (pme0.c:209-224).
@@ -452,14 +476,7 @@ bool build_plan(const BuildRequest &r, Plan &p, std::string &err) {
   std::memcpy(p.A, r.edges, sizeof p.A);
   p.detA = invert3(p.A, p.Ai);
   if (!(p.detA > 0) || !std::isfinite(p.detA)) { err = "box matrix is singular"; return false; }
-  for (int i = 0; i < 3; ++i)
-    for (int j = 0; j < 3; ++j)
-      if (i != j && p.A[i][j] != 0.0) p.ortho = false;
-  if (!p.ortho) {
-    err = "only orthorhombic boxes (diagonal edge matrix) are supported";
-    return false;
-  }
-  if (r.method == kPme) return build_pme(r, p, err);
+  if (r.method == kPme) return build_pme(r, p, err) && check_box(p, err);
   p.estErr = r.estErr > 0 ? r.estErr : 1e-3;
   p.L = r.L > 0 ? r.L : 1;
   if (p.L > kMaxLevel) {
@@ -537,16 +554,14 @@ bool build_plan(const BuildRequest &r, Plan &p, std::string &err) {
   bspline_pieces(nu, p.Q);
   for (int n = 0; n <= nu; ++n) p.J[n] = (double)binom(nu, n) * std::pow(2.0, 1 - nu);
   p.gamma0 = soft(p, -1.0);
-  return box_dependent(p, err);
+  return check_box(p, err) && box_dependent(p, err);
 }
 
 bool rebuild_plan(Plan &p, const double edges[3][3], std::string &err, bool stencils) {
   std::memcpy(p.A, edges, sizeof p.A);
   p.detA = invert3(p.A, p.Ai);
   if (!(p.detA > 0)) { err = "box matrix is singular"; return false; }
-  for (int i = 0; i < 3; ++i)
-    for (int j = 0; j < 3; ++j)
-      if (i != j && p.A[i][j] != 0.0) { err = "only orthorhombic boxes are supported"; return false; }
+  if (!check_box(p, err)) return false;
   for (int l = 1; l <= p.L; ++l) {
     for (int a = 0; a < 3; ++a) p.omegap[l][a].clear();
     p.khat[l].clear();

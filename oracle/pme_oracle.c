@@ -224,7 +224,9 @@ static double real_space(const pmo_t *o, const double *pos, const double *q, dou
      * (pme1.c:126-128) */
     double as = sqrt(o->Ai[0][a] * o->Ai[0][a] + o->Ai[1][a] * o->Ai[1][a] + o->Ai[2][a] * o->Ai[2][a]);
     lim[a] = (int)ceil(a0 * as - 0.5);
-    nc[a] = (int)floor(1.0 / (a0 * as));
+    /* cells at least a0 wide across: face-to-face width 1 / |row a of Ai| */
+    double rs = sqrt(o->Ai[a][0] * o->Ai[a][0] + o->Ai[a][1] * o->Ai[a][1] + o->Ai[a][2] * o->Ai[a][2]);
+    nc[a] = (int)floor(1.0 / (a0 * rs));
     if (nc[a] < 3) nc[a] = 1;
   }
   int ncell = nc[0] * nc[1] * nc[2];

@@ -40,11 +40,13 @@ def test_build_rejects_parameters_with_reason(kw, msg, water):
     assert msg in str(e.value)
 
 
-def test_build_rejects_skewed_box(water):
+def test_build_rejects_skewed_box_beyond_its_domain(water):
+    """Triclinic boxes need the cutoff below half the smallest face-to-face
+    width (14.4 A box, cutoff 8 A)."""
     s = water
     e = s.edges.copy()
     e[0, 1] = 1.0
-    with pytest.raises(MSMError, match="only orthorhombic boxes"):
+    with pytest.raises(MSMError, match="triclinic box: the cutoff 8 must be below half"):
         MSM(s.pos, s.q, e, nu=4, levels=2, top_grid=4, cutoff=8.0)
 
 

@@ -1,0 +1,192 @@
+"""Triclinic boxes (edge matrix with off-diagonal entries).
+
+The reference accepts any box matrix (neighborlist's skew handling,
+forcefield0.c:61-107; fractional coordinates throughout forcefield1.c).
+This build supports triclinic boxes whose cutoff is below half the smallest
+face-to-face width, where the reference's pair rule (the image nearest in
+fractional coordinates) and the sum over all images inside the cutoff agree;
+outside it FF_build refuses the box.
+
+Pins of the oracles (CPU): the PME oracle reproduces the closed-form
+Madelung energy of a CsCl crystal written in sheared cells (the same
+lattice, a' = a, b' = a + b, c' = b + c); the MSM oracle's forces and
+potentials are central differences of its energy in a skewed box.
+The GPU (marked gpu): MSM and PME energies, forces and potentials against
+the oracles in skewed boxes (FP64 to 1e-10, mixed to 1e-5), the Madelung
+energy on the GPU, pair counts, and FF_rebuild into a changed skewed box.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+from helpers import force_err, force_scale, rel_err, rms_rel_force
+
+TOL = 1e-10
+# columns are the box vectors (the reference's A; edges[i][j] = A[i][j])
+SKEW = np.array([[30.0, 5.0, 3.0], [0.0, 32.0, 4.0], [0.0, 0.0, 35.0]])
+
+
+def _widths(A):
+    return 1.0 / np.linalg.norm(np.linalg.inv(A), axis=1)
+
+
+def _cscl_sheared(U, n=4):
+    g = np.arange(n)
+    c = np.array(np.meshgrid(g, g, g, indexing="ij")).reshape(3, -1).T.astype(float)
+    pos = np.concatenate([c, c + 0.5])
+    q = np.concatenate([np.ones(len(c)), -np.ones(len(c))])
+    return pos, q, (np.eye(3) * n) @ U
+
+
+U2 = np.array([[1, 1, 0], [0, 1, 1], [0, 0, 1]])  # a' = a, b' = a + b, c' = b + c
+ALPHA, DNN = 1.76267477307099, np.sqrt(3) / 2     # CsCl Madelung constant, nearest distance
+
+
+def _random_skewed(n=3000, seed=5, charged=0.0):
+    rng = np.random.default_rng(seed)
+    s = rng.uniform(0, 1, (n, 3))
+    pos = s @ SKEW.T  # r = A s
+    q = rng.normal(size=n)
+    q -= q.mean() - charged / n
+    return pos, q, SKEW.copy()
+
+
+def test_domain_is_half_the_smallest_width():
+    w = _widths(SKEW)
+    assert w.min() / 2 > 9.0  # the GPU cases below use cutoffs of 9 A
+
+
+def test_pme_oracle_madelung_in_sheared_cell(oracle_mod):
+    pos, q, A = _cscl_sheared(U2)
+    a0 = 0.49 * _widths(A).min()
+    o = oracle_mod.PmeOracle(pos, q, A, nu=10, M=96, a0=a0, tolDir=1e-10)
+    E, F, phi, _ = o.energy(pos)
+    exact = -(len(q) / 2) * ALPHA / DNN
+    assert rel_err(E, exact) < 1e-11, (E, exact)
+    assert np.max(np.abs(F)) < 1e-10
+    assert np.max(np.abs(phi + q * ALPHA / DNN)) < 1e-10
+
+
+def test_msm_oracle_forces_are_derivatives_in_skewed_box(oracle_mod):
+    pos, q, A = _random_skewed(n=40, seed=9, charged=0.2)
+    o = oracle_mod.Oracle(pos, q, A, nu=6, L=2, Mtop=(6, 6, 7), a0=8.0)
+    E, F, phi, _ = o.energy(pos)
+    h, hq = 1e-5, 1e-6
+    for i in (1, 17, 33):
+        for k in range(3):
+            p1, p2 = pos.copy(), pos.copy()
+            p1[i, k] += h
+            p2[i, k] -= h
+            fd = -(o.energy(p1, want_phi=False)[0] - o.energy(p2, want_phi=False)[0]) / (2 * h)
+            assert abs(fd - F[i, k]) < 2e-6 * max(1.0, abs(F[i, k])), (i, k, fd, F[i, k])
+        q1, q2 = q.copy(), q.copy()
+        q1[i] += hq
+        q2[i] -= hq
+        fdq = (o.energy(pos, q1, want_phi=False)[0] - o.energy(pos, q2, want_phi=False)[0]) / (2 * hq)
+        assert abs(fdq - phi[i]) < 1e-6 * max(1.0, abs(phi[i]))
+
+
+@pytest.mark.parametrize("method", ["msm", "pme"])
+def test_cutoff_beyond_half_width_refused(method):
+    from msmpb_amd import MSM
+    from msmpb_amd.msm import MSMError
+    pos, q, A = _random_skewed(n=200)
+    a0 = 0.51 * _widths(A).min()
+    kw = dict(nu=6, levels=2, top_grid=8) if method == "msm" else dict(nu=6, top_grid=32)
+    with pytest.raises(MSMError, match="triclinic box: the cutoff"):
+        MSM(pos, q, A, cutoff=a0, method=method, **kw)
+
+
+# ---------------------------------------------------------------- GPU
+@pytest.fixture(scope="module")
+def torch_cuda():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return torch
+
+
+CASES = {  # name: (system kwargs, method, MSM/PME parameters)
+    "msm_L2_nu6": (dict(seed=5), "msm", dict(nu=6, levels=2, top_grid=(8, 8, 9), cutoff=9.0)),
+    "msm_L3_nu8_charged": (dict(seed=6, charged=0.4), "msm",
+                           dict(nu=8, levels=3, top_grid=(4, 4, 5), cutoff=9.0)),
+    "pme_nu6": (dict(seed=7), "pme", dict(nu=6, top_grid=(32, 36, 40), cutoff=9.0)),
+}
+
+
+def _oracle(oracle_mod, method, pos, q, A, kw):
+    if method == "msm":
+        return oracle_mod.Oracle(pos, q, A, nu=kw["nu"], L=kw["levels"], Mtop=kw["top_grid"],
+                                 a0=kw["cutoff"])
+    return oracle_mod.PmeOracle(pos, q, A, nu=kw["nu"], M=kw["top_grid"], a0=kw["cutoff"])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", list(CASES))
+def test_triclinic_matches_oracle(case, oracle_mod, torch_cuda):
+    from msmpb_amd import MSM
+    sk, method, kw = CASES[case]
+    pos, q, A = _random_skewed(**sk)
+    print(f"[triclinic] {case}", flush=True)
+    o = _oracle(oracle_mod, method, pos, q, A, kw)
+    Eo, Fo, phio, _ = o.energy(pos)
+    for precision in ("fp64", "mixed"):
+        m = MSM(pos, q, A, method=method, precision=precision, **kw)
+        try:
+            count = o.pair_count if method == "pme" else o.pair_count()
+            assert m.pair_count == count
+            E, F, phi = m.energy(pos, potential=True)
+            if precision == "fp64":
+                assert rel_err(E, Eo) < TOL, (E, Eo)
+                assert force_err(F, Fo) < TOL
+                assert np.max(np.abs(phi - phio)) < TOL * np.max(np.abs(phio))
+            else:
+                assert rel_err(E, Eo) < 1e-5 and rms_rel_force(F, Fo) < 1e-5
+        finally:
+            m.close()
+
+
+@pytest.mark.gpu
+def test_pme_madelung_sheared_cell_on_gpu(torch_cuda):
+    from msmpb_amd import MSM
+    pos, q, A = _cscl_sheared(U2)
+    a0 = 0.49 * _widths(A).min()
+    print("[triclinic] PME Madelung, sheared CsCl cell", flush=True)
+    m = MSM(pos, q, A, nu=10, top_grid=96, cutoff=a0, method="pme", tol_dir=1e-10)
+    try:
+        E, F, phi = m.energy(pos, potential=True)
+        exact = -(len(q) / 2) * ALPHA / DNN
+        assert rel_err(E, exact) < 1e-11, (E, exact)
+        assert np.max(np.abs(F)) < 1e-9 * force_scale(q, A)
+        assert np.max(np.abs(phi + q * ALPHA / DNN)) < 1e-10
+    finally:
+        m.close()
+
+
+@pytest.mark.gpu
+def test_triclinic_rebuild_matches_oracle(oracle_mod, torch_cuda):
+    """FF_rebuild into a changed skewed box (stencils and the top-level kernel
+    regenerated on the GPU), positions moved with the box."""
+    from msmpb_amd import MSM, _lib
+    pos, q, A = _random_skewed(seed=8)
+    kw = dict(nu=6, levels=2, top_grid=(8, 8, 9), cutoff=9.0)
+    B = A.copy()
+    B[0, 1] += 0.4  # more shear
+    B *= 1.003
+    x1 = pos @ np.linalg.solve(A, B).T  # same fractional coordinates in the new box
+    o = _oracle(oracle_mod, "msm", x1, q, B, kw)
+    Eo, Fo, _, _ = o.energy(x1, want_phi=False)
+    print("[triclinic] rebuild", flush=True)
+    m = MSM(pos, q, A, **kw)
+    try:
+        e = np.ascontiguousarray(B)
+        x = np.ascontiguousarray(x1)
+        dp = ctypes.POINTER(ctypes.c_double)
+        _lib.lib().FF_rebuild(m._ff, e.ctypes.data_as(dp), x.ctypes.data_as(dp), ctypes.c_double(0.0))
+        assert m._built(), _lib.last_error()
+        E, F = m.energy(x1)
+        assert rel_err(E, Eo) < TOL, (E, Eo)
+        assert force_err(F, Fo) < TOL
+    finally:
+        m.close()
