@@ -69,15 +69,16 @@ def algorithmic(stage, m, P, nu):
     N = m.N
     mixed = "true" if m.precision == "mixed" else "false"
     if stage == "partcl2partcl":  # P within-cutoff pairs i < j, 35 + 4(nu-1) FLOP each
-        return P * (35 + 4 * (nu - 1)), "flop", f"k_pair_cell<{mixed}, {nu}>"
+        # the orthorhombic instance (template arguments: precision, order, triclinic)
+        return P * (35 + 4 * (nu - 1)), "flop", f"k_pair_cell<{mixed}, {nu}, false>"
     if stage == "grid2grid":  # 2 * sum_l prod(M_l) prod(s_l), l < L
         f = sum(2 * np.prod(m.grid_dims(l)) * np.prod(m.stencil_dims(l))
                 for l in range(1, m.levels))
         return f, "flop", "k_g2g_tiles<" + ("1" if mixed == "true" else "0")
     if stage == "anterpolation":  # N (2 nu^3 + 6 nu^2) FLOP
-        return N * (2 * nu ** 3 + 6 * nu ** 2), "flop", f"k_spread_mfma<{nu}>"
+        return N * (2 * nu ** 3 + 6 * nu ** 2), "flop", f"k_spread_mfma<{nu},"
     if stage == "interpolation":  # N (8 nu^3 + 6 nu^2) FLOP
-        return N * (8 * nu ** 3 + 6 * nu ** 2), "flop", f"k_gather<{nu}>"
+        return N * (8 * nu ** 3 + 6 * nu ** 2), "flop", f"k_gather_tile<{nu},"
     return None
 
 
@@ -278,9 +279,9 @@ def pme_comparator(args, sysm, F64=None):
     # the dominant kernel (the Ewald pairs, mixed): VALU issue against the
     # machine's issue rate; no FLOP roofline (erfc and exp dominate the term)
     st = rp["mixed"]["stages"].get("partcl2partcl", 0.0)
-    vi = valu_insts_of("k_pair_cell<true, 0>", PME_SQ_FILE)
+    vi = valu_insts_of("k_pair_cell<true, 0, false>", PME_SQ_FILE)
     if vi is not None and st > 0:
-        out["valu"] = {"kernel": "k_pair_cell<true, 0>", "insts_per_launch": vi,
+        out["valu"] = {"kernel": "k_pair_cell<true, 0, false>", "insts_per_launch": vi,
                        "issue_frac": vi / VALU_WAVE_INSTS_PER_S / st, "launch_ms": st * 1e3,
                        "source": "SQ_INSTS_VALU, profiles/water96k_pme_kernel_stats.md"}
     if not args.no_cpu:
