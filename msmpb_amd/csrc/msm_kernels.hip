@@ -861,7 +861,8 @@ __device__ __forceinline__ void cell_span(int g0, int B, int half, int M, int nc
 __device__ __forceinline__ int spread_index(const int *roff, const int *rbeg, int nr, int k,
                                             int total, int &lo) {
   if (k >= total) return -1;
-  while (lo + 1 < nr && roff[lo + 1] <= k) ++lo;
+  // roff[nr] = total > k ends the walk at the last range
+  while (roff[lo + 1] <= k) ++lo;
   return rbeg[lo] + (k - roff[lo]);
 }
 
