@@ -838,6 +838,7 @@ __global__ __launch_bounds__(kPairBS) void k_pair_count(
 // pair kernel's wavefronts.
 constexpr int kSX = 16, kSY = 4, kSZ = 4;
 constexpr int kSRanges = 512;       // index ranges per batch of cell columns
+constexpr int kSJump = 8;           // ranges skipped per step of the candidate walk
 constexpr int kWXs = 17, kWYs = 5;  // padded record strides (doubles)
 constexpr int kSFlush = 32;         // atoms per flush (records per wavefront)
 constexpr int kSQueue = kSFlush + 64;  // queued atoms: < kSFlush + one batch of 64
@@ -861,7 +862,8 @@ __device__ __forceinline__ void cell_span(int g0, int B, int half, int M, int nc
 __device__ __forceinline__ int spread_index(const int *roff, const int *rbeg, int nr, int k,
                                             int total, int &lo) {
   if (k >= total) return -1;
-  // roff[nr] = total > k ends the walk at the last range
+  // roff[nr .. nr + kSJump] = total > k ends both walks by the last range
+  while (roff[lo + kSJump] <= k) lo += kSJump;
   while (roff[lo + 1] <= k) ++lo;
   return rbeg[lo] + (k - roff[lo]);
 }
@@ -971,7 +973,7 @@ __global__ __launch_bounds__(64 * WPB) void k_spread_mfma(DevParams P, GridDims 
   __shared__ double WX[WPB * kSFlush * kWXs];
   __shared__ double WY[WPB * kSFlush * kWYs], WZ[WPB * kSFlush * kWYs];
   __shared__ double4 queue[WPB * kSQueue];
-  __shared__ int rbeg[kSRanges], roff[kSRanges + 1];
+  __shared__ int rbeg[kSRanges], roff[kSRanges + 1 + kSJump];  // total repeated past the end
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int nbz = (g.z + kSZ - 1) / kSZ, nby = (g.y + kSY - 1) / kSY;
   int b = blockIdx.x;
@@ -1026,6 +1028,8 @@ __global__ __launch_bounds__(64 * WPB) void k_spread_mfma(DevParams P, GridDims 
         if (k < nr) roff[k + 1] = excl + loc[u];
       }
       if (lane == 0) roff[0] = 0;
+      const int tot = __shfl(incl, 63, 64);
+      if (lane < kSJump) roff[nr + 1 + lane] = tot;  // the jumps of spread_index stop here
     }
     __syncthreads();
     const int total = roff[nr];
