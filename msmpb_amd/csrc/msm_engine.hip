@@ -470,8 +470,7 @@ int rebuild_box(Engine &E) {
   msm::StencilBox b{};
   std::memcpy(b.A, p.A, sizeof b.A);
   std::memcpy(b.Ai, p.Ai, sizeof b.Ai);
-  for (int a = 0; a < 3; ++a)
-    b.as[a] = std::sqrt(p.Ai[0][a] * p.Ai[0][a] + p.Ai[1][a] * p.Ai[1][a] + p.Ai[2][a] * p.Ai[2][a]);
+  for (int a = 0; a < 3; ++a) b.as[a] = p.kas[a];
   for (int l = 0; l <= p.L; ++l) b.aCut[l] = p.aCut[l];
   for (int k = 0; k < p.nu; ++k) b.tau[k] = p.tau[k];
   msm::top_series(p.nu, b.taylor);
@@ -933,6 +932,7 @@ int build_impl(FF *ff, int N, double (*position)[3], double edges[3][3]) {
   r.estErr = ff->estErr;
   r.method = x->method == MSM_METHOD_PME ? msm::kPme : msm::kMsm;
   r.tolDir = x->tolDir;
+  r.complete_images = x->layout != MSM_LAYOUT_ASIS;
   msm::Plan plan;
   std::string err;
   if (!ff->FFT) return fail(MSM_ERR_ARG, "FF_build: only the FFT top level is supported");
@@ -943,6 +943,17 @@ int build_impl(FF *ff, int N, double (*position)[3], double edges[3][3]) {
   if (!msm::build_plan(r, plan, err)) return fail(MSM_ERR_ARG, "FF_build: " + err);
   double tstencil =
       std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  // the reference's FF_build writes the parameters its rules chose into the
+  // public fields (forcefield0.c:287-348); they are the host plan's, so they
+  // are visible before (and whether or not) the device part succeeds
+  std::memcpy(ff->A, plan.A, sizeof ff->A);
+  std::memcpy(ff->Ai, plan.Ai, sizeof ff->Ai);
+  ff->detA = plan.detA;
+  ff->estErr = plan.estErr;
+  ff->orderAcc = plan.nu;
+  ff->maxLevel = plan.L;
+  for (int a = 0; a < 3; ++a) ff->topGridDim[a] = plan.M[plan.L][a];
+  ff->cutoff = plan.aCut[0];
   TRY(create_engine(x, plan));
   publish_plan(x);
   for (int l = 1; l <= plan.L && l < 10; ++l) ff->time_stencil[l] = tstencil / plan.L;
@@ -1240,7 +1251,27 @@ int FF_set_layout_compat(FF *ff, int layout) {
     return fail(MSM_ERR_ARG, "layout must be MSM_LAYOUT_FIXED or MSM_LAYOUT_ASIS");
   if (layout == MSM_LAYOUT_ASIS && X(ff)->method == MSM_METHOD_PME)
     return fail(MSM_ERR_ARG, "the as-is index layout does not apply to the PME method");
-  X(ff)->layout = layout;
+  FFX *x = X(ff);
+  x->layout = layout;
+  if (x->built && x->eng && x->eng->plan.method == msm::kMsm) {
+    // the layout also selects the level kernels' image range: regenerate
+    // the stencils of a built FF when that changes them
+    msm::Plan &p = x->eng->plan;
+    const double old[3] = {p.kas[0], p.kas[1], p.kas[2]};
+    p.complete_images = layout != MSM_LAYOUT_ASIS;
+    msm::kappa_image_bound(p);
+    if (p.L >= 2 && (old[0] != p.kas[0] || old[1] != p.kas[1] || old[2] != p.kas[2])) {
+      std::string err;
+      double edges[3][3];
+      std::memcpy(edges, p.A, sizeof edges);
+      if (!msm::rebuild_plan(p, edges, err, /*stencils=*/false) || rebuild_box(*x->eng) != MSM_OK) {
+        x->built = false;
+        if (!err.empty()) g_err = "FF_set_layout_compat: " + err;
+        return MSM_ERR_ARG;
+      }
+      publish_plan(x);
+    }
+  }
   return MSM_OK;
 }
 

@@ -869,8 +869,10 @@ __device__ __forceinline__ int spread_index(const int *roff, const int *rbeg, in
 }
 
 // Fractional offsets t and the brick-relative index of the lowest support
-// point m - NU/2 + 1 per axis (m lies in [0, M] and org in [0, M), so one
-// wrap suffices, NU >= 4); reach: the support meets the brick.
+// point m - NU/2 + 1 per axis, reduced into [0, M) (m lies in [0, M] and org
+// in [0, M): one addition of M leaves it negative for NU >= 6 with org close
+// to M, and a negative index would write below the atom's LDS record);
+// reach: the support meets the brick.
 template <int NU, bool TRI>
 __device__ __forceinline__ bool spread_place(const DevParams &P, const double4 &at,
                                              const int dims[3], const int org[3],
@@ -885,7 +887,7 @@ __device__ __forceinline__ bool spread_place(const DevParams &P, const double4 &
     sv = sv - floor(sv);
     const double tt = dims[ax] * sv, m = floor(tt);
     t[ax] = tt - m;
-    int v = (int)m - NU / 2 + 1 - org[ax];
+    int v = ((int)m - NU / 2 + 1 - org[ax]) % dims[ax];
     v += v < 0 ? dims[ax] : 0;
     rlo[ax] = v;
     reach = reach && (v < ext[ax] || v > dims[ax] - NU);

@@ -264,7 +264,9 @@ void build_level_kernel(Plan &p, int l, const double as[3]) {
 }
 
 bool box_dependent(Plan &p, std::string &err, bool stencils = true) {
+  // the reference's image-range vector: column norms of A^-1
   double as[3] = {col_norm(p.Ai, 0), col_norm(p.Ai, 1), col_norm(p.Ai, 2)};
+  kappa_image_bound(p);
   // constant-energy coefficients (forcefield0.c:612-643): own images of
   // each charge inside the cutoff, and the neutralising background
   double a0 = p.aCut[0];
@@ -288,7 +290,7 @@ bool box_dependent(Plan &p, std::string &err, bool stencils = true) {
   build_omegap(p);
   if (stencils) {
     build_top_kernel(p);
-    for (int l = p.L - 1; l >= 1; --l) build_level_kernel(p, l, as);
+    for (int l = p.L - 1; l >= 1; --l) build_level_kernel(p, l, p.kas);
   }
   (void)err;
   return true;
@@ -468,10 +470,18 @@ double delta_f_estimate(const Plan &p, const double *q) {
          std::pow(p.detA, -1. / 3.) * std::pow(p.N, -2.0 / 3.0) / std::pow(p.aCut[0], p.nu);
 }
 
+void kappa_image_bound(Plan &p) {
+  for (int a = 0; a < 3; ++a) {
+    const double c = col_norm(p.Ai, a);
+    p.kas[a] = p.complete_images ? std::fmax(c, row_norm(p.Ai, a)) : c;
+  }
+}
+
 bool build_plan(const BuildRequest &r, Plan &p, std::string &err) {
   char buf[256];
   p = Plan();
   p.N = r.N;
+  p.complete_images = r.complete_images;
   if (r.N <= 0) { err = "N must be positive"; return false; }
   std::memcpy(p.A, r.edges, sizeof p.A);
   p.detA = invert3(p.A, p.Ai);

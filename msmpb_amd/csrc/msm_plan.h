@@ -37,6 +37,16 @@ struct Plan {
   double J[11] = {};           // two-scale stencil J[n + nu/2], n = -nu/2..nu/2
   double gamma0 = 0;           // gamma(0) = tau(-1)
   double coeff1 = 0, coeff2 = 0;
+  // Image range of the level kernels kappa_l: images p with
+  // |s_a - p_a| <= a_l * kas[a] are summed.  Every image inside a_l
+  // satisfies it for kas[a] >= |row a of A^-1|.  The reference passes the
+  // column norms of A^-1 (forcefield0.c:618-620, 670), which equal the row
+  // norms for orthorhombic boxes but can drop images of a triclinic one;
+  // complete_images = false reproduces that exactly (the as-is layout),
+  // true takes max(row, column) norm, so every box the reference's range
+  // already covers gets its numbers bit for bit.
+  bool complete_images = true;
+  double kas[3] = {};
   std::vector<double> omegap[kMaxLevel + 1][3];
   std::vector<double> cL[3];
   std::vector<double> khat[kMaxLevel + 1];  // l < L: S^3 real-space; l = L: M_L^3 Fourier
@@ -50,6 +60,7 @@ struct BuildRequest {
   double a0 = -1, estErr = -1;
   int method = kMsm;
   double tolDir = -1;          // PME only
+  bool complete_images = true; // level-kernel image range (Plan::kas)
 };
 
 // Resolves parameters with the reference's rules (FF_build of
@@ -63,6 +74,9 @@ bool build_plan(const BuildRequest &req, Plan &plan, std::string &err);
 // false the level and top-level stencils (khat) are left empty for the GPU
 // to generate; omega', cL and the constants are still computed.
 bool rebuild_plan(Plan &plan, const double edges[3][3], std::string &err, bool stencils = true);
+
+// Sets plan.kas from the box and plan.complete_images.
+void kappa_image_bound(Plan &plan);
 
 // Series coefficients b_0..b_6 of t^2 psi(t) of the top-level kernel.
 void top_series(int nu, double b[7]);

@@ -12,7 +12,15 @@
  * arithmetic inside the reference's valid parameter domain
  * (min(2*nLim+1, M_l) >= nu for every level l < L), and rejects inputs
  * outside it.  compat = 1 re-creates the two layout quirks of the
- * reference HEAD as explicit index maps (see energy() below).
+ * reference HEAD as explicit index maps (see energy() below), and the
+ * HEAD's image range of the level kernels: kappaA sums the images p with
+ * |s_a - p_a| < a_l * as_a where FF_rebuild passes the column norms of
+ * A^-1 (forcefield0.c:618-620, 670, 1049-1051).  Every image within a_l
+ * needs the row norm |row a of A^-1| instead; the two agree for
+ * orthorhombic boxes, and in a triclinic box the column norms can drop
+ * images (a sheared CsCl cell at L = 2 loses 1e-3 of its energy).
+ * compat = 0 sums over max(row, column) norm: identical terms wherever the
+ * HEAD's range already holds every image, the complete kernel otherwise.
  */
 #include "msm_oracle.h"
 
@@ -437,7 +445,9 @@ orc_t *orc_build(int N, const double *q, const double *pos, const double *edges,
       }
   o->coeff2 = pi_() * o->aCut[o->L] * o->aCut[o->L] / ((4 * nu + 2) * o->detA);
   build_top_kernel(o);
-  for (int l = o->L - 1; l >= 1; l--) build_level_kernel(o, l, as);
+  double kas[3];
+  for (int a = 0; a < 3; a++) kas[a] = o->compat ? as[a] : fmax(as[a], rownorm(o->Ai, a));
+  for (int l = o->L - 1; l >= 1; l--) build_level_kernel(o, l, kas);
   for (int l = 1; l <= o->L; l++) {
     size_t n = (size_t)o->gd[l][0] * o->gd[l][1] * o->gd[l][2];
     o->qg[l] = (double *)calloc(n, sizeof(double));

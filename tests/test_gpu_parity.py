@@ -58,6 +58,8 @@ CASES = {
     "spce300_L1_nu4": lambda: _dataset_system("spceN300", 4, 1, 24, 9.0),
     "spce300_L2_nu6": lambda: _dataset_system("spceN300", 6, 2, 6, 9.0),
     "spce300_L1_nu8_a11": lambda: _dataset_system("spceN300", 8, 1, 30, 11.0),
+    "spce300_L1_nu6": lambda: _dataset_system("spceN300", 6, 1, 24, 9.0),
+    "spce300_L2_nu4_M8": lambda: _dataset_system("spceN300", 4, 2, 8, 9.0),
     "nacl512_L3_nu6": lambda: _dataset_system("NaClN512", 6, 3, 4, 3.0),
     "tiny_box_self_images": lambda: _random_system("tiny", 60, (12.0, 13.0, 14.0), 13.0, 1, 8, 7),
     "dense400_L2": lambda: _random_system("dense", 400, (10.0, 11.0, 12.0), 6.0, 2, 5, 7),
@@ -99,7 +101,8 @@ def test_fp64_stage_grids(case, oracle_mod, torch_cuda):
         assert np.max(np.abs(st - sto)) <= 1e-11 * np.max(np.abs(sto)), l
 
 
-@pytest.mark.parametrize("case", ["spce300_L1_nu4", "spce300_L2_nu6", "water300", "nacl512_L3_nu6"])
+@pytest.mark.parametrize("case", ["spce300_L1_nu4", "spce300_L1_nu6", "spce300_L2_nu6",
+                                  "spce300_L2_nu4_M8", "water300", "nacl512_L3_nu6"])
 def test_asis_layout(case, oracle_mod, torch_cuda):
     print(f"[parity] asis {case}", flush=True)
     s = CASES[case]()
