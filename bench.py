@@ -209,20 +209,48 @@ def autograd_ms(m, pos, q, steps, warmup, recompute=None):
     return (time.perf_counter() - t0) * 1e3 / steps
 
 
-def cpu_baseline(sysm):
-    """The oracle (single-threaded C restatement of the reference) on one full
-    evaluation of the same workload, pair-list build excluded like the
-    reference's time_energy."""
+def host_cpu():
+    """(CPU model, logical CPUs this process may use) of the host."""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count()
+    return model, n
+
+
+def timed_evals(fn, reps):
+    """Median (and min, max) of reps single-thread CPU times of fn(); the
+    result of the last call."""
+    ts, out = [], None
+    for _ in range(reps):
+        t0 = time.process_time()
+        out = fn()
+        ts.append(time.process_time() - t0)
+    return float(np.median(ts)), min(ts), max(ts), out
+
+
+def cpu_baseline(sysm, reps=5):
+    """The oracle (single-threaded C restatement of the reference) on full
+    evaluations of the same workload, pair-list build excluded like the
+    reference's time_energy: the median of reps evaluations."""
     from oracle.oracle import Oracle
     p = sysm.params
     o = Oracle(sysm.pos, sysm.q, sysm.edges, nu=p["nu"], L=p["levels"], Mtop=p["top_grid"],
                a0=p["cutoff"])
-    t0 = time.process_time()
-    E, F, _, _ = o.energy(sysm.pos, want_phi=False)
-    t = time.process_time() - t0
-    return dict(value=1.0 / t, unit="evals/s", cores=1, kind="port",
-                sample=f"1 full evaluation of the same {sysm.N}-atom workload (energy+forces, "
-                       f"pair-list build excluded), {t:.2f} s CPU, one core"), E, F
+    t, tmin, tmax, (E, F, _, _) = timed_evals(lambda: o.energy(sysm.pos, want_phi=False), reps)
+    model, ncpu = host_cpu()
+    return dict(value=1.0 / t, unit="evals/s", cores=1, kind="port", cpu=model, nproc=ncpu,
+                sample=f"median of {reps} full evaluations of the same {sysm.N}-atom workload "
+                       f"(energy+forces, pair-list build excluded): {t:.2f} s CPU each "
+                       f"(min {tmin:.2f}, max {tmax:.2f}), one thread"), E, F
 
 
 def rms_rel(Fa, Fb):
@@ -241,7 +269,10 @@ def roofline(stages, m, pairs, nu):
     peak = {"partcl2partcl": PEAK["fp32"] if m.precision == "mixed" else PEAK["fp64"],
             "grid2grid": PEAK["bf16"] if m.precision == "mixed" else PEAK["fp64"]}.get(
                 dom, PEAK["fp64"])
-    out = dict(bound="mfma", achieved=ach, peak=peak, unit="TFLOP/s", frac=ach / peak,
+    # the pair kernel is bound by VALU issue (FP32 or FP64 vector arithmetic),
+    # the level stencils by the matrix cores
+    bound = "valu" if dom == "partcl2partcl" else "mfma"
+    out = dict(bound=bound, achieved=ach, peak=peak, unit="TFLOP/s", frac=ach / peak,
                traffic=traffic_of(kernel), kernel=kernel, stage=dom,
                work_per_launch=float(amount), launch_ms=stages[dom] * 1e3)
     vi = valu_insts_of(kernel)
@@ -252,17 +283,17 @@ def roofline(stages, m, pairs, nu):
     return out
 
 
-def pme_cpu_baseline(sysm):
-    """The PME oracle (single-threaded C restatement of pme0.c/pme1.c) on one
-    evaluation of the same workload, pair count excluded."""
+def pme_cpu_baseline(sysm, reps=3):
+    """The PME oracle (single-threaded C restatement of pme0.c/pme1.c) on
+    evaluations of the same workload, pair count excluded: the median of reps."""
     from oracle.oracle import PmeOracle
     o = PmeOracle(sysm.pos, sysm.q, sysm.edges, nu=6, M=PME_GRID, a0=10.0)
-    t0 = time.process_time()
-    E, F, _, _ = o.energy(sysm.pos, want_phi=False)
-    t = time.process_time() - t0
-    return dict(value=1.0 / t, unit="evals/s", cores=1, kind="port",
-                sample=f"1 full PME evaluation of the same {sysm.N}-atom workload (energy+forces), "
-                       f"{t:.2f} s CPU, one core"), E, F
+    t, tmin, tmax, (E, F, _, _) = timed_evals(lambda: o.energy(sysm.pos, want_phi=False), reps)
+    model, ncpu = host_cpu()
+    return dict(value=1.0 / t, unit="evals/s", cores=1, kind="port", cpu=model, nproc=ncpu,
+                sample=f"median of {reps} full PME evaluations of the same {sysm.N}-atom "
+                       f"workload (energy+forces): {t:.2f} s CPU each (min {tmin:.2f}, "
+                       f"max {tmax:.2f}), one thread"), E, F
 
 
 def pme_comparator(args, sysm, F64=None):
