@@ -123,6 +123,19 @@ int msm_copy_stencil(FF *ff, int level, double *host_out);
  * the GPU for the given positions (host N x 3). */
 int msm_count_pairs(FF *ff, double (*position)[3], long long *count);
 
+/* grid2grid (forcefield.h, the reference's forcefield0.c:898-931) on device
+ * pointers, allocating nothing: no counterpart in the reference; for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops).  d_el (gd.x * gd.y * gd.z doubles) += the correlation of
+ * the stencil d_kh (sd.x * sd.y * sd.z) with d_ql, a level grid in the
+ * reference's padded layout (pad max(min(2 nLim + 1, gd), nu) per axis,
+ * interior at max(pad / 2, nu / 2); nu and nLim of the built ff).  Its two
+ * scratch grids live in the library's static device array: at most 524,288
+ * grid points per call (larger grids: MSM_ERR_ARG).  Stream-ordered on
+ * `stream`; calls that overlap on one GPU share the array and race. */
+int msm_grid2grid_device_noalloc(FF *ff, int l, Triple gd, double *d_el, const double *d_ql,
+                                 Triple sd, const double *d_kh, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

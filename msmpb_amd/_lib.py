@@ -84,7 +84,7 @@ EXPORTS = [
     "FF_set_stage_timing", "FF_get_energy_terms", "FF_energy_potential", "msm_eval_device",
     "msm_forward_device", "msm_adjoint_device", "msm_grid_dims", "msm_stencil_dims",
     "msm_copy_grid", "msm_copy_stencil", "msm_count_pairs", "msm_stage_times", "msm_stage_name",
-    "FF_set_method", "FF_get_beta",
+    "FF_set_method", "FF_get_beta", "msm_grid2grid_device_noalloc",
     # include/msm_bridge.h
     "msmb_create", "msmb_build", "msmb_rebuild", "msmb_energy", "msmb_configure", "msmb_info",
     "msmb_tables", "msmb_times", "msmb_delete",

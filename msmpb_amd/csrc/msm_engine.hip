@@ -1020,6 +1020,49 @@ double my_errest(FF *ff, int nu) {
   return dF / Fref;
 }
 
+// ---- grid2grid without allocation -----------------------------------------
+// Scratch of msm_grid2grid_device_noalloc: the unpadded input grid and the
+// stencil result, kNoAllocGrid doubles each.  Every GPU the library runs on
+// holds one copy (8 MiB) until the process ends.  In host code `scratch`
+// only names the symbol; kernels get the device address that
+// hipGetSymbolAddress returns for the current device.
+constexpr long kNoAllocGrid = 1L << 19;  // grid points per call (80^3 = 512,000 fit)
+__device__ __align__(256) double scratch[2 * kNoAllocGrid];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch[kMaxDevices] = {};
+
+// Device address of `scratch` on device `dev` (looked up once per device,
+// with that device current).
+int scratch_on(int dev, double **out) {
+  if (dev < 0 || dev >= kMaxDevices) return fail(MSM_ERR_ARG, "device index out of range");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch[dev]) {
+    void *ptr = nullptr;
+    HIP_TRY(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    g_scratch[dev] = static_cast<double *>(ptr);
+  }
+  *out = g_scratch[dev];
+  return MSM_OK;
+}
+
+// q[m] = ql[m + of] of the reference's padded level array (dims g + pad).
+__global__ __launch_bounds__(256) void k_unpad(msm::GridDims g, int3 pad, int3 of,
+                                               const double *__restrict__ ql,
+                                               double *__restrict__ q) {
+  const long e = (long)blockIdx.x * 256 + threadIdx.x;
+  if (e >= g.size()) return;
+  const int k = (int)(e % g.z), j = (int)((e / g.z) % g.y), i = (int)(e / ((long)g.y * g.z));
+  q[e] = ql[((long)(i + of.x) * (g.y + pad.y) + (j + of.y)) * (g.z + pad.z) + k + of.z];
+}
+
+__global__ __launch_bounds__(256) void k_accumulate(long n, const double *__restrict__ e,
+                                                    double *__restrict__ el) {
+  const long i = (long)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) el[i] += e[i];
+}
+
 }  // namespace
 
 // ============================================================ forcefield.h
@@ -1217,6 +1260,46 @@ void grid2grid(FF *ff, const int l, const Triple gd, double *el, const double *q
   }
   for (long m = 0; m < n; ++m) el[m] += e[m];
   (void)l;
+}
+
+int msm_grid2grid_device_noalloc(FF *ff, int l, Triple gd, double *d_el, const double *d_ql,
+                                 Triple sd, const double *d_kh, void *stream) {
+  // grid2grid (forcefield0.c:898-931) on device pointers, with its two
+  // scratch grids in the library's static device array
+  TRY(check_built(ff));
+  (void)l;
+  if (!d_el || !d_ql || !d_kh) return fail(MSM_ERR_ARG, "grid2grid: NULL device pointer");
+  const int g[3] = {gd.x, gd.y, gd.z}, st[3] = {sd.x, sd.y, sd.z};
+  for (int a = 0; a < 3; ++a)
+    if (g[a] < 1 || st[a] < 1) return fail(MSM_ERR_ARG, "grid2grid: grid and stencil dims must be positive");
+  const long n = (long)g[0] * g[1] * g[2];
+  if (n > kNoAllocGrid)
+    return fail(MSM_ERR_ARG, "grid2grid: " + std::to_string(n) + " grid points exceed the " +
+                                 std::to_string(kNoAllocGrid) +
+                                 " of the no-allocation entry point's scratch");
+  // the stencil kernel's LDS slab (launch_level_fp64): rows x pitch doubles
+  int LZ = std::min((g[2] + 7) / 8, 16), LY = std::min(64 / LZ, g[1]);
+  const long lds = 8L * (LY + st[1] - 1) * (LZ * 8 + st[2] - 1 + 8);
+  if (lds > 65536) return fail(MSM_ERR_ARG, "grid2grid: stencil too large for one workgroup's LDS");
+  Engine &E = *X(ff)->eng;
+  double *scr = nullptr;
+  TRY(scratch_on(E.device, &scr));
+  const int nu = E.plan.nu, dmax = 2 * E.plan.nLim + 1;
+  int pad[3], of[3];
+  for (int a = 0; a < 3; ++a) {
+    pad[a] = std::max(std::min(dmax, g[a]), nu);
+    of[a] = std::max(pad[a] / 2, nu / 2);
+  }
+  const msm::GridDims G = {g[0], g[1], g[2]};
+  const hipStream_t s = static_cast<hipStream_t>(stream);
+  const unsigned nb = (unsigned)((n + 255) / 256);
+  double *q = scr, *e = scr + kNoAllocGrid;
+  k_unpad<<<nb, 256, 0, s>>>(G, make_int3(pad[0], pad[1], pad[2]), make_int3(of[0], of[1], of[2]),
+                             d_ql, q);
+  msm::launch_level_fp64(E.P, G, {1, 1, 1}, {st[0], st[1], st[2]}, d_kh, q, nullptr, e, s);
+  k_accumulate<<<nb, 256, 0, s>>>(n, e, d_el);
+  HIP_TRY(hipGetLastError());
+  return MSM_OK;
 }
 
 // ============================================================ msmpb_hip.h
