@@ -105,6 +105,7 @@ struct Engine {
   int4 *d_sten = nullptr;
   unsigned long long *d_count = nullptr;
   double *d_ql[msm::kMaxLevel + 1] = {}, *d_el[msm::kMaxLevel + 1] = {};
+  unsigned *d_qb[msm::kMaxLevel + 1] = {};  // q^l as bf16 hi/lo pairs (l < L)
   double *d_khat[msm::kMaxLevel + 1] = {};
   void *d_ksplit[msm::kMaxLevel + 1] = {};
   void *d_qsplit = nullptr;
@@ -147,6 +148,8 @@ struct Engine {
       if (d_el[l]) hipFree(d_el[l]);
       if (d_khat[l]) hipFree(d_khat[l]);
       if (d_ksplit[l]) hipFree(d_ksplit[l]);
+      if (d_qb[l]) hipFree(d_qb[l]);
+      d_qb[l] = nullptr;
       d_ql[l] = d_el[l] = d_khat[l] = nullptr;
       d_ksplit[l] = nullptr;
     }
@@ -539,32 +542,33 @@ int device_streams(int dev, hipStream_t (&out)[2]) {
 
 int plan_levels(Engine &E) {
   const msm::Plan &p = E.plan;
-  static const int target[2] = {4096, 2048};
+  static const int target[2] = {4096, 1024};
   for (int pr = 0; pr < 2; ++pr) {
     msm::G2GArgs &A = E.g2g[pr];
     A.nlev = 0;
     A.ntasks = 0;
     E.g2g_lds[pr] = 0;
     for (int l = 1; l < p.L; ++l) {
-      msm::G2GLevel &lv = A.lv[l - 1];
-      msm::plan_g2g_level(pr, E.gd[l], E.sd[l], target[pr], lv);
-      lv.task0 = A.ntasks;
-      A.ntasks += (long)lv.g.x * lv.ty * lv.tz * lv.G;
-      lv.q = E.d_ql[l];
-      TRY(dalloc(&E.d_part[pr][l], (size_t)lv.G * E.gd[l].size()));
-      lv.part = E.d_part[pr][l];
+      msm::plan_g2g_level(pr, E.gd[l], E.sd[l], target[pr], A.lv[l - 1]);
       A.nlev = l;
     }
-    // one launch covers every level, so they share the K-chunk count (extra
-    // chunks of a smaller stencil hold zero Toeplitz columns)
-    int nch = 0;
-    for (int k = 0; k < A.nlev; ++k) nch = std::max(nch, A.lv[k].nch);
-    if (nch > msm::g2g_max_nch(pr))
-      return fail(MSM_ERR_ARG, "FF_build: level stencil too wide for the MFMA tiles");
+    // level 1 has a launch of its own, the levels 2 .. L-1 share one: within
+    // a launch the levels share the K-chunk count (extra chunks of a smaller
+    // stencil hold zero Toeplitz columns) and the z tiles per wavefront
+    if (A.nlev >= 1) A.ntasks = msm::plan_g2g_launch(pr, &A.lv[0], 1, target[pr]);
+    if (A.nlev >= 2) A.ntasks += msm::plan_g2g_launch(pr, &A.lv[1], A.nlev - 1, target[pr]);
     for (int k = 0; k < A.nlev; ++k) {
-      A.lv[k].nch = nch;
-      E.g2g_lds[pr] = std::max(E.g2g_lds[pr], msm::g2g_lds_bytes(pr, A.lv[k]));
+      msm::G2GLevel &lv = A.lv[k];
+      if (lv.nch > msm::g2g_max_nch(pr))
+        return fail(MSM_ERR_ARG, "FF_build: level stencil too wide for the MFMA tiles");
+      lv.q = E.d_ql[k + 1];
+      lv.qb = E.d_qb[k + 1];
+      TRY(dalloc(&E.d_part[pr][k + 1], (size_t)lv.G * E.gd[k + 1].size()));
+      lv.part = E.d_part[pr][k + 1];
+      E.g2g_lds[pr] = std::max(E.g2g_lds[pr], msm::g2g_lds_bytes(pr, lv));
     }
+    if (E.g2g_lds[pr] > 65536)
+      return fail(MSM_ERR_ARG, "FF_build: level stencil too wide for one workgroup's LDS");
   }
   return MSM_OK;
 }
@@ -629,6 +633,7 @@ int create_engine(FFX *x, const msm::Plan &plan) {
   for (int l = 1; l <= p.L; ++l) {
     TRY(dalloc(&E->d_ql[l], E->gd[l].size()));
     TRY(dalloc(&E->d_el[l], E->gd[l].size()));
+    if (l < p.L) TRY(dalloc(&E->d_qb[l], E->gd[l].size()));  // BF16 stencil operand
   }
   TRY(dalloc(&E->d_topwork, 4 * E->gd[p.L].size()));
   TRY(plan_levels(*E));
@@ -666,7 +671,7 @@ msm::G2GArgs g2g_levels(const msm::G2GArgs &A, int first, int last) {
   for (int k = first; k < last && k < A.nlev; ++k) {
     R.lv[R.nlev] = A.lv[k];
     R.lv[R.nlev].task0 = R.ntasks;
-    R.ntasks += (long)A.lv[k].g.x * A.lv[k].ty * A.lv[k].tz * A.lv[k].G;
+    R.ntasks += msm::g2g_level_tasks(A.lv[k]);
     ++R.nlev;
   }
   return R;
@@ -682,7 +687,10 @@ int stage_grids(Engine &E, int precision, int layout, Timing &T, hipStream_t s) 
   const int L = p.L;
   const int pr = precision == MSM_PREC_MIXED ? msm::MSM_G2G_BF16X3 : msm::MSM_G2G_FP64;
   int t = T.open(s);
-  msm::launch_spread(E.P, E.gd[1], E.d_spos, E.d_start, E.d_ql[1], s);
+  // the BF16 stencils read q^l as bf16 hi/lo pairs, written with q^l
+  const bool split = pr == msm::MSM_G2G_BF16X3;
+  msm::launch_spread(E.P, E.gd[1], E.d_spos, E.d_start, E.d_ql[1], s,
+                     split && L >= 2 ? E.d_qb[1] : nullptr);
   T.close(t, "anterpolation", s);
   if (L >= 2) {
     HIP_TRY(hipEventRecord(E.ev_q1, s));
@@ -694,7 +702,8 @@ int stage_grids(Engine &E, int precision, int layout, Timing &T, hipStream_t s) 
   }
   for (int l = 2; l <= L; ++l) {
     t = T.open(s);
-    msm::launch_restrict(E.P, E.gd[l - 1], E.gd[l], E.d_ql[l - 1], E.d_ql[l], s);
+    msm::launch_restrict(E.P, E.gd[l - 1], E.gd[l], E.d_ql[l - 1], E.d_ql[l], s,
+                         split && l < L ? E.d_qb[l] : nullptr);
     T.close(t, "restriction", s);
   }
   if (L >= 3) {  // stencil passes of the levels 2 .. L-1 in one launch

@@ -98,11 +98,13 @@ void launch_pair_count(const DevParams &P, const double4 *spos, const int *scell
 
 // B-spline spreading of charges onto the level-1 grid, one workgroup per
 // grid brick (every grid point written exactly once).
+// q1b (optional): q^1 also as bf16_split pairs.
 void launch_spread(const DevParams &P, GridDims g, const double4 *spos, const int *start,
-                   double *q1, hipStream_t s);
+                   double *q1, hipStream_t s, unsigned *q1b = nullptr);
 // q^l = R q^{l-1} (full weighting with the two-scale stencil).
+// qcb (optional): q^l also as bf16_split pairs.
 void launch_restrict(const DevParams &P, GridDims fine, GridDims coarse, const double *qf,
-                     double *qc, hipStream_t s);
+                     double *qc, hipStream_t s, unsigned *qcb = nullptr);
 // Top level: e^L = IDFT(DFT(q^L) * khat) / M^3.  work: 4 * M^3 doubles.
 // asis != 0 reads q^L with the reference HEAD's padded index map.
 void launch_top(const DevParams &P, GridDims g, const double *qL, const double *khat,
@@ -117,13 +119,33 @@ void launch_level_fp64(const DevParams &P, GridDims g, GridDims gc, GridDims sd,
 constexpr int MSM_G2G_FP64 = 0;    // v_mfma_f64_16x16x4_f64
 constexpr int MSM_G2G_BF16X3 = 1;  // v_mfma_f32_16x16x32_bf16, hi/lo split, FP32 accumulation
 
+constexpr int kG2GMaxTpw = 5;  // BF16: z tiles one wavefront computes per B operand
+
+// A grid value for the BF16 level stencils: its FP32 rounding split into a
+// bf16 high part (low 16 bits, round to nearest even) and the bf16 of the
+// remainder (high 16 bits); values are finite.
+__host__ __device__ inline unsigned bf16_split(double v) {
+  auto bits = [](float f) {
+    union { float f; unsigned u; } c{f};
+    return (c.u + 0x7FFFu + ((c.u >> 16) & 1u)) >> 16;
+  };
+  const float f = (float)v;
+  const unsigned hi = bits(f);
+  union { unsigned u; float f; } h{hi << 16};
+  return hi | (bits(f - h.f) << 16);
+}
+
 struct G2GLevel {
   GridDims g, s;          // level grid and stencil dims
   int nch;                // K chunks per (kx, ky)
   int G;                  // kx groups (one partial grid each)
-  int ty, tz;             // 16-wide tiles along y and z
+  int ty, tz;             // tiles along y (16 rows) and z (nz columns)
+  int nz;                 // output columns per tile (16; BF16: a multiple of 4)
+  int nzseg;              // z segments (BF16: a wavefront takes tpw consecutive z tiles)
+  int tpw;                // z tiles per wavefront (1 .. kG2GMaxTpw; FP64: 1)
   long task0;             // first wavefront of this level in the launch
   const double *q;        // q^l
+  const unsigned *qb;     // q^l as bf16_split pairs (BF16 stencils)
   double *part;           // G x M^3 partial correlations
   const void *bfrag;      // Toeplitz B operands in per-lane layout
 };
@@ -134,6 +156,11 @@ struct G2GArgs {
 };
 // Chooses tiles, K chunks and kx groups for about target_waves wavefronts.
 void plan_g2g_level(int prec, GridDims g, GridDims s, int target_waves, G2GLevel &L);
+// Levels launched together share the chunk count and (BF16) the z tiles per
+// wavefront: equalises them over lv[0..n) and fixes G, nz, nzseg; returns
+// the tasks of the launch.
+long plan_g2g_launch(int prec, G2GLevel *lv, int n, int target_waves);
+long g2g_level_tasks(const G2GLevel &L);
 size_t g2g_bfrag_bytes(int prec, const G2GLevel &L);
 void g2g_bfrag_fill(int prec, const G2GLevel &L, const double *khat, void *host_out);
 size_t g2g_lds_bytes(int prec, const G2GLevel &L);

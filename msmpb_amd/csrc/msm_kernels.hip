@@ -969,7 +969,8 @@ template <int NU, int WPB, bool TRI>
 __global__ __launch_bounds__(64 * WPB) void k_spread_mfma(DevParams P, GridDims g,
                                                           const double4 *__restrict__ spos,
                                                           const int *__restrict__ start,
-                                                          double *__restrict__ q1) {
+                                                          double *__restrict__ q1,
+                                                          unsigned *__restrict__ q1b) {
   constexpr int BS = 64 * WPB;
   // per wavefront: records of kSFlush atoms and a queue of sorted indices
   __shared__ double WX[WPB * kSFlush * kWXs];
@@ -1086,7 +1087,11 @@ __global__ __launch_bounds__(64 * WPB) void k_spread_mfma(DevParams P, GridDims 
       for (int w = 1; w < WPB; ++w) v += WX[(w * 4 + i) * 64 + lane];
       const int x = gx0 + 4 * i + (lane >> 4), col = lane & 15;
       const int y = gy0 + (col >> 2), z = gz0 + (col & 3);
-      if (x < g.x && y < g.y && z < g.z) q1[((long)x * g.y + y) * g.z + z] = v;
+      if (x < g.x && y < g.y && z < g.z) {
+        const long id = ((long)x * g.y + y) * g.z + z;
+        q1[id] = v;
+        if (q1b) q1b[id] = bf16_split(v);
+      }
     }
   }
 }
@@ -1105,7 +1110,8 @@ constexpr int kRBS = 64;  // one wavefront per brick: the work is small, and few
 template <int NU>
 __global__ __launch_bounds__(kRBS) void k_restrict(DevParams P, GridDims f, GridDims c,
                                                   const double *__restrict__ qf,
-                                                  double *__restrict__ qc) {
+                                                  double *__restrict__ qc,
+                                                  unsigned *__restrict__ qcb) {
   // coarse brick of kRB^3 points; fine indices 2m + n - H, n in [0, NU]
   constexpr int H = NU / 2, FW = 2 * (kRB - 1) + NU + 1;
   __shared__ double fs[FW * FW * FW];
@@ -1147,7 +1153,9 @@ __global__ __launch_bounds__(kRBS) void k_restrict(DevParams P, GridDims f, Grid
       double acc = 0.0;
 #pragma unroll
       for (int n = 0; n <= NU; ++n) acc += P.J[n] * t2[((2 * cx + n) * kRB + cy) * kRB + cz];
-      qc[((long)x * c.y + y) * c.z + z] = acc;
+      const long id = ((long)x * c.y + y) * c.z + z;
+      qc[id] = acc;
+      if (qcb) qcb[id] = bf16_split(acc);
     }
   }
 }
@@ -1990,28 +1998,28 @@ void launch_pair_count(const DevParams &P, const double4 *spos, const int *scell
 }
 
 void launch_spread(const DevParams &P, GridDims g, const double4 *spos, const int *start,
-                   double *q1, hipStream_t s) {
+                   double *q1, hipStream_t s, unsigned *q1b) {
   unsigned nb = (unsigned)(((g.x + kSX - 1) / kSX) * ((g.y + kSY - 1) / kSY) * ((g.z + kSZ - 1) / kSZ));
   if (nb >= kSpreadFine) {
 #define CALL(NU)                                                    \
-  if (P.tri) k_spread_mfma<NU, 1, true><<<nb, 64, 0, s>>>(P, g, spos, start, q1); \
-  else k_spread_mfma<NU, 1, false><<<nb, 64, 0, s>>>(P, g, spos, start, q1)
+  if (P.tri) k_spread_mfma<NU, 1, true><<<nb, 64, 0, s>>>(P, g, spos, start, q1, q1b); \
+  else k_spread_mfma<NU, 1, false><<<nb, 64, 0, s>>>(P, g, spos, start, q1, q1b)
     MSM_NU_SWITCH(P.nu, CALL)
 #undef CALL
   } else {
 #define CALL(NU)                                                      \
-  if (P.tri) k_spread_mfma<NU, 4, true><<<nb, 256, 0, s>>>(P, g, spos, start, q1); \
-  else k_spread_mfma<NU, 4, false><<<nb, 256, 0, s>>>(P, g, spos, start, q1)
+  if (P.tri) k_spread_mfma<NU, 4, true><<<nb, 256, 0, s>>>(P, g, spos, start, q1, q1b); \
+  else k_spread_mfma<NU, 4, false><<<nb, 256, 0, s>>>(P, g, spos, start, q1, q1b)
     MSM_NU_SWITCH(P.nu, CALL)
 #undef CALL
   }
 }
 
 void launch_restrict(const DevParams &P, GridDims fine, GridDims coarse, const double *qf,
-                     double *qc, hipStream_t s) {
+                     double *qc, hipStream_t s, unsigned *qcb) {
   const unsigned nb = (unsigned)(((coarse.x + kRB - 1) / kRB) * ((coarse.y + kRB - 1) / kRB) *
                                  ((coarse.z + kRB - 1) / kRB));
-#define CALL(NU) k_restrict<NU><<<nb, kRBS, 0, s>>>(P, fine, coarse, qf, qc)
+#define CALL(NU) k_restrict<NU><<<nb, kRBS, 0, s>>>(P, fine, coarse, qf, qc, qcb)
   MSM_NU_SWITCH(P.nu, CALL)
 #undef CALL
 }

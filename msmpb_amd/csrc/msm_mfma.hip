@@ -23,6 +23,7 @@
 // potential (forcefield1.c:530-567) level by level, deterministically.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -187,6 +188,134 @@ __global__ __launch_bounds__(64) void k_g2g_tiles(G2GArgs args) {
   }
 }
 
+// BF16 hi/lo level stencils.  A workgroup of four wavefronts takes one
+// (plane x, 16-row band, segment of T z tiles, kx group): per kx the four
+// stage the input slab together (q^l as bf16_split pairs), then wavefront w
+// runs the rows ky = w, w + 4, ... of the stencil over all T tiles -- the
+// B operands of a (kx, ky) are loaded once and serve the T tiles, whose K
+// windows (32 NCH columns starting nz apart; nz a multiple of 4 keeps the
+// 8-byte LDS reads of the A operand aligned) lie in the slab.  Columns >= nz
+// of a tile's 16 are dropped (nz + s_z - 1 fits the chunks).  The four
+// partial tiles are summed through LDS in a fixed order.
+constexpr int kRowWaves = 1;
+constexpr int kSlabU = 16;  // slab elements per thread and batch (loads in flight)
+
+template <int NCH, int T>
+__global__ __launch_bounds__(64 * kRowWaves) void k_g2g_rows(G2GArgs args) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  constexpr int KC = 32, KPL = 8, PAD = 8, BS = 64 * kRowWaves;
+  long task = blockIdx.x;
+  int li = 0;
+  while (li + 1 < args.nlev && task >= args.lv[li + 1].task0) ++li;
+  const G2GLevel &L = args.lv[li];
+  long t = task - L.task0;
+  const int grp = t % L.G;
+  t /= L.G;
+  const int zs = t % L.nzseg;
+  t /= L.nzseg;
+  const int yt = t % L.ty;
+  const int x = (int)(t / L.ty);
+  const GridDims g = L.g, s = L.s;
+  const int nz = L.nz;
+  const int per = (s.x + L.G - 1) / L.G;
+  const int kx0 = grp * per, kx1 = min(s.x, kx0 + per);
+  const int y0 = yt * 16, z0 = zs * T * nz;
+  const int R = 16 + s.y - 1, W = (T - 1) * nz + NCH * KC, Wp = W + PAD;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);  // uniform over the wavefront
+  const int row = lane & 15, kg = lane >> 4;
+  uint16_t *const shi = reinterpret_cast<uint16_t *>(lds_raw);
+  uint16_t *const slo = shi + R * Wp;
+  const int yb = y0 - s.y / 2, zb = z0 - s.z / 2;
+  const float igy = 1.0f / g.y, igz = 1.0f / g.z, iW = 1.0f / W;
+  auto wrapf = [](int v, int m, float im) { return v - m * (int)floorf((v + 0.5f) * im); };
+  f32x4 acc[T];
+#pragma unroll
+  for (int tt = 0; tt < T; ++tt) acc[tt] = f32x4{0.f, 0.f, 0.f, 0.f};
+  for (int kx = kx0; kx < kx1; ++kx) {
+    const int xi = wrapm(x + kx - s.x / 2, g.x);
+    const unsigned *plane = L.qb + (long)xi * g.y * g.z;
+    __syncthreads();  // previous slab consumed
+    const int nel = R * W;
+    for (int e0 = tid; e0 < nel; e0 += BS * kSlabU) {
+      unsigned v[kSlabU];
+      int at[kSlabU];
+#pragma unroll
+      for (int u = 0; u < kSlabU; ++u) {
+        const int e = min(e0 + BS * u, nel - 1);  // past the end: a valid address, not stored
+        const int r = (int)((e + 0.5f) * iW), cc = e - r * W;  // e / W (exact: e < 2^20)
+        at[u] = r * Wp + cc;
+        v[u] = plane[(long)wrapf(yb + r, g.y, igy) * g.z + wrapf(zb + cc, g.z, igz)];
+      }
+#pragma unroll
+      for (int u = 0; u < kSlabU; ++u) {
+        if (e0 + BS * u < nel) {
+          shi[at[u]] = (uint16_t)(v[u] & 0xFFFFu);
+          slo[at[u]] = (uint16_t)(v[u] >> 16);
+        }
+      }
+    }
+    __syncthreads();
+    BRow<MSM_G2G_BF16X3, NCH> bnext;
+    if (wv < s.y) bnext.load(L.bfrag, (long)(kx * s.y + wv) * NCH, lane);
+    for (int ky = wv; ky < s.y; ky += kRowWaves) {
+      const BRow<MSM_G2G_BF16X3, NCH> bcur = bnext;
+      if (ky + kRowWaves < s.y) bnext.load(L.bfrag, (long)(kx * s.y + ky + kRowWaves) * NCH, lane);
+#pragma unroll
+      for (int ch = 0; ch < NCH; ++ch) {
+        const bf16x8 bhi = __builtin_bit_cast(bf16x8, bcur.hi[ch]);
+        const bf16x8 blo = __builtin_bit_cast(bf16x8, bcur.lo[ch]);
+        bf16x8 ahi[T], alo[T];
+#pragma unroll
+        for (int tt = 0; tt < T; ++tt) {
+          const int off = (row + ky) * Wp + tt * nz + ch * KC + kg * KPL;  // multiple of 4
+          const uint2 h0 = *reinterpret_cast<const uint2 *>(shi + off);
+          const uint2 h1 = *reinterpret_cast<const uint2 *>(shi + off + 4);
+          const uint2 l0 = *reinterpret_cast<const uint2 *>(slo + off);
+          const uint2 l1 = *reinterpret_cast<const uint2 *>(slo + off + 4);
+          ahi[tt] = __builtin_bit_cast(bf16x8, u32x4{h0.x, h0.y, h1.x, h1.y});
+          alo[tt] = __builtin_bit_cast(bf16x8, u32x4{l0.x, l0.y, l1.x, l1.y});
+        }
+#pragma unroll
+        for (int tt = 0; tt < T; ++tt)
+          acc[tt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alo[tt], bhi, acc[tt], 0, 0, 0);
+#pragma unroll
+        for (int tt = 0; tt < T; ++tt)
+          acc[tt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi[tt], blo, acc[tt], 0, 0, 0);
+#pragma unroll
+        for (int tt = 0; tt < T; ++tt)
+          acc[tt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi[tt], bhi, acc[tt], 0, 0, 0);
+      }
+    }
+  }
+  // the four wavefronts' tiles, summed in a fixed order by wavefront 0
+  float *red = reinterpret_cast<float *>(lds_raw);
+  __syncthreads();
+  if (wv > 0) {
+#pragma unroll
+    for (int tt = 0; tt < T; ++tt)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) red[(((wv - 1) * T + tt) * 4 + i) * 64 + lane] = acc[tt][i];
+  }
+  __syncthreads();
+  if (wv != 0) return;
+  // f32 16x16 result: register i of lane l holds row 4 (l / 16) + i, column l % 16
+  double *out = L.part + (long)grp * g.size() + (long)x * g.y * g.z;
+  if (row < nz) {
+#pragma unroll
+    for (int tt = 0; tt < T; ++tt) {
+      const int z = z0 + tt * nz + row;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        float v = acc[tt][i];
+        for (int w = 1; w < kRowWaves; ++w) v += red[(((w - 1) * T + tt) * 4 + i) * 64 + lane];
+        const int y = y0 + 4 * kg + i;
+        if (y < g.y && z < g.z) out[(long)y * g.z + z] = (double)v;
+      }
+    }
+  }
+}
+
 // e^l = sum_g part[g] + P e^{l+1} (prolongate, forcefield1.c:530-567), as
 // separable passes over an LDS tile.  Fine index v receives J_{v-2m+H} e_m
 // from every coarse m with |v - 2m| <= H (H = NU/2), periodic.  A workgroup
@@ -269,19 +398,78 @@ __global__ __launch_bounds__(256) void k_prolong_sum(DevParams P, GridDims g, Gr
 // ---------------------------------------------------------------- host side
 int g2g_chunk_k(int prec) { return prec == MSM_G2G_FP64 ? Cfg<MSM_G2G_FP64>::KC : Cfg<MSM_G2G_BF16X3>::KC; }
 
+// BF16 tile width for nch chunks: the widest multiple of 4 up to 16 whose K
+// window nz + s_z - 1 fits 32 nch columns (0 if none).
+static int bf16_nz(int nch, int sz) {
+  const int w = std::min(16, 32 * nch - sz + 1);
+  return w >= 4 ? w & ~3 : 0;
+}
+
 void plan_g2g_level(int prec, GridDims g, GridDims s, int target_waves, G2GLevel &L) {
   int KC = g2g_chunk_k(prec);
   L.g = g;
   L.s = s;
-  L.nch = (16 + s.z - 1 + KC - 1) / KC;
   L.ty = (g.y + 15) / 16;
-  L.tz = (g.z + 15) / 16;
+  if (prec == MSM_G2G_FP64) {
+    L.nz = 16;
+    L.nch = (16 + s.z - 1 + KC - 1) / KC;
+  } else {  // fewest chunks per output column
+    L.nch = 0;
+    for (int c = 1; c <= g2g_max_nch(prec); ++c) {
+      const int w = bf16_nz(c, s.z);
+      if (w > 0 && (L.nch == 0 || (double)c / w < (double)L.nch / L.nz - 1e-12)) L.nch = c, L.nz = w;
+    }
+    if (L.nch == 0) L.nch = g2g_max_nch(prec) + 1, L.nz = 16;  // too wide: refused by the caller
+  }
+  L.tz = (g.z + L.nz - 1) / L.nz;
+  L.tpw = 1;
+  L.nzseg = L.tz;
   long tiles = (long)g.x * L.ty * L.tz;
   long G = (target_waves + tiles - 1) / tiles;
   if (G < 1) G = 1;
   if (G > s.x) G = s.x;
   int per = (s.x + (int)G - 1) / (int)G;
   L.G = (s.x + per - 1) / per;  // no empty groups
+}
+
+long g2g_level_tasks(const G2GLevel &L) { return (long)L.g.x * L.ty * L.nzseg * L.G; }
+
+long plan_g2g_launch(int prec, G2GLevel *lv, int n, int target_waves) {
+  int nch = 0;
+  for (int k = 0; k < n; ++k) nch = std::max(nch, lv[k].nch);
+  int tpw = 1;
+  long big = -1;  // z tiles per wavefront: fewest wasted tiles on the largest level
+  for (int k = 0; k < n; ++k) {
+    G2GLevel &L = lv[k];
+    L.nch = nch;
+    if (prec == MSM_G2G_BF16X3 && nch <= g2g_max_nch(prec)) {
+      L.nz = bf16_nz(nch, L.s.z);
+      L.tz = (L.g.z + L.nz - 1) / L.nz;
+      if (L.g.size() > big) {
+        big = L.g.size();
+        int best = -1;
+        for (int t = 1; t <= kG2GMaxTpw; ++t) {
+          const int waste = (L.tz + t - 1) / t * t - L.tz;
+          if (best < 0 || waste <= best) best = waste, tpw = t;
+        }
+      }
+    }
+  }
+  long tasks = 0;
+  for (int k = 0; k < n; ++k) {
+    G2GLevel &L = lv[k];
+    L.tpw = tpw;
+    L.nzseg = (L.tz + tpw - 1) / tpw;
+    const long units = (long)L.g.x * L.ty * L.nzseg;
+    long G = (target_waves + units - 1) / units;
+    if (G < 1) G = 1;
+    if (G > L.s.x) G = L.s.x;
+    const int per = (L.s.x + (int)G - 1) / (int)G;
+    L.G = (L.s.x + per - 1) / per;
+    L.task0 = tasks;
+    tasks += g2g_level_tasks(L);
+  }
+  return tasks;
 }
 
 size_t g2g_bfrag_elems(int prec, const G2GLevel &L) {
@@ -366,7 +554,10 @@ void launch_g2g_bfrag_fill(int prec, const G2GLevel &L, const double *khat, void
 size_t g2g_lds_bytes(int prec, const G2GLevel &L) {
   int R = 16 + L.s.y - 1;
   if (prec == MSM_G2G_FP64) return sizeof(double) * R * (L.nch * 4 + Cfg<MSM_G2G_FP64>::PAD);
-  return 2 * sizeof(uint16_t) * R * (L.nch * 32 + Cfg<MSM_G2G_BF16X3>::PAD);
+  const size_t slab = 2 * sizeof(uint16_t) * R *
+                      ((L.tpw - 1) * L.nz + L.nch * 32 + Cfg<MSM_G2G_BF16X3>::PAD);
+  const size_t red = sizeof(float) * (kRowWaves - 1) * L.tpw * 4 * 64;  // tile reduction
+  return std::max(slab, red);
 }
 
 template <int PREC, int NCH>
@@ -387,12 +578,13 @@ void launch_g2g_tiles(int prec, G2GArgs args, size_t lds, hipStream_t s) {
 #undef C
     }
   } else {
-    switch (nch) {
-      case 1: g2g_launch<MSM_G2G_BF16X3, 1>(args, lds, s); break;
-      case 2: g2g_launch<MSM_G2G_BF16X3, 2>(args, lds, s); break;
-      case 3: g2g_launch<MSM_G2G_BF16X3, 3>(args, lds, s); break;
-      case 4: g2g_launch<MSM_G2G_BF16X3, 4>(args, lds, s); break;
-    }
+    const int T = args.lv[0].tpw;
+#define R(N, TT) \
+  if (nch == N && T == TT) k_g2g_rows<N, TT><<<(unsigned)args.ntasks, 64 * kRowWaves, lds, s>>>(args);
+#define RT(N) R(N, 1) R(N, 2) R(N, 3) R(N, 4) R(N, 5)
+    RT(1) RT(2) RT(3) RT(4)
+#undef RT
+#undef R
   }
 }
 
