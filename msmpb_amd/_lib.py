@@ -190,6 +190,8 @@ def lib():
     L.FF_set_estErr.argtypes = [FFP, ctypes.c_double]
     L.FF_set_topGridDim.argtypes = [FFP, _ip]
     L.FF_set_FFT.argtypes = [FFP, ctypes.c_bool]
+    L.FF_get_FFT.argtypes = [FFP]
+    L.FF_get_FFT.restype = ctypes.c_bool
     L.FF_build.argtypes = [FFP, ctypes.c_int, _dp, _dp, ctypes.c_double]
     L.FF_rebuild.argtypes = [FFP, _dp, _dp, ctypes.c_double]
     L.FF_energy.restype = ctypes.c_double

@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include <chrono>
+#include <climits>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -181,6 +182,7 @@ struct FFX {
   int timing = 0;
   int method = MSM_METHOD_MSM;
   double tolDir = -1;  // PME direct-sum tolerance (<= 0: the default)
+  double margin = 0;   // neighbour-list margin of the last build or rebuild
   bool built = false;
   // host copies behind the public FF pointers
   std::vector<double> h_tau, h_Q, h_J, h_aCut, h_cL[3];
@@ -208,13 +210,65 @@ int dalloc(T **p, size_t n) {
     if (r_ != MSM_OK) return r_; \
   } while (0)
 
-// The pair-search cell grid and its neighbour cell columns.  Cells are at
-// least a0/3 wide (a0/2 where that would leave fewer than about two atoms per
-// cell).  A column (dx, dy, dzlo, dzhi) lists the offsets dz in [dzlo, dzhi]
-// of the cells (dx, dy, dz) whose nearest points lie inside the cutoff; longer
-// ranges are split so that each spans fewer than nc[2] cells (one wrap at
-// most).  Also the self-image constant: sum of g(|k L|) over the images
-// k != 0 of an atom itself inside the cutoff.
+// Neighbour cell columns of the pair-search cell grid for pairs out to
+// radius R.  A column (dx, dy, dzlo, dzhi) lists the offsets dz in
+// [dzlo, dzhi] of the cells (dx, dy, dz) whose nearest points lie inside R;
+// longer ranges are split so that each spans fewer than nc[2] cells (one
+// wrap at most).  Triclinic cells are parallelepipeds: offset d is kept where
+// |A (d / nc)| minus the longest cell diagonal stays below R (the reference's
+// neighbour-cell rule, forcefield0.c:61-107); for each (dx, dy) those dz form
+// one interval (the distance is convex in dz).
+void cell_columns(const msm::Plan &p, const msm::DevParams &P, double R, std::vector<int4> &cols) {
+  cols.clear();
+  if (!p.ortho) {
+    auto vec = [&](double fx, double fy, double fz, double v[3]) {
+      for (int a = 0; a < 3; ++a) v[a] = p.A[a][0] * fx + p.A[a][1] * fy + p.A[a][2] * fz;
+    };
+    double diam = 0.0, v[3];
+    for (int sx = -1; sx <= 1; sx += 2)
+      for (int sy = -1; sy <= 1; sy += 2) {
+        vec((double)sx / P.nc[0], (double)sy / P.nc[1], 1.0 / P.nc[2], v);
+        diam = std::max(diam, std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]));
+      }
+    int lim[3];
+    for (int a = 0; a < 3; ++a) lim[a] = (int)std::ceil((R + diam) / P.cw[a]);
+    for (int dx = -lim[0]; dx <= lim[0]; ++dx)
+      for (int dy = -lim[1]; dy <= lim[1]; ++dy) {
+        int zlo = 1, zhi = 0;
+        for (int dz = -lim[2]; dz <= lim[2]; ++dz) {
+          vec((double)dx / P.nc[0], (double)dy / P.nc[1], (double)dz / P.nc[2], v);
+          if (std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]) - diam < R) {
+            if (zlo > zhi) zlo = dz;
+            zhi = dz;
+          }
+        }
+        for (int z0 = zlo; z0 <= zhi; z0 += P.nc[2])
+          cols.push_back(make_int4(dx, dy, z0, std::min(zhi, z0 + P.nc[2] - 1)));
+      }
+    return;
+  }
+  int lim[3];
+  for (int a = 0; a < 3; ++a) lim[a] = (int)std::ceil(R / P.cw[a]);
+  auto gap2 = [&](int d, int a) {
+    double g = (std::abs(d) - 1) * P.cw[a];
+    return g > 0 ? g * g : 0.0;
+  };
+  for (int dx = -lim[0]; dx <= lim[0]; ++dx)
+    for (int dy = -lim[1]; dy <= lim[1]; ++dy) {
+      const double gxy = gap2(dx, 0) + gap2(dy, 1);
+      if (gxy >= R * R) continue;
+      int zmax = 0;
+      while (zmax + 1 <= lim[2] && gxy + gap2(zmax + 1, 2) < R * R) ++zmax;
+      for (int z0 = -zmax; z0 <= zmax; z0 += P.nc[2])
+        cols.push_back(make_int4(dx, dy, z0, std::min(zmax, z0 + P.nc[2] - 1)));
+    }
+}
+
+// The pair-search cell grid, its neighbour cell columns for the cutoff and
+// the self-image constant: sum of g(|A k|) over the images k != 0 of an atom
+// itself inside the cutoff (forcefield1.c:240-321 skips i == j at every
+// image).  Cells are at least a0/3 wide (a0/2 where that would leave fewer
+// than about two atoms per cell).
 void pair_geometry(const msm::Plan &p, msm::DevParams &P, std::vector<int4> &cols) {
   const double a0 = p.aCut[0];
   const bool tri = !p.ortho;
@@ -227,7 +281,6 @@ void pair_geometry(const msm::Plan &p, msm::DevParams &P, std::vector<int4> &col
   const double vol = tri ? p.detA : p.A[0][0] * p.A[1][1] * p.A[2][2];
   const double dens = p.N / vol, w3 = a0 / 3.0;
   const double div = dens * w3 * w3 * w3 >= 2.0 ? 3.0 : 2.0;
-  int lim[3];
   for (int a = 0; a < 3; ++a) {
     double L = W[a];
     int n = (int)std::floor(L / (a0 / div));
@@ -235,85 +288,18 @@ void pair_geometry(const msm::Plan &p, msm::DevParams &P, std::vector<int4> &col
     if (n > 256) n = 256;
     P.nc[a] = n;
     P.cw[a] = L / n;
-    lim[a] = (int)std::ceil(a0 / P.cw[a]);
   }
   P.ncell = P.nc[0] * P.nc[1] * P.nc[2];
-  if (tri) {
-    // cells are parallelepipeds: offset d is kept where |A (d / nc)| minus
-    // the longest cell diagonal stays below a0 (the reference's neighbour-
-    // cell rule, forcefield0.c:61-107); for each (dx, dy) those dz form one
-    // interval (the distance is convex in dz)
-    auto vec = [&](double fx, double fy, double fz, double v[3]) {
-      for (int a = 0; a < 3; ++a) v[a] = p.A[a][0] * fx + p.A[a][1] * fy + p.A[a][2] * fz;
-    };
-    double diam = 0.0, v[3];
-    for (int sx = -1; sx <= 1; sx += 2)
-      for (int sy = -1; sy <= 1; sy += 2) {
-        vec((double)sx / P.nc[0], (double)sy / P.nc[1], 1.0 / P.nc[2], v);
-        diam = std::max(diam, std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]));
-      }
-    for (int a = 0; a < 3; ++a) lim[a] = (int)std::ceil((a0 + diam) / P.cw[a]);
-    cols.clear();
-    for (int dx = -lim[0]; dx <= lim[0]; ++dx)
-      for (int dy = -lim[1]; dy <= lim[1]; ++dy) {
-        int zlo = 1, zhi = 0;
-        for (int dz = -lim[2]; dz <= lim[2]; ++dz) {
-          vec((double)dx / P.nc[0], (double)dy / P.nc[1], (double)dz / P.nc[2], v);
-          if (std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]) - diam < a0) {
-            if (zlo > zhi) zlo = dz;
-            zhi = dz;
-          }
-        }
-        for (int z0 = zlo; z0 <= zhi; z0 += P.nc[2])
-          cols.push_back(make_int4(dx, dy, z0, std::min(zhi, z0 + P.nc[2] - 1)));
-      }
-    // g over the self images: lattice vectors A k inside the cutoff
-    double C = 0.0;
-    int K[3];
-    for (int a = 0; a < 3; ++a) K[a] = (int)std::ceil(a0 / W[a]);
-    for (int kx = -K[0]; kx <= K[0]; ++kx)
-      for (int ky = -K[1]; ky <= K[1]; ++ky)
-        for (int kz = -K[2]; kz <= K[2]; ++kz) {
-          if (!kx && !ky && !kz) continue;
-          vec(kx, ky, kz, v);
-          const double r2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
-          if (r2 >= a0 * a0) continue;
-          if (p.method == msm::kPme) {
-            C += std::erfc(p.beta * std::sqrt(r2)) / std::sqrt(r2);
-            continue;
-          }
-          const double sv = r2 / (a0 * a0) - 1.0;
-          double ts = p.tau[p.nu - 1];
-          for (int k = p.nu - 2; k >= 0; --k) ts = p.tau[k] + sv * ts;
-          C += 1.0 / std::sqrt(r2) - ts / a0;
-        }
-    P.self_img = C;
-    return;
-  }
-  auto gap2 = [&](int d, int a) {
-    double g = (std::abs(d) - 1) * P.cw[a];
-    return g > 0 ? g * g : 0.0;
-  };
-  cols.clear();
-  for (int dx = -lim[0]; dx <= lim[0]; ++dx)
-    for (int dy = -lim[1]; dy <= lim[1]; ++dy) {
-      const double gxy = gap2(dx, 0) + gap2(dy, 1);
-      if (gxy >= a0 * a0) continue;
-      int zmax = 0;
-      while (zmax + 1 <= lim[2] && gxy + gap2(zmax + 1, 2) < a0 * a0) ++zmax;
-      for (int z0 = -zmax; z0 <= zmax; z0 += P.nc[2])
-        cols.push_back(make_int4(dx, dy, z0, std::min(zmax, z0 + P.nc[2] - 1)));
-    }
-  // g over the self images (forcefield1.c:240-321 skips i == j at every image)
-  double C = 0.0;
+  cell_columns(p, P, a0, cols);
+  double C = 0.0, v[3];
   int K[3];
-  for (int a = 0; a < 3; ++a) K[a] = (int)std::ceil(a0 / p.A[a][a]);
+  for (int a = 0; a < 3; ++a) K[a] = (int)std::ceil(a0 / W[a]);
   for (int kx = -K[0]; kx <= K[0]; ++kx)
     for (int ky = -K[1]; ky <= K[1]; ++ky)
       for (int kz = -K[2]; kz <= K[2]; ++kz) {
         if (!kx && !ky && !kz) continue;
-        const double x = kx * p.A[0][0], y = ky * p.A[1][1], z = kz * p.A[2][2];
-        const double r2 = x * x + y * y + z * z;
+        for (int a = 0; a < 3; ++a) v[a] = p.A[a][0] * kx + p.A[a][1] * ky + p.A[a][2] * kz;
+        const double r2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
         if (r2 >= a0 * a0) continue;
         if (p.method == msm::kPme) {
           C += std::erfc(p.beta * std::sqrt(r2)) / std::sqrt(r2);
@@ -712,8 +698,11 @@ int stage_grids(Engine &E, int precision, int layout, Timing &T, hipStream_t s) 
     T.close(t, "grid2grid", s);
   }
   t = T.open(s);
-  msm::launch_top(E.P, E.gd[L], E.d_ql[L], E.d_khat[L], E.d_tw, E.d_topwork, E.d_el[L],
-                  layout == MSM_LAYOUT_ASIS && L >= 2, s);
+  const int top_map = layout != MSM_LAYOUT_ASIS ? 0
+                      : L >= 2                  ? (E.plan.fft ? msm::kTopAsisStride : 0)
+                                                : (E.plan.fft ? 0 : msm::kTopAsisFlat);
+  msm::launch_top(E.P, E.gd[L], E.d_ql[L], E.d_khat[L], E.d_tw, E.d_topwork, E.d_el[L], top_map,
+                  s);
   T.close(t, "grid2grid_top", s);
   for (int l = L - 1; l >= 1; --l) {
     if (l == 1) HIP_TRY(hipStreamWaitEvent(s, E.ev_g1, 0));
@@ -863,7 +852,6 @@ void publish_plan(FFX *x) {
   ff->coeff1 = p.coeff1;
   ff->coeff2 = p.coeff2;
   ff->nlist = nullptr;
-  ff->nlist_len = 0;
   x->h_tau.assign(p.tau, p.tau + p.nu);
   x->h_Q.assign(p.Q, p.Q + p.nu / 2 * p.nu);
   x->h_J.assign(p.J, p.J + p.nu + 1);
@@ -897,16 +885,47 @@ void publish_plan(FFX *x) {
   ff->omegap = x->h_omegap;
 }
 
-int count_pairs_dev(FFX *x, const double *d_pos, const double *d_q, long long *count) {
+// The reference's neighbour-list statistics for pairs out to a = a0 + margin
+// (forcefield0.c:21-36,109-163): the capacity it allocates for the list and
+// the number of pairs i < j whose nearest image lies within a.  The capacity
+// is c N + 2.1 (4/3 pi a^3) N (N - 1) / (2 det A), truncated, with c = 200 in
+// the source as it stands (forcefield0.c:32) and c = 3 in the build that
+// produced the published runs (every nlist_len in results/TableII.json,
+// Figure4.json and Figure6.json): the as-is layout reports the first, the
+// fixed layout the second.
+void publish_nlist_len(FFX *x) {
+  FF *ff = &x->ff;
+  const double a = ff->aCut[0] + x->margin, N = ff->N;
+  const double c = x->layout == MSM_LAYOUT_ASIS ? 200.0 : 3.0;
+  const double v = c * N + 2.1 * ((4.0 / 3.0) * M_PI * a * a * a * 0.5 * N * (N - 1.0) / ff->detA);
+  ff->nlist_len = v < (double)INT_MAX ? (int)v : INT_MAX;
+}
+
+int count_pairs_dev(FFX *x, const double *d_pos, const double *d_q, double margin,
+                    long long *count) {
   Engine &E = *x->eng;
   hipStream_t s = E.stream;
+  const double R = E.plan.aCut[0] + margin;
+  *count = 0;
+  if (!(R > 0.0)) return MSM_OK;
+  const int4 *cols = E.d_sten;
+  int ncol = E.nsten;
+  int4 *d_cols = nullptr;  // columns for a radius other than the cutoff
+  if (margin != 0.0) {
+    std::vector<int4> c;
+    cell_columns(E.plan, E.P, R, c);
+    TRY(dalloc(&d_cols, c.size()));
+    HIP_TRY(hipMemcpy(d_cols, c.data(), c.size() * sizeof(int4), hipMemcpyHostToDevice));
+    cols = d_cols, ncol = (int)c.size();
+  }
   stage_prepare(E, d_pos, d_q, s);
   HIP_TRY(hipMemsetAsync(E.d_count, 0, sizeof(unsigned long long), s));
-  msm::launch_pair_count(E.P, E.d_spos, E.d_scell, E.d_start, E.d_sten, E.nsten, E.d_count, s);
+  msm::launch_pair_count(E.P, E.d_spos, E.d_start, cols, ncol, R * R, E.d_count, s);
   msm::launch_fill_int(E.d_flags, 1, 0, s);  // no energy reduction follows to clear it
   unsigned long long c = 0;
   HIP_TRY(hipMemcpyAsync(&c, E.d_count, sizeof c, hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
+  if (d_cols) hipFree(d_cols);
   *count = (long long)c;
   return MSM_OK;
 }
@@ -919,7 +938,7 @@ int stage_host_inputs(FFX *x, double (*position)[3]) {
   return MSM_OK;
 }
 
-int build_impl(FF *ff, int N, double (*position)[3], double edges[3][3]) {
+int build_impl(FF *ff, int N, double (*position)[3], double edges[3][3], double margin = 0.0) {
   FFX *x = X(ff);
   x->built = false;
   g_err.clear();
@@ -942,14 +961,16 @@ int build_impl(FF *ff, int N, double (*position)[3], double edges[3][3]) {
   r.method = x->method == MSM_METHOD_PME ? msm::kPme : msm::kMsm;
   r.tolDir = x->tolDir;
   r.complete_images = x->layout != MSM_LAYOUT_ASIS;
+  r.fft = ff->FFT || x->method == MSM_METHOD_PME;
   msm::Plan plan;
   std::string err;
-  if (!ff->FFT) return fail(MSM_ERR_ARG, "FF_build: only the FFT top level is supported");
   if (x->method == MSM_METHOD_PME && x->layout == MSM_LAYOUT_ASIS)
     return fail(MSM_ERR_ARG, "FF_build: the as-is index layout reproduces the MSM reference's "
                              "quirks and does not apply to the PME method");
   auto t0 = std::chrono::steady_clock::now();
   if (!msm::build_plan(r, plan, err)) return fail(MSM_ERR_ARG, "FF_build: " + err);
+  if (!r.fft && r.Mtop[0] <= 0 && plan.L != (r.L > 0 ? r.L : 1))
+    std::fprintf(stderr, "L=%d is replaced by L=%d\n", r.L > 0 ? r.L : 1, plan.L);
   double tstencil =
       std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   // the reference's FF_build writes the parameters its rules chose into the
@@ -972,13 +993,15 @@ int build_impl(FF *ff, int N, double (*position)[3], double edges[3][3]) {
   long long cnt = 0;
   auto t1 = std::chrono::steady_clock::now();
   int rc = stage_host_inputs(x, position);
-  if (rc == MSM_OK) rc = count_pairs_dev(x, x->eng->d_pos, x->eng->d_q, &cnt);
+  if (rc == MSM_OK) rc = count_pairs_dev(x, x->eng->d_pos, x->eng->d_q, margin, &cnt);
   if (rc != MSM_OK) {
     x->built = false;
     return rc;
   }
   ff->time_nlist = std::chrono::duration<double>(std::chrono::steady_clock::now() - t1).count();
   ff->nlist_interaction_count = (int)cnt;
+  x->margin = margin;
+  publish_nlist_len(x);
   return MSM_OK;
 }
 
@@ -1101,10 +1124,10 @@ void FF_set_estErr(FF *ff, double estErr) { ff->estErr = estErr; }
 double FF_get_estErr(FF *ff) { return ff->estErr; }
 void FF_set_cutoff(FF *ff, double cutoff) { ff->cutoff = cutoff; }
 double FF_get_cutoff(FF *ff) { return ff->cutoff; }
-void FF_set_FFT(FF *ff, bool FFT) {
-  if (!FFT) std::fprintf(stderr, "msmpb_hip: only the FFT top level is implemented\n");
-  ff->FFT = FFT;
-}
+// false selects the reference's DFT build (forcefield0.c:209-215): its own
+// default grid and level rules, no rounding to 7-smooth sizes; the top level
+// is the same circular convolution either way.
+void FF_set_FFT(FF *ff, bool FFT) { ff->FFT = FFT; }
 int FF_get_orderAcc(FF *ff) { return ff->orderAcc; }
 int FF_get_maxLevel(FF *ff) { return ff->maxLevel; }
 void FF_get_topGridDim(FF *ff, int topGridDim[3]) {
@@ -1112,15 +1135,15 @@ void FF_get_topGridDim(FF *ff, int topGridDim[3]) {
 }
 bool FF_get_FFT(FF *ff) { return ff->FFT; }
 
+// margin widens only the neighbour-list statistics, as in the reference (pairs
+// are re-binned every evaluation, so no list is kept).
 void FF_build(FF *ff, int N, double (*position)[3], double edges[3][3], double margin) {
-  (void)margin;  // pairs are re-binned every evaluation
   if (!ff) return;
-  if (build_impl(ff, N, position, edges) != MSM_OK)
+  if (build_impl(ff, N, position, edges, margin) != MSM_OK)
     std::fprintf(stderr, "msmpb_hip: %s\n", g_err.c_str());
 }
 
 void FF_rebuild(FF *ff, double edges[3][3], double (*position)[3], double margin) {
-  (void)margin;
   if (check_built(ff) != MSM_OK) {
     std::fprintf(stderr, "msmpb_hip: %s\n", g_err.c_str());
     return;
@@ -1138,8 +1161,11 @@ void FF_rebuild(FF *ff, double edges[3][3], double (*position)[3], double margin
   if (position) {
     long long cnt = 0;
     if (stage_host_inputs(x, position) == MSM_OK &&
-        count_pairs_dev(x, E.d_pos, E.d_q, &cnt) == MSM_OK)
+        count_pairs_dev(x, E.d_pos, E.d_q, margin, &cnt) == MSM_OK) {
       ff->nlist_interaction_count = (int)cnt;
+      x->margin = margin;
+      publish_nlist_len(x);
+    }
   }
 }
 
@@ -1363,6 +1389,7 @@ int FF_set_layout_compat(FF *ff, int layout) {
       }
       publish_plan(x);
     }
+    publish_nlist_len(x);
   }
   return MSM_OK;
 }
@@ -1526,7 +1553,7 @@ int msm_count_pairs(FF *ff, double (*position)[3], long long *count) {
   if (!position || !count) return fail(MSM_ERR_ARG, "msm_count_pairs: NULL argument");
   FFX *x = X(ff);
   TRY(stage_host_inputs(x, position));
-  return count_pairs_dev(x, x->eng->d_pos, x->eng->d_q, count);
+  return count_pairs_dev(x, x->eng->d_pos, x->eng->d_q, 0.0, count);
 }
 
 

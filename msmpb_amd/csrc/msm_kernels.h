@@ -92,9 +92,10 @@ void launch_pair_fp64(const DevParams &P, const double4 *spos, const int *sidx, 
 void launch_pair_fp32(const DevParams &P, const double4 *spos, const int *sidx, const int *start,
                       const int4 *cols, int ncol, double *F, double *phi, double *epart,
                       hipStream_t s);
-// Pair count within the cutoff (i < j, all images counted once per pair).
-void launch_pair_count(const DevParams &P, const double4 *spos, const int *scell, const int *start,
-                       const int4 *cols, int ncol, unsigned long long *count, hipStream_t s);
+// Pairs i < j whose nearest image lies within sqrt(r2max), added to *count;
+// cols are cell columns built for that radius.
+void launch_pair_count(const DevParams &P, const double4 *spos, const int *start, const int4 *cols,
+                       int ncol, double r2max, unsigned long long *count, hipStream_t s);
 
 // B-spline spreading of charges onto the level-1 grid, one workgroup per
 // grid brick (every grid point written exactly once).
@@ -106,7 +107,11 @@ void launch_spread(const DevParams &P, GridDims g, const double4 *spos, const in
 void launch_restrict(const DevParams &P, GridDims fine, GridDims coarse, const double *qf,
                      double *qc, hipStream_t s, unsigned *qcb = nullptr);
 // Top level: e^L = IDFT(DFT(q^L) * khat) / M^3.  work: 4 * M^3 doubles.
-// asis != 0 reads q^L with the reference HEAD's padded index map.
+// asis selects how q^L is read: directly (0), or by one of the reference
+// HEAD's index maps -- the FFT build's padded stride and offset at L >= 2
+// (kTopAsisStride) or the DFT build's flat read of the padded level-1 array
+// at L = 1 (kTopAsisFlat).
+constexpr int kTopAsisStride = 1, kTopAsisFlat = 2;
 void launch_top(const DevParams &P, GridDims g, const double *qL, const double *khat,
                 const double *tw, double *work, double *eL, int asis, hipStream_t s);
 // e^l = P e^{l+1} + K_l * q^l (prolongation fused with the stencil pass);

@@ -355,7 +355,7 @@ __global__ __launch_bounds__(256) void k_cell_sort(DevParams P, const int *__res
 // Periodic images of an atom itself (boxes narrower than the cutoff) are
 // evaluated like any other neighbour, and their sum q_i * self_img, which
 // depends on the box only, is taken off phi_i; their forces cancel pairwise.
-constexpr int kPairBS = 256;    // workgroup of the pair-count kernel
+constexpr int kPairBS = 256;    // atoms per pair-partial block (energy partial buffer sizing)
 constexpr int kPassAtoms = 8;   // home atoms per pass
 constexpr int kSegMax = 128;    // atom ranges per round: 64 columns x <= 2 pieces
 
@@ -776,48 +776,90 @@ __global__ __launch_bounds__(64) void k_pair_cell(DevParams P, const double4 *__
   if (lane == 0) epart[c] = esum;
 }
 
-// Pairs i < j whose nearest image lies inside the cutoff (the reference's
-// neighbour-list statistic), by the same cell columns.
-__global__ __launch_bounds__(kPairBS) void k_pair_count(
-    DevParams P, const double4 *__restrict__ spos, const int *__restrict__ scell,
-    const int *__restrict__ start, const int4 *__restrict__ cols, int ncol,
-    unsigned long long *__restrict__ count) {
-  int i = blockIdx.x * kPairBS + threadIdx.x;
-  double n = 0;
-  if (i < P.N) {
-    double4 ai = spos[i];
-    int c = scell[i];
-    int cz = c % P.nc[2], cy = (c / P.nc[2]) % P.nc[1], cx = c / (P.nc[1] * P.nc[2]);
-    for (int o = 0; o < ncol; ++o) {
-      const int4 col = cols[o];
-      for (int dz = col.z; dz <= col.w; ++dz) {
-        int ox = cx + col.x, oy = cy + col.y, oz = cz + dz;
-        int kx = floordiv(ox, P.nc[0]), ky = floordiv(oy, P.nc[1]), kz = floordiv(oz, P.nc[2]);
-        ox -= kx * P.nc[0], oy -= ky * P.nc[1], oz -= kz * P.nc[2];
-        double bx, by, bz;
-        lattice_rt(P, kx, ky, kz, bx, by, bz);
-        bx -= ai.x, by -= ai.y, bz -= ai.z;
-        int cn = (ox * P.nc[1] + oy) * P.nc[2] + oz;
-        for (int j = start[cn]; j < start[cn + 1]; ++j) {
-          if (j <= i) continue;
-          double4 aj = spos[j];
-          double dx = aj.x + bx, dy = aj.y + by, dz2 = aj.z + bz;
-          // count each pair once, at its nearest image: the one with every
-          // component in [-L/2, L/2), r - floor(r/L + 1/2) L as in the
-          // reference (forcefield0.c:140-148), so that of two images tied at
-          // exactly half a box only one is counted
-          const double d[3] = {dx, dy, dz2};
-          double sd[3];
-          to_frac_rt(P, d, sd);
-          if (floor(sd[0] + 0.5) != 0.0 || floor(sd[1] + 0.5) != 0.0 || floor(sd[2] + 0.5) != 0.0)
-            continue;
-          if (dx * dx + dy * dy + dz2 * dz2 < P.a02) n += 1.0;
-        }
+// Pairs i < j (cell-sorted order) whose nearest image lies within radius
+// sqrt(r2max): the reference's neighbour-list statistic (forcefield0.c:109-160;
+// radius a0 + margin), counted by cell columns built for that radius.  One wavefront per home cell; a round's
+// columns (up to 64) become at most two contiguous index ranges each, laid
+// out by a prefix sum, and the lanes take the candidates in turn, each
+// against every home atom.  A pair is counted at the image whose fractional
+// separation has every component in [-1/2, 1/2), r - floor(r/L + 1/2) L as
+// in the reference (forcefield0.c:140-148), so that of two images tied at
+// exactly half a box only one is counted.
+__global__ __launch_bounds__(64) void k_pair_count(DevParams P, const double4 *__restrict__ spos,
+                                                   const int *__restrict__ start,
+                                                   const int4 *__restrict__ cols, int ncol,
+                                                   double r2max,
+                                                   unsigned long long *__restrict__ count) {
+  __shared__ int seg_off[kSegMax + 1], seg_beg[kSegMax];
+  __shared__ double seg_sx[kSegMax], seg_sy[kSegMax], seg_sz[kSegMax];
+  const int c = blockIdx.x, lane = threadIdx.x;
+  const int b0 = start[c], n = start[c + 1] - b0;
+  if (n == 0) return;  // wavefront-uniform
+  const int cz = c % P.nc[2], cy = (c / P.nc[2]) % P.nc[1], cx = c / (P.nc[1] * P.nc[2]);
+  unsigned cnt = 0;
+  for (int c0 = 0; c0 < ncol; c0 += 64) {
+    int beg0 = 0, cnt0 = 0, beg1 = 0, cnt1 = 0;
+    double sx = 0, sy = 0, sz0 = 0, sz1 = 0, sx1 = 0, sy1 = 0;
+    if (c0 + lane < ncol) {
+      const int4 col = cols[c0 + lane];  // dx, dy, dzlo, dzhi (dzhi - dzlo < nc[2])
+      int X = cx + col.x, Y = cy + col.y;
+      const int kx = floordiv(X, P.nc[0]), ky = floordiv(Y, P.nc[1]);
+      X -= kx * P.nc[0], Y -= ky * P.nc[1];
+      const int base = (X * P.nc[1] + Y) * P.nc[2];
+      const int Z0 = cz + col.z, Z1 = cz + col.w;
+      const int kz0 = floordiv(Z0, P.nc[2]);
+      const int e0 = min(Z1, (kz0 + 1) * P.nc[2] - 1);
+      beg0 = start[base + Z0 - kz0 * P.nc[2]];
+      cnt0 = start[base + e0 - kz0 * P.nc[2] + 1] - beg0;
+      lattice_rt(P, kx, ky, kz0, sx, sy, sz0);
+      if (e0 < Z1) {  // the rest lies in the next periodic image along z
+        beg1 = start[base];
+        cnt1 = start[base + Z1 - (kz0 + 1) * P.nc[2] + 1] - beg1;
+        lattice_rt(P, kx, ky, kz0 + 1, sx1, sy1, sz1);
       }
     }
+    const int mine = ((int)(cnt0 > 0) + (int)(cnt1 > 0)) * (1 << 24) + cnt0 + cnt1;
+    int incl = mine;
+    for (int off = 1; off < 64; off <<= 1) {
+      const int t = __shfl_up(incl, off, 64);
+      if (lane >= off) incl += t;
+    }
+    const int excl = incl - mine, last = __shfl(incl, 63, 64);
+    int k = excl >> 24;
+    const int o = excl & 0xffffff;
+    if (cnt0 > 0) {
+      seg_off[k] = o, seg_beg[k] = beg0;
+      seg_sx[k] = sx, seg_sy[k] = sy, seg_sz[k] = sz0;
+      ++k;
+    }
+    if (cnt1 > 0) {
+      seg_off[k] = o + cnt0, seg_beg[k] = beg1;
+      seg_sx[k] = sx1, seg_sy[k] = sy1, seg_sz[k] = sz1;
+    }
+    const int total = last & 0xffffff;
+    if (lane == 0) seg_off[last >> 24] = total;  // sentinel
+    __syncthreads();
+    int sl = 0;
+    for (int g = lane; g < total; g += 64) {
+      while (seg_off[sl + 1] <= g) ++sl;  // the sentinel total > g ends it
+      const int j = seg_beg[sl] + g - seg_off[sl];
+      const double4 aj = spos[j];
+      const double lx = seg_sx[sl], ly = seg_sy[sl], lz = seg_sz[sl];
+      for (int h = 0; h < n; ++h) {
+        const int i = b0 + h;
+        if (j <= i) continue;
+        const double4 ai = spos[i];
+        const double d[3] = {aj.x + (lx - ai.x), aj.y + (ly - ai.y), aj.z + (lz - ai.z)};
+        if (d[0] * d[0] + d[1] * d[1] + d[2] * d[2] >= r2max) continue;
+        double sd[3];
+        to_frac_rt(P, d, sd);
+        cnt += floor(sd[0] + 0.5) == 0.0 && floor(sd[1] + 0.5) == 0.0 && floor(sd[2] + 0.5) == 0.0;
+      }
+    }
+    __syncthreads();  // the tables are rewritten by the next round
   }
-  double t = block_sum<kPairBS>(n);
-  if (threadIdx.x == 0) atomicAdd(count, (unsigned long long)t);
+  for (int off = 32; off > 0; off >>= 1) cnt += __shfl_down(cnt, off, 64);
+  if (lane == 0 && cnt) atomicAdd(count, (unsigned long long)cnt);
 }
 
 // ---------------------------------------------------------------- spread
@@ -1166,12 +1208,26 @@ __global__ __launch_bounds__(kRBS) void k_restrict(DevParams P, GridDims f, Grid
 // tw holds cos/sin(2 pi j / M_axis) for the three axes back to back.
 __device__ __forceinline__ double top_input(GridDims g, const double *qL, int x, int y, int z,
                                             int asis, int nu) {
-  if (!asis) return qL[((long)x * g.y + y) * g.z + z];
-  // reference HEAD (forcefield1.c:504-515 vs :406-415): unpadded data read
-  // with the padded stride M + nu - 1 and offset nu/2 - 1
-  int o = nu / 2 - 1;
-  long f = ((long)(x + o) * (g.y + nu - 1) + (y + o)) * (g.z + nu - 1) + (z + o);
-  return f < g.size() ? qL[f] : 0.0;
+  if (asis == kTopAsisStride) {
+    // reference HEAD, FFT build (forcefield1.c:504-515 vs :406-415): unpadded
+    // data read with the padded stride M + nu - 1 and offset nu/2 - 1
+    int o = nu / 2 - 1;
+    long f = ((long)(x + o) * (g.y + nu - 1) + (y + o)) * (g.z + nu - 1) + (z + o);
+    return f < g.size() ? qL[f] : 0.0;
+  }
+  if (asis == kTopAsisFlat) {
+    // reference HEAD, DFT build at L = 1 (forcefield1.c:483-486 on the array
+    // of :338-346): the padded level-1 array (dims M + nu - 1, interior at
+    // nu/2 - 1, zero ghosts) read at the flat unpadded index
+    const long f = ((long)x * g.y + y) * g.z + z;
+    const int P1 = g.y + nu - 1, P2 = g.z + nu - 1, o = nu / 2 - 1;
+    const int a = (int)(f / ((long)P1 * P2)) - o, b = (int)((f / P2) % P1) - o,
+              c = (int)(f % P2) - o;
+    return a >= 0 && a < g.x && b >= 0 && b < g.y && c >= 0 && c < g.z
+               ? qL[((long)a * g.y + b) * g.z + c]
+               : 0.0;
+  }
+  return qL[((long)x * g.y + y) * g.z + z];
 }
 
 // One workgroup of 256 threads with LDS sized to the grid (32 bytes per
@@ -1992,9 +2048,9 @@ void launch_pair_fp32(const DevParams &P, const double4 *spos, const int *sidx, 
 #undef CALL
 }
 
-void launch_pair_count(const DevParams &P, const double4 *spos, const int *scell, const int *start,
-                       const int4 *cols, int ncol, unsigned long long *count, hipStream_t s) {
-  k_pair_count<<<pair_blocks(P.N), kPairBS, 0, s>>>(P, spos, scell, start, cols, ncol, count);
+void launch_pair_count(const DevParams &P, const double4 *spos, const int *start, const int4 *cols,
+                       int ncol, double r2max, unsigned long long *count, hipStream_t s) {
+  k_pair_count<<<P.ncell, 64, 0, s>>>(P, spos, start, cols, ncol, r2max, count);
 }
 
 void launch_spread(const DevParams &P, GridDims g, const double4 *spos, const int *start,

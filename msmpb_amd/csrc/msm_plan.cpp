@@ -489,24 +489,42 @@ bool build_plan(const BuildRequest &r, Plan &p, std::string &err) {
   if (r.method == kPme) return build_pme(r, p, err) && check_box(p, err);
   p.estErr = r.estErr > 0 ? r.estErr : 1e-3;
   p.L = r.L > 0 ? r.L : 1;
+  p.fft = r.fft;
+  // determineM (forcefield0.c:247-269): about eta grid points per atom on the
+  // finest level, equal spacing on every axis (eta 5.7 for one level and 0.83
+  // for more with the FFT, 1 for the DFT build)
+  auto determine_M = [&](int L, int Mt[3]) {
+    const double eta = p.fft ? (L >= 2 ? 0.83 : 5.7) : 1.0;
+    double ax = row_norm(p.A, 0), ay = row_norm(p.A, 1), az = row_norm(p.A, 2), av[3] = {ax, ay, az};
+    for (int a = 0; a < 3; ++a)
+      Mt[a] = (int)std::ceil(std::pow(2., 1. - L) * std::pow(eta * r.N, 1. / 3.) * av[a] *
+                             std::pow(ax * ay * az, -1. / 3.));
+  };
+  int Mt[3];
+  if (r.Mtop[0] > 0) {
+    for (int a = 0; a < 3; ++a) Mt[a] = r.Mtop[a];
+  } else if (!p.fft) {
+    // DFT build (forcefield0.c:292-307): the number of levels becomes the
+    // first whose top grid has at most sqrt(N) points
+    for (int l = 1;; ++l) {
+      determine_M(l, Mt);
+      if ((double)Mt[0] * Mt[1] * Mt[2] <= std::sqrt((double)r.N)) {
+        p.L = l;
+        break;
+      }
+    }
+  } else {
+    determine_M(p.L, Mt);
+  }
   if (p.L > kMaxLevel) {
     std::snprintf(buf, sizeof buf, "maxLevel %d exceeds %d", p.L, kMaxLevel);
     err = buf;
     return false;
   }
-  int Mt[3];
-  if (r.Mtop[0] > 0) {
-    for (int a = 0; a < 3; ++a) Mt[a] = r.Mtop[a];
-  } else {  // determineM (forcefield0.c:247-269), FFT build
-    double eta = p.L >= 2 ? 0.83 : 5.7;
-    double ax = row_norm(p.A, 0), ay = row_norm(p.A, 1), az = row_norm(p.A, 2), av[3] = {ax, ay, az};
-    for (int a = 0; a < 3; ++a)
-      Mt[a] = (int)std::ceil(std::pow(2., 1. - p.L) * std::pow(eta * r.N, 1. / 3.) * av[a] *
-                             std::pow(ax * ay * az, -1. / 3.));
-  }
   for (int a = 0; a < 3; ++a) {
     if (Mt[a] < 1) { err = "topGridDim must be positive"; return false; }
-    p.M[p.L][a] = next_7smooth(Mt[a]);
+    // the FFT build rounds up to 7-smooth sizes (forcefield0.c:309-316)
+    p.M[p.L][a] = p.fft ? next_7smooth(Mt[a]) : Mt[a];
   }
   for (int l = p.L - 1; l >= 1; --l)
     for (int a = 0; a < 3; ++a) p.M[l][a] = 2 * p.M[l + 1][a];

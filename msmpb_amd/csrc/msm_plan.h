@@ -26,6 +26,7 @@ struct Plan {
   double A[3][3] = {}, Ai[3][3] = {}, detA = 0;
   bool ortho = true;           // box matrix diagonal
   int nu = 0, L = 0;
+  bool fft = true;             // FF_set_FFT: top level by FFT (else the DFT build's rules)
   int M[kMaxLevel + 1][3] = {};  // level grid dims, l = 1..L
   int S[kMaxLevel + 1][3] = {};  // stencil dims, l = 1..L-1 (s_l = min(dmax, M_l))
   double aCut[kMaxLevel + 1] = {};
@@ -61,6 +62,7 @@ struct BuildRequest {
   int method = kMsm;
   double tolDir = -1;          // PME only
   bool complete_images = true; // level-kernel image range (Plan::kas)
+  bool fft = true;             // FF_set_FFT (forcefield0.c:209-215, 247-316)
 };
 
 // Resolves parameters with the reference's rules (FF_build of

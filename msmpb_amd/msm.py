@@ -43,11 +43,16 @@ class MSM:
     pme1.c) with the same kernels: one grid of top_grid points, the Ewald
     pair term with erfc(beta cutoff) = tol_dir * cutoff (tol_dir <= 0:
     1e-5); levels and est_err do not apply, nu defaults to 6, cutoff to 8.
+    fft=False selects the reference's DFT build (FF_set_FFT(false)): its
+    default grid and level rules, no rounding to 7-smooth sizes.
+    margin widens the neighbour-list statistics (pair_count, nlist_len) to
+    pairs within cutoff + margin, as FF_build's margin does in the reference;
+    the energy does not depend on it.
     """
 
     def __init__(self, positions, charges, edges, nu=-1, levels=-1, top_grid=None, cutoff=-1.0,
                  est_err=-1.0, precision="fp64", layout="fixed", stage_timing=False,
-                 method="msm", tol_dir=-1.0):
+                 method="msm", tol_dir=-1.0, margin=0.0, fft=True):
         for name, val, table in (("precision", precision, PRECISIONS), ("layout", layout, LAYOUTS),
                                  ("method", method, METHODS)):
             if val not in table:
@@ -70,13 +75,14 @@ class MSM:
             L.FF_set_cutoff(self._ff, float(cutoff))
         if est_err is not None and est_err > 0:
             L.FF_set_estErr(self._ff, float(est_err))
+        L.FF_set_FFT(self._ff, bool(fft))
         _check(L.FF_set_precision(self._ff, PRECISIONS[precision]), "precision")
         _check(L.FF_set_layout_compat(self._ff, LAYOUTS[layout]), "layout")
         _check(L.FF_set_stage_timing(self._ff, 1 if stage_timing else 0), "stage timing")
         _check(L.FF_set_method(self._ff, METHODS[method], float(tol_dir)), "method")
         self.precision = precision
         self.method = method
-        L.FF_build(self._ff, self.N, _p(pos), _p(self.edges), 0.0)
+        L.FF_build(self._ff, self.N, _p(pos), _p(self.edges), float(margin))
         err = _lib.last_error()  # the build's own message, before any other call
         if L.FF_get_maxLevel(self._ff) <= 0 or not self._built():
             self.close()
@@ -88,6 +94,7 @@ class MSM:
         self.cutoff = ff.cutoff
         self.nlim = ff.nLim
         self.pair_count = ff.nlist_interaction_count
+        self.nlist_len = ff.nlist_len
         self.beta = L.FF_get_beta(self._ff)
 
     def _built(self):

@@ -50,6 +50,7 @@ struct orc {
   double *khat[MAXL + 1];
   double coeff1, coeff2;
   int compat;
+  int fft;                  /* 0: the reference's DFT build (FF_set_FFT(false)) */
   const double *qref;       /* charges used by the error estimators */
   int pair_count;
   long *pstart;             /* CSR pair list (j > i) */
@@ -350,30 +351,47 @@ static void build_level_kernel(orc_t *o, int l, const double as[3]) {
   o->khat[l] = kh;
 }
 
+/* determineM (forcefield0.c:247-269): eta grid points per atom on the
+ * finest level, equal spacing on every axis; eta = 5.7 (L = 1) or 0.83
+ * (L >= 2) for the FFT build, 1 for the DFT build. */
+static void determine_M(const orc_t *o, int L, int M[3]) {
+  double eta = o->fft ? (L >= 2 ? 0.83 : 5.7) : 1.0;
+  double ax = rownorm(o->A, 0), ay = rownorm(o->A, 1), az = rownorm(o->A, 2);
+  double av[3] = {ax, ay, az};
+  for (int a = 0; a < 3; a++)
+    M[a] = (int)ceil(pow(2., 1. - L) * pow(eta * o->N, 1. / 3.) * av[a] *
+                     pow(ax * ay * az, -1. / 3.));
+}
+
 orc_t *orc_build(int N, const double *q, const double *pos, const double *edges, int nu,
-                 int L, const int *Mtop, double a0, double estErr, int compat, char *err) {
+                 int L, const int *Mtop, double a0, double estErr, int compat, int fft,
+                 char *err) {
   orc_t *o = (orc_t *)calloc(1, sizeof(orc_t));
   o->N = N;
   o->qref = q;
   o->compat = compat;
+  o->fft = fft;
   for (int i = 0; i < 3; i++)
     for (int j = 0; j < 3; j++) o->A[i][j] = edges[3 * i + j];
   o->detA = inv3(o->A, o->Ai);
-  /* parameter rules of FF_build (forcefield0.c:286-367), FFT build */
+  /* parameter rules of FF_build (forcefield0.c:286-367) */
   o->estErr = estErr > 0 ? estErr : 0.001;
   o->L = L > 0 ? L : 1;
-  if (o->L > MAXL) { if (err) sprintf(err, "maxLevel > %d", MAXL); free(o); return NULL; }
   if (Mtop && Mtop[0] > 0) {
     o->M[0] = Mtop[0]; o->M[1] = Mtop[1]; o->M[2] = Mtop[2];
-  } else { /* determineM (forcefield0.c:247-269) */
-    double eta = o->L >= 2 ? 0.83 : 5.7;
-    double ax = rownorm(o->A, 0), ay = rownorm(o->A, 1), az = rownorm(o->A, 2);
-    double av[3] = {ax, ay, az};
-    for (int a = 0; a < 3; a++)
-      o->M[a] = (int)ceil(pow(2., 1. - o->L) * pow(eta * N, 1. / 3.) * av[a] *
-                          pow(ax * ay * az, -1. / 3.));
+  } else if (!fft) {
+    /* DFT build (forcefield0.c:292-307): the first level count whose
+     * determineM grid (eta = 1) has at most sqrt(N) points */
+    for (int l = 1;; l++) {
+      determine_M(o, l, o->M);
+      if ((double)o->M[0] * o->M[1] * o->M[2] <= sqrt((double)N)) { o->L = l; break; }
+    }
+  } else {
+    determine_M(o, o->L, o->M);
   }
-  for (int a = 0; a < 3; a++) o->M[a] = smooth7(o->M[a]);
+  if (o->L > MAXL) { if (err) sprintf(err, "maxLevel > %d", MAXL); free(o); return NULL; }
+  if (fft) /* FFT sizes (forcefield0.c:309-316) */
+    for (int a = 0; a < 3; a++) o->M[a] = smooth7(o->M[a]);
   if (a0 <= 0) {
     if (nu <= 0) {
       double best = INFINITY;
@@ -785,6 +803,18 @@ static void interpolate(const orc_t *o, const double *pos, const double *q, cons
   }
 }
 
+/* Element f (flat) of the reference's padded level-1 array at L = 1: dims
+ * M + nu - 1, the periodic grid at offset nu/2 - 1, zero ghosts
+ * (forcefield1.c:143, 338-346; anterpolate reduces every index modulo M, so
+ * the fold at :370-397 adds only zeros). */
+static double padded_l1_flat(const orc_t *o, size_t f) {
+  const int *g = o->gd[1], nu = o->nu;
+  int P1 = g[1] + nu - 1, P2 = g[2] + nu - 1, of = nu / 2 - 1;
+  int a = (int)(f / ((size_t)P1 * P2)) - of, b = (int)((f / P2) % P1) - of, c = (int)(f % P2) - of;
+  if (a < 0 || a >= g[0] || b < 0 || b >= g[1] || c < 0 || c >= g[2]) return 0.;
+  return o->qg[1][((size_t)a * g[1] + b) * g[2] + c];
+}
+
 /* FF_energy (forcefield1.c:101-235). */
 double orc_energy(orc_t *o, const double *pos, const double *q, double *F, double *phi,
                   double *terms) {
@@ -804,10 +834,11 @@ double orc_energy(orc_t *o, const double *pos, const double *q, double *F, doubl
   /* top level */
   const int *gt = o->gd[L];
   size_t nt = (size_t)gt[0] * gt[1] * gt[2];
-  if (o->compat && L >= 2) {
-    /* HEAD quirk (forcefield1.c:504-515 vs :406-415): the unpadded top grid
-     * sits at the start of a zeroed (M+nu-1)^3 buffer that is read with the
-     * padded stride and an offset of nu/2-1. */
+  if (o->compat && L >= 2 && o->fft) {
+    /* HEAD quirk, FFT build (forcefield1.c:504-515 vs :406-415): the
+     * unpadded top grid sits at the start of a zeroed (M+nu-1)^3 buffer
+     * that is read with the padded stride and an offset of nu/2-1.  (The
+     * DFT build reads that buffer at flat indices, i.e. correctly.) */
     int P[3] = {gt[0] + nu - 1, gt[1] + nu - 1, gt[2] + nu - 1}, of = nu / 2 - 1;
     double *x = (double *)malloc(sizeof(double) * nt);
     for (int i = 0; i < gt[0]; i++)
@@ -816,6 +847,14 @@ double orc_energy(orc_t *o, const double *pos, const double *q, double *F, doubl
           size_t f = ((size_t)(i + of) * P[1] + (j + of)) * P[2] + (k + of);
           x[((size_t)i * gt[1] + j) * gt[2] + k] = f < nt ? o->qg[L][f] : 0.;
         }
+    orc_top_level(o, x, o->eg[L]);
+    free(x);
+  } else if (o->compat && L == 1 && !o->fft) {
+    /* HEAD quirk, DFT build (forcefield1.c:483-486 on the array of
+     * :338-346): DFTg2g reads the padded level-1 array at the unpadded flat
+     * indices */
+    double *x = (double *)malloc(sizeof(double) * nt);
+    for (size_t f = 0; f < nt; f++) x[f] = padded_l1_flat(o, f);
     orc_top_level(o, x, o->eg[L]);
     free(x);
   } else {
@@ -831,17 +870,9 @@ double orc_energy(orc_t *o, const double *pos, const double *q, double *F, doubl
   size_t n1 = (size_t)g1[0] * g1[1] * g1[2];
   double gsum = 0.;
   if (o->compat && L == 1) {
-    /* HEAD quirk (forcefield1.c:196-212): the padded level-1 array
-     * (dims M+nu-1, interior at offset nu/2-1, zero ghosts) is read at the
-     * unpadded flat indices. */
-    int P[3] = {g1[0] + nu - 1, g1[1] + nu - 1, g1[2] + nu - 1}, of = nu / 2 - 1;
-    for (size_t f = 0; f < n1; f++) {
-      int a = (int)(f / ((size_t)P[1] * P[2])), b = (int)((f / P[2]) % P[1]), c = (int)(f % P[2]);
-      double v = 0.;
-      if (a >= of && a < of + g1[0] && b >= of && b < of + g1[1] && c >= of && c < of + g1[2])
-        v = o->qg[1][((size_t)(a - of) * g1[1] + (b - of)) * g1[2] + (c - of)];
-      gsum += v * o->eg[1][f];
-    }
+    /* HEAD quirk (forcefield1.c:196-212): the padded level-1 array is read
+     * at the unpadded flat indices */
+    for (size_t f = 0; f < n1; f++) gsum += padded_l1_flat(o, f) * o->eg[1][f];
   } else {
     for (size_t m = 0; m < n1; m++) gsum += o->qg[1][m] * o->eg[1][m];
   }

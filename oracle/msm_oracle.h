@@ -23,11 +23,13 @@ typedef struct orc orc_t;
 
 /* Build: parameters follow FF_build (forcefield0.c:271-367).  Pass -1 (or
  * NULL for Mtop) to let the reference's auto-selection rules choose.
+ * fft = 0 selects the reference's DFT build (FF_set_FFT(false)): its own
+ * grid and level rules and, with compat, its top-level read.
  * Returns NULL and fills err (if given, >= 256 bytes) on invalid input. */
 orc_t *orc_build(int N, const double *q, const double *pos /* N x 3 */,
                  const double *edges /* 3 x 3 row-major, A[i][j] */,
                  int nu, int L, const int *Mtop, double a0, double estErr,
-                 int compat, char *err);
+                 int compat, int fft, char *err);
 
 /* (Re)build the pair list like neighborlist() with the given margin
  * (forcefield0.c:21-164); orc_build does it with margin 0.  Returns the

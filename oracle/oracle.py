@@ -35,7 +35,8 @@ def lib():
         L = ctypes.CDLL(_LIB_PATH)
         L.orc_build.restype = ctypes.c_void_p
         L.orc_build.argtypes = [ctypes.c_int, _dp, _dp, _dp, ctypes.c_int, ctypes.c_int, _ip,
-                                ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_char_p]
+                                ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                                ctypes.c_char_p]
         L.orc_energy.restype = ctypes.c_double
         L.orc_energy.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp, _dp]
         L.orc_info.argtypes = [ctypes.c_void_p, _ip, _dp]
@@ -81,9 +82,11 @@ class Oracle:
 
     nu, L, a0 = -1 and Mtop = None select the reference's defaults.
     compat = 0: corrected algorithm; 1: reference HEAD index layout.
+    fft = False: the reference's DFT build (FF_set_FFT(false)).
     """
 
-    def __init__(self, pos, q, edges, nu=-1, L=-1, Mtop=None, a0=-1.0, estErr=-1.0, compat=0):
+    def __init__(self, pos, q, edges, nu=-1, L=-1, Mtop=None, a0=-1.0, estErr=-1.0, compat=0,
+                 fft=True):
         self.q = np.ascontiguousarray(q, dtype=np.float64)
         self.N = self.q.shape[0]
         pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(self.N, 3)
@@ -97,7 +100,8 @@ class Oracle:
         else:
             mtp = None
         self._h = lib().orc_build(self.N, _p(self.q), _p(pos), _p(self.edges), int(nu), int(L),
-                                  mtp, float(a0), float(estErr), int(compat), err)
+                                  mtp, float(a0), float(estErr), int(compat), 1 if fft else 0,
+                                  err)
         if not self._h:
             raise ValueError(err.value.decode())
         info = (ctypes.c_int * 8)()

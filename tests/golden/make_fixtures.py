@@ -8,9 +8,10 @@ Outputs (all data, no code):
       data/LICENSE); Box44.ini is stored gzip-compressed.
   tests/golden/published.json         -- values the reference's own runs
       published in results/: total energies of TableII, Figure4 and Figure6
-      (Box44), and the 4-digit force errors of Figure1 and Figure2; and
+      (Box44) with their pair counts and pair-list capacities, and the
+      4-digit force errors of Figure1 and Figure2; and
       the PME comparator's runs of Figure4 ("Figure4_pme": energy, beta,
-      pair count; the reference's pme CLI ignores --nu, so every run has
+      pair count, list capacity; the reference's pme CLI ignores --nu, so every run has
       the order "nu" = 6 it reports, whatever "nu_arg" asked for).
 """
 import gzip
@@ -41,7 +42,7 @@ def main():
 
     pub = {}
     keep = ["NumberOfLevels", "nu", "cutoff", "TopLevelMx", "utotal", "nlist_interaction_count",
-            "poterror", "time_energy", "time_nlist"]
+            "nlist_len", "poterror", "time_energy", "time_nlist"]
     for fig in ("TableII", "Figure4", "Figure6"):
         with open(os.path.join(REF, "results", fig + ".json")) as f:
             d = json.load(f)
@@ -50,7 +51,8 @@ def main():
                     for t in d["trial"]]
         if fig == "Figure4":
             pk = ["nu", "cutoff", "TopLevelMx", "beta", "tol_dir", "utotal",
-                  "nlist_interaction_count", "poterror", "deltaF/Fref", "time_energy", "time_nlist"]
+                  "nlist_interaction_count", "nlist_len", "poterror", "deltaF/Fref",
+                  "time_energy", "time_nlist"]
             pub["Figure4_pme"] = [{"M": t["M"], "a0": t["a0"], "nu_arg": t["nu"],
                                    **{k: t["pmeout"][k] for k in pk}} for t in d["trial"]]
     rows = []

@@ -304,8 +304,125 @@ def test_mixed_precision_accuracy(oracle_mod, torch_cuda):
         assert rel_err(E, Eo) < 1e-5, case
 
 
+def _capacity(c, n, a, edges):
+    """The reference's pair-list capacity c N + 2.1 (4/3 pi a^3) N (N-1) / (2 det A),
+    truncated (forcefield0.c:32 has c = 200; the published runs c = 3)."""
+    return int(c * n + 2.1 * ((4.0 / 3.0) * np.pi * a * a * a * 0.5 * n * (n - 1.0)
+                              / abs(np.linalg.det(edges))))
+
+
+@pytest.mark.parametrize("case", ["water300", "random3000", "tiny_box_self_images", "dense400_L2"])
+def test_pair_statistics_with_margin(case, oracle_mod, torch_cuda):
+    """FF_build's and FF_rebuild's margin widens the neighbour-list
+    statistics to a0 + margin (forcefield0.c:21-36,109-163): the pair count
+    against the oracle's list built with the same margin (its own cell
+    enumeration), and the list capacity, by the published runs' rule in the
+    fixed layout and by the source's in the as-is layout."""
+    import ctypes
+    from msmpb_amd import MSM, _lib
+    print(f"[parity] margin {case}", flush=True)
+    s = CASES[case]()
+    p = s.params
+    o = oracle_mod.Oracle(s.pos, s.q, s.edges, nu=p["nu"], L=p["levels"], Mtop=p["top_grid"],
+                          a0=p["cutoff"])
+    want0 = o.pair_count()
+    for margin in (1.5, 0.25):
+        want = o.build_pairs(s.pos, margin)
+        assert want >= want0  # equal where every pair is already inside a0
+        for layout, c in (("fixed", 3), ("asis", 200)):
+            m = MSM(s.pos, s.q, s.edges, nu=p["nu"], levels=p["levels"], top_grid=p["top_grid"],
+                    cutoff=p["cutoff"], layout=layout, margin=margin)
+            try:
+                assert m.pair_count == want, (margin, layout)
+                assert m.nlist_len == _capacity(c, s.N, m.cutoff + margin, s.edges)
+                # a rebuild in the same box with margin 0 counts within a0
+                e = np.ascontiguousarray(s.edges, dtype=np.float64)
+                x = np.ascontiguousarray(s.pos, dtype=np.float64)
+                dp = ctypes.POINTER(ctypes.c_double)
+                _lib.lib().FF_rebuild(m._ff, e.ctypes.data_as(dp), x.ctypes.data_as(dp),
+                                      ctypes.c_double(0.0))
+                assert m.ff.nlist_interaction_count == want0
+                assert m.ff.nlist_len == _capacity(c, s.N, m.cutoff, s.edges)
+            finally:
+                m.close()
+
+
+def test_pair_range_table_overflow_dense(oracle_mod, torch_cuda):
+    """20,000 random charges in a 30 A box with a0 = 10: every pass of the
+    pair kernel has more candidates than its range table holds (2,048
+    entries mixed, 1,024 FP64), so both kernels take the walk over the range
+    offsets.  Cells are >= a0/3 wide, so |dx|, |dy| <= 3 and the <= 49 cell
+    columns form one round, whose candidates include every atom within a0 of
+    the pass's first home atom."""
+    print("[parity] dense20k", flush=True)
+    s = _random_system("dense20k", 20000, (30.0, 30.0, 30.0), 10.0, 2, 6, 11)
+    d = s.pos[:100, None, :] - s.pos[None, :, :]
+    d -= 30.0 * np.floor(d / 30.0 + 0.5)
+    assert np.sum(np.sum(d * d, axis=2) < 100.0, axis=1).min() > 2048
+    m, o = _pair(s, oracle_mod)
+    E, F, phi = m.energy(s.pos, potential=True)
+    Eo, Fo, phio, _ = o.energy(s.pos)
+    assert rel_err(E, Eo) < TOL, (E, Eo)
+    assert force_err(F, Fo, force_scale(s.q, s.edges)) < TOL
+    assert np.max(np.abs(phi - phio)) / np.sqrt(np.mean(phio ** 2)) < TOL
+    assert m.pair_count == o.pair_count()
+    m.set_precision("mixed")
+    Em, Fm = m.energy(s.pos)
+    assert rms_rel_force(Fm, Fo) < 1e-5
+    assert rel_err(Em, Eo) < 1e-5
+
+
+def _dft_pair(name, layout, oracle_mod):
+    """Systems for the DFT build (FF_set_FFT(false)): automatic levels, grid
+    and cutoff at a fixed order, and explicit grids that are not 7-smooth
+    (13^3 = 2197 points takes the general DFT pass)."""
+    from msmpb_amd import MSM
+    if name.startswith("auto"):
+        n, box, nu = {"auto_L3": (1500, (25.0, 27.0, 29.0), 4),
+                      "auto_L4": (20000, (60.0, 60.0, 60.0), 6)}[name]
+        r = np.random.default_rng(3)
+        pos = r.uniform(0, 1, (n, 3)) * np.asarray(box)
+        q = r.uniform(-1, 1, n)
+        q -= q.mean()
+        edges = np.diag(box)
+        kw = dict(nu=nu)
+        okw = dict(nu=nu)
+    else:
+        edges, q, pos, _ = load_dataset("spceN300")
+        L, M = {"spce_L1_M13": (1, 13), "spce_L1_M11": (1, 11), "spce_L2_M11": (2, 11)}[name]
+        kw = dict(nu=4, levels=L, top_grid=M, cutoff=9.0)
+        okw = dict(nu=4, L=L, Mtop=M, a0=9.0)
+    m = MSM(pos, q, edges, layout=layout, fft=False, **kw)
+    o = oracle_mod.Oracle(pos, q, edges, compat=1 if layout == "asis" else 0, fft=False, **okw)
+    return m, o, pos, q, edges
+
+
+@pytest.mark.parametrize("name", ["auto_L3", "auto_L4", "spce_L1_M13", "spce_L1_M11",
+                                  "spce_L2_M11"])
+@pytest.mark.parametrize("layout", ["fixed", "asis"])
+def test_dft_build(name, layout, oracle_mod, torch_cuda):
+    """FF_set_FFT(false): the DFT build's parameter rules and, in the as-is
+    layout, its level-1 top-level read (forcefield1.c:483-497), FP64 against
+    the oracle to 1e-10."""
+    print(f"[parity] dft build {name} {layout}", flush=True)
+    m, o, pos, q, edges = _dft_pair(name, layout, oracle_mod)
+    try:
+        assert (m.levels, m.top_grid, m.nu) == (o.L, o.Mtop, o.nu)
+        assert not m.ff.FFT
+        E, F, phi = m.energy(pos, potential=True)
+        Eo, Fo, phio, to = o.energy(pos)
+        assert rel_err(E, Eo) < TOL, (E, Eo)
+        assert force_err(F, Fo, force_scale(q, edges)) < TOL
+        assert np.max(np.abs(phi - phio)) / np.sqrt(np.mean(phio ** 2)) < TOL
+        t = m.terms()
+        assert abs(t[2] - to[2]) <= TOL * max(abs(Eo), abs(to[2]))
+        assert m.pair_count == o.pair_count()
+    finally:
+        m.close()
+
+
 def test_water96k_fp64_and_mixed(oracle_mod, torch_cuda):
-    """The benchmark workload (96,000 atoms, 6,859 pair-search cells): FP64 to
+    """The benchmark workload (96,000 atoms, 24,389 pair-search cells): FP64 to
     the oracle at 1e-10 and the mixed path's rms relative force error."""
     print("[parity] water96k", flush=True)
     s = inputs.cfg3_water96k(6)

@@ -133,7 +133,8 @@ def test_pme_mixed_matches_oracle(case, reference, torch_cuda):
 @pytest.mark.parametrize("M,a0", RUNS)
 def test_pme_published_box44(M, a0, torch_cuda):
     """Each distinct PME run of results/Figure4.json (order 6, as the
-    reference's pme always used): energy, beta, pair count, poterror."""
+    reference's pme always used): energy, beta, pair count, list capacity,
+    poterror."""
     from msmpb_amd import MSM
     t = next(t for t in PUB if t["M"] == M and t["a0"] == a0)
     edges, q, pos, _ = load_dataset("Box44")
@@ -144,6 +145,7 @@ def test_pme_published_box44(M, a0, torch_cuda):
     try:
         assert round(m.beta, 6) == t["beta"]
         assert m.pair_count == t["nlist_interaction_count"]
+        assert m.nlist_len == t["nlist_len"]
         E, _ = m.energy(pos)
         assert rel_err(E, t["utotal"]) < TOL, (E, t["utotal"])
         assert rel_err(abs(epot - E) / abs(epot), t["poterror"]) < 1e-6
@@ -184,6 +186,7 @@ def test_pme_capi_published_and_struct(torch_cuda):
         assert s.orderAcc == 6 and list(s.topGridDim) == [48, 48, 48]
         assert s.tolDir == 1e-5 and round(s.beta, 6) == t["beta"]
         assert s.nlist_interaction_count == t["nlist_interaction_count"]
+        assert s.nlist_len == t["nlist_len"]  # pme0.c:31
         assert s.detA == pytest.approx(88.0 ** 3, rel=1e-15)
         F = np.zeros((len(qq), 3))
         x = np.ascontiguousarray(pos)

@@ -2,11 +2,11 @@
 
 results/TableII.json, results/Figure4.json and results/Figure6.json of the
 reference hold the full-precision total energy (`utotal`) and the
-neighbour-list pair count of every run on Box44 (65,850 atoms of SPC/E
+neighbour-list pair count and list capacity of every run on Box44 (65,850 atoms of SPC/E
 water, code/journal/TableII_run.py, Figure4_run.py, Figure6_run.py); the
 fixture tests/golden/published.json carries them.  The plain FP64 path must
-reproduce each energy to the 1e-10 relative bar and each pair count
-exactly, for every level count (1-3), order (4, 6), cutoff (9, 12 A) and
+reproduce each energy to the 1e-10 relative bar and each pair count and
+capacity exactly, for every level count (1-3), order (4, 6), cutoff (9, 12 A) and
 grid the journal swept.  `poterror` is |E_pot - E| / |E_pot| with E_pot the
 first line of data/Box44.pot (code/msmpb.c), a function of the energy alone.
 """
@@ -49,5 +49,6 @@ def test_published_box44_energy_gpu(fig, idx, box44):
         assert rel_err(E, t["utotal"]) < 1e-10, (E, t["utotal"])
         assert rel_err(abs(epot - E) / abs(epot), t["poterror"]) < 1e-6
         assert m.pair_count == t["nlist_interaction_count"]
+        assert m.nlist_len == t["nlist_len"]  # the pair-list capacity the runs report
     finally:
         m.close()

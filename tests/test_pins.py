@@ -146,7 +146,7 @@ DEFAULT_CASES = [
 ]
 
 
-def _product_build(pos, q, edges, L=None, nu=None, a0=None, estErr=None, Mtop=None):
+def _product_build(pos, q, edges, L=None, nu=None, a0=None, estErr=None, Mtop=None, fft=True):
     """FF_build of libmsmpb_hip.so through the C ABI.  Without a GPU the
     device part fails, after the host rules have published their choices
     into the public fields (as the reference's FF_build does)."""
@@ -166,6 +166,7 @@ def _product_build(pos, q, edges, L=None, nu=None, a0=None, estErr=None, Mtop=No
         lib.FF_set_estErr(ff, estErr)
     if Mtop is not None:
         lib.FF_set_topGridDim(ff, (ctypes.c_int * 3)(*Mtop))
+    lib.FF_set_FFT(ff, fft)
     lib.FF_build(ff, len(q), pos.ctypes.data_as(dp), edges.ctypes.data_as(dp), 0.0)
     return lib, ff, (q, pos, edges)
 
@@ -223,6 +224,118 @@ def test_example_c_parameters(oracle_mod):
         assert lib.FF_get_cutoff(ff) == 4.0
     finally:
         lib.FF_delete(ff)
+
+
+# ------------------------------------------------------------- DFT build
+def _equal_spacing_dft(N, box, L):
+    """determineM in the DFT build (forcefield0.c:247-269): eta = 1 grid point
+    per atom on the finest level, h^3 = V / N, the top grid the finest
+    divided by 2^(L-1), rounded up."""
+    h = (np.prod(box) / N) ** (1.0 / 3.0)
+    return [int(np.ceil(a / h / 2 ** (L - 1) - 1e-12)) for a in box]
+
+
+# (N, box, nu, maxLevel requested, levels chosen, top grid chosen)
+DFT_CASES = [
+    (3000, (30.0, 33.0, 37.0), 4, None, 4, (2, 2, 3)),
+    (1500, (25.0, 27.0, 29.0), 4, None, 3, (3, 3, 4)),
+    (20000, (60.0, 60.0, 60.0), 6, None, 4, (4, 4, 4)),
+    (600, (20.0, 26.0, 30.0), 4, 2, 3, (2, 3, 3)),
+]
+
+
+@pytest.mark.parametrize("case", range(len(DFT_CASES)))
+def test_dft_build_levels_and_grid(case, oracle_mod, capfd):
+    """FF_set_FFT(false) (forcefield0.c:292-307): the level count becomes the
+    first whose determineM grid has at most sqrt(N) points (a requested
+    maxLevel is replaced, with a note on stderr), and that grid is kept as
+    it is: no rounding to 7-smooth sizes (:309-316)."""
+    N, box, nu, L, Lw, Mw = DFT_CASES[case]
+    pos, q, edges = _random(N, box, case)
+    grids = [_equal_spacing_dft(N, box, l) for l in range(1, Lw + 1)]
+    # the defining property: the chosen count is the first that fits
+    assert [bool(np.prod(g) <= np.sqrt(N)) for g in grids] == [False] * (Lw - 1) + [True]
+    assert tuple(grids[-1]) == Mw
+    o = oracle_mod.Oracle(pos, q, edges, nu=nu, L=L or -1, fft=False)
+    assert o.L == Lw and o.Mtop == Mw
+    capfd.readouterr()
+    lib, ff, keep = _product_build(pos, q, edges, L=L, nu=nu, fft=False)
+    try:
+        d = (ctypes.c_int * 3)()
+        lib.FF_get_topGridDim(ff, d)
+        assert tuple(d) == Mw and lib.FF_get_maxLevel(ff) == Lw
+        assert rel_err(lib.FF_get_cutoff(ff), o.a0) < 1e-14
+        assert not lib.FF_get_FFT(ff)
+    finally:
+        lib.FF_delete(ff)
+    note = f"L={L or 1} is replaced by L={Lw}"
+    assert (note in capfd.readouterr().err) == ((L or 1) != Lw)
+
+
+def test_dft_build_keeps_grid_size(oracle_mod):
+    """An explicit top grid that is not 7-smooth (11) is kept by the DFT
+    build and rounded up to 12 by the FFT build."""
+    pos, q, edges = _random(300, (20.0, 20.0, 20.0), 3)
+    for fft, want in ((False, 11), (True, 12)):
+        o = oracle_mod.Oracle(pos, q, edges, nu=4, L=1, Mtop=11, a0=8.0, fft=fft)
+        assert o.Mtop == (want,) * 3
+        lib, ff, keep = _product_build(pos, q, edges, L=1, nu=4, a0=8.0, Mtop=(11, 11, 11),
+                                       fft=fft)
+        try:
+            d = (ctypes.c_int * 3)()
+            lib.FF_get_topGridDim(ff, d)
+            assert tuple(d) == (want,) * 3 and bool(lib.FF_get_FFT(ff)) == fft
+        finally:
+            lib.FF_delete(ff)
+
+
+def _reference_dft_l1(fx, nu, of_shift=0, stride_extra=None):
+    """The HEAD's DFT build at L = 1: DFTg2g (forcefield1.c:171-172, 483-497)
+    reads q[1] -- the (M + nu - 1)^3 array anterpolate fills at offset
+    nu/2 - 1 (zero ghosts) -- at the flat unpadded index, convolves that with
+    the top-level kernel, and the energy loop (:195-219) reads q[1] the same
+    way."""
+    q1 = fx.grid(1, 0)
+    M = np.array(q1.shape)
+    sd = nu - 1 if stride_extra is None else stride_extra
+    of = nu // 2 - 1 + of_shift
+    P = np.zeros(tuple(M + sd))
+    P[of:of + M[0], of:of + M[1], of:of + M[2]] = q1
+    x = P.reshape(-1)[:q1.size].reshape(q1.shape)
+    e = np.real(np.fft.ifftn(np.fft.fftn(x) * fx.stencil(1)))
+    return 0.5 * np.sum(x * e), e
+
+
+@pytest.mark.parametrize("nu,M", [(4, 22), (6, 20)])
+def test_asis_dft_build_from_reference_index_maps(nu, M, oracle_mod):
+    edges, q, pos, _ = load_dataset("spceN300")
+    fx = oracle_mod.Oracle(pos, q, edges, nu=nu, L=1, Mtop=M, a0=9.0, fft=False)
+    ax = oracle_mod.Oracle(pos, q, edges, nu=nu, L=1, Mtop=M, a0=9.0, compat=1, fft=False)
+    assert fx.Mtop == (M,) * 3
+    _, _, _, t = fx.energy(pos)
+    Ea, _, _, ta = ax.energy(pos)
+    grid_e, e1 = _reference_dft_l1(fx, nu)
+    assert ta[0] == t[0] and ta[1] == t[1]
+    assert abs(ta[2] - grid_e) <= 1e-12 * abs(Ea), (ta[2], grid_e)
+    assert np.max(np.abs(ax.grid(1, 1) - e1)) <= 1e-12 * np.max(np.abs(e1))
+    for mut in (dict(of_shift=1), dict(stride_extra=nu)):
+        wrong, _ = _reference_dft_l1(fx, nu, **mut)
+        assert abs(wrong - grid_e) > 1e-9 * abs(Ea), mut
+
+
+def test_asis_dft_build_reads_top_grid_directly(oracle_mod):
+    """At L >= 2 the DFT build's top level reads q^L, which restrict_ wrote
+    unpadded, at the flat index: the HEAD's result equals the corrected one
+    in an orthorhombic box, where the layouts differ in nothing else."""
+    edges, q, pos, _ = load_dataset("spceN300")
+    for L, M in ((2, 11), (3, 5)):
+        a0 = 9.0 if L == 2 else 5.0
+        fx = oracle_mod.Oracle(pos, q, edges, nu=4, L=L, Mtop=M, a0=a0, fft=False)
+        ax = oracle_mod.Oracle(pos, q, edges, nu=4, L=L, Mtop=M, a0=a0, compat=1, fft=False)
+        assert fx.Mtop == (M,) * 3 and fx.L == L
+        Ef, Ff, _, _ = fx.energy(pos)
+        Ea, Fa, _, _ = ax.energy(pos)
+        assert Ea == Ef and np.array_equal(Fa, Ff)
 
 
 # --------------------------------------------------- invert / prod / prodT
