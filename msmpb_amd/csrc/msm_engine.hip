@@ -103,6 +103,9 @@ struct Engine {
   unsigned *d_done = nullptr;  // workgroup counter of the interpolation's energy reduction
   int *d_scell = nullptr, *d_flags = nullptr;
   double4 *d_wpos = nullptr, *d_spos = nullptr;
+  // per-atom spreading placements (packed support origin; t and q), sorted order
+  unsigned *d_sbase = nullptr;
+  double4 *d_st = nullptr;
   int4 *d_sten = nullptr;
   unsigned long long *d_count = nullptr;
   double *d_ql[msm::kMaxLevel + 1] = {}, *d_el[msm::kMaxLevel + 1] = {};
@@ -134,7 +137,8 @@ struct Engine {
     void *ptrs[] = {d_pos,  d_q,    d_F,    d_phi,   d_out,  d_qpart, d_scal,  d_epair,
                     d_egrid, d_cell_of, d_cnt, d_start, d_sidx, d_scell, d_flags, d_wpos,
                     d_rank, d_done, d_Fp, d_phip, d_ep2,
-                    d_spos, d_sten, d_count, d_qsplit, d_tw, d_topwork, d_stab, d_sscr};
+                    d_spos, d_sten, d_count, d_qsplit, d_tw, d_topwork, d_stab, d_sscr,
+                    d_sbase, d_st};
     for (void *p : ptrs)
       if (p) hipFree(p);
     for (int pr = 0; pr < 2; ++pr)
@@ -158,6 +162,8 @@ struct Engine {
     d_cell_of = d_cnt = d_start = d_sidx = d_scell = d_flags = d_rank = nullptr;
     d_done = nullptr;
     d_Fp = d_phip = d_ep2 = nullptr;
+    d_sbase = nullptr;
+    d_st = nullptr;
     d_wpos = d_spos = nullptr;
     d_sten = nullptr;
     d_count = nullptr;
@@ -570,8 +576,10 @@ int create_engine(FFX *x, const msm::Plan &plan) {
     E->stream = nullptr;  // HIP's null stream: host-buffer entry points, FF_build
     E->stream2 = st[0], E->stream3 = st[1];
   }
+  // fork/join events order kernels of this device only: no system-scope
+  // fence (a cache writeback and invalidation per record) is needed
   for (hipEvent_t *e : {&E->ev_fork, &E->ev_join, &E->ev_q1, &E->ev_g1})
-    HIP_TRY(hipEventCreateWithFlags(e, hipEventDisableTiming));
+    HIP_TRY(hipEventCreateWithFlags(e, hipEventDisableTiming | hipEventDisableSystemFence));
   const msm::Plan &p = E->plan;
   for (int l = 1; l <= p.L; ++l) {
     E->gd[l] = {p.M[l][0], p.M[l][1], p.M[l][2]};
@@ -616,6 +624,10 @@ int create_engine(FFX *x, const msm::Plan &plan) {
   TRY(dalloc(&E->d_wpos, N));
   TRY(dalloc(&E->d_spos, N));
   TRY(dalloc(&E->d_count, 1));
+  if (msm::spread_by_placements(E->P, E->gd[1])) {
+    TRY(dalloc(&E->d_sbase, N));
+    TRY(dalloc(&E->d_st, N));
+  }
   for (int l = 1; l <= p.L; ++l) {
     TRY(dalloc(&E->d_ql[l], E->gd[l].size()));
     TRY(dalloc(&E->d_el[l], E->gd[l].size()));
@@ -676,7 +688,7 @@ int stage_grids(Engine &E, int precision, int layout, Timing &T, hipStream_t s) 
   // the BF16 stencils read q^l as bf16 hi/lo pairs, written with q^l
   const bool split = pr == msm::MSM_G2G_BF16X3;
   msm::launch_spread(E.P, E.gd[1], E.d_spos, E.d_start, E.d_ql[1], s,
-                     split && L >= 2 ? E.d_qb[1] : nullptr);
+                     split && L >= 2 ? E.d_qb[1] : nullptr, E.d_sbase, E.d_st);
   T.close(t, "anterpolation", s);
   if (L >= 2) {
     HIP_TRY(hipEventRecord(E.ev_q1, s));

@@ -8,6 +8,11 @@
 
 namespace msm {
 
+// The grid chain runs beside the pair kernel and bounds the evaluation: its
+// wavefronts take instruction issue ahead of the pair kernel's on a shared
+// SIMD (wave priority 3; the pair kernel keeps the default 0).
+__device__ __forceinline__ void chain_issue_priority() { __builtin_amdgcn_s_setprio(3); }
+
 constexpr int kMaxNu = 10;
 
 // Passed by value to every kernel (lives in the kernarg segment, read with
@@ -99,9 +104,13 @@ void launch_pair_count(const DevParams &P, const double4 *spos, const int *start
 
 // B-spline spreading of charges onto the level-1 grid, one workgroup per
 // grid brick (every grid point written exactly once).
-// q1b (optional): q^1 also as bf16_split pairs.
+// q1b (optional): q^1 also as bf16_split pairs.  sbase, st (optional, N
+// entries each): per-atom placements, computed first and used where
+// spread_by_placements(P, g) holds.
 void launch_spread(const DevParams &P, GridDims g, const double4 *spos, const int *start,
-                   double *q1, hipStream_t s, unsigned *q1b = nullptr);
+                   double *q1, hipStream_t s, unsigned *q1b = nullptr, unsigned *sbase = nullptr,
+                   double4 *st = nullptr);
+bool spread_by_placements(const DevParams &P, GridDims g);
 // q^l = R q^{l-1} (full weighting with the two-scale stencil).
 // qcb (optional): q^l also as bf16_split pairs.
 void launch_restrict(const DevParams &P, GridDims fine, GridDims coarse, const double *qf,

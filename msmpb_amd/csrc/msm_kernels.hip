@@ -784,7 +784,12 @@ __global__ __launch_bounds__(64) void k_pair_cell(DevParams P, const double4 *__
 // against every home atom.  A pair is counted at the image whose fractional
 // separation has every component in [-1/2, 1/2), r - floor(r/L + 1/2) L as
 // in the reference (forcefield0.c:140-148), so that of two images tied at
-// exactly half a box only one is counted.
+// exactly half a box only one is counted.  In an orthorhombic box whose
+// edges exceed twice the radius that image is the only one inside it, and
+// IMG = false skips the test.
+constexpr int kCountAhead = 4;  // candidate loads in flight per lane
+
+template <bool IMG>
 __global__ __launch_bounds__(64) void k_pair_count(DevParams P, const double4 *__restrict__ spos,
                                                    const int *__restrict__ start,
                                                    const int4 *__restrict__ cols, int ncol,
@@ -797,66 +802,94 @@ __global__ __launch_bounds__(64) void k_pair_count(DevParams P, const double4 *_
   if (n == 0) return;  // wavefront-uniform
   const int cz = c % P.nc[2], cy = (c / P.nc[2]) % P.nc[1], cx = c / (P.nc[1] * P.nc[2]);
   unsigned cnt = 0;
-  for (int c0 = 0; c0 < ncol; c0 += 64) {
-    int beg0 = 0, cnt0 = 0, beg1 = 0, cnt1 = 0;
-    double sx = 0, sy = 0, sz0 = 0, sz1 = 0, sx1 = 0, sy1 = 0;
-    if (c0 + lane < ncol) {
-      const int4 col = cols[c0 + lane];  // dx, dy, dzlo, dzhi (dzhi - dzlo < nc[2])
-      int X = cx + col.x, Y = cy + col.y;
-      const int kx = floordiv(X, P.nc[0]), ky = floordiv(Y, P.nc[1]);
-      X -= kx * P.nc[0], Y -= ky * P.nc[1];
-      const int base = (X * P.nc[1] + Y) * P.nc[2];
-      const int Z0 = cz + col.z, Z1 = cz + col.w;
-      const int kz0 = floordiv(Z0, P.nc[2]);
-      const int e0 = min(Z1, (kz0 + 1) * P.nc[2] - 1);
-      beg0 = start[base + Z0 - kz0 * P.nc[2]];
-      cnt0 = start[base + e0 - kz0 * P.nc[2] + 1] - beg0;
-      lattice_rt(P, kx, ky, kz0, sx, sy, sz0);
-      if (e0 < Z1) {  // the rest lies in the next periodic image along z
-        beg1 = start[base];
-        cnt1 = start[base + Z1 - (kz0 + 1) * P.nc[2] + 1] - beg1;
-        lattice_rt(P, kx, ky, kz0 + 1, sx1, sy1, sz1);
+  // home atoms kPassAtoms at a time, held in registers (wavefront-uniform)
+  for (int ip = 0; ip < n; ip += kPassAtoms) {
+    const int ni = min(kPassAtoms, n - ip);
+    double hx[kPassAtoms], hy[kPassAtoms], hz[kPassAtoms];
+#pragma unroll
+    for (int u = 0; u < kPassAtoms; ++u) {
+      const double4 a = spos[b0 + ip + min(u, ni - 1)];
+      hx[u] = wave_uniform(a.x), hy[u] = wave_uniform(a.y), hz[u] = wave_uniform(a.z);
+    }
+    const int i0 = b0 + ip;  // sorted index of home atom u: i0 + u
+    for (int c0 = 0; c0 < ncol; c0 += 64) {
+      int beg0 = 0, cnt0 = 0, beg1 = 0, cnt1 = 0;
+      double sx = 0, sy = 0, sz0 = 0, sz1 = 0, sx1 = 0, sy1 = 0;
+      if (c0 + lane < ncol) {
+        const int4 col = cols[c0 + lane];  // dx, dy, dzlo, dzhi (dzhi - dzlo < nc[2])
+        int X = cx + col.x, Y = cy + col.y;
+        const int kx = floordiv(X, P.nc[0]), ky = floordiv(Y, P.nc[1]);
+        X -= kx * P.nc[0], Y -= ky * P.nc[1];
+        const int base = (X * P.nc[1] + Y) * P.nc[2];
+        const int Z0 = cz + col.z, Z1 = cz + col.w;
+        const int kz0 = floordiv(Z0, P.nc[2]);
+        const int e0 = min(Z1, (kz0 + 1) * P.nc[2] - 1);
+        beg0 = start[base + Z0 - kz0 * P.nc[2]];
+        cnt0 = start[base + e0 - kz0 * P.nc[2] + 1] - beg0;
+        lattice_rt(P, kx, ky, kz0, sx, sy, sz0);
+        if (e0 < Z1) {  // the rest lies in the next periodic image along z
+          beg1 = start[base];
+          cnt1 = start[base + Z1 - (kz0 + 1) * P.nc[2] + 1] - beg1;
+          lattice_rt(P, kx, ky, kz0 + 1, sx1, sy1, sz1);
+        }
       }
-    }
-    const int mine = ((int)(cnt0 > 0) + (int)(cnt1 > 0)) * (1 << 24) + cnt0 + cnt1;
-    int incl = mine;
-    for (int off = 1; off < 64; off <<= 1) {
-      const int t = __shfl_up(incl, off, 64);
-      if (lane >= off) incl += t;
-    }
-    const int excl = incl - mine, last = __shfl(incl, 63, 64);
-    int k = excl >> 24;
-    const int o = excl & 0xffffff;
-    if (cnt0 > 0) {
-      seg_off[k] = o, seg_beg[k] = beg0;
-      seg_sx[k] = sx, seg_sy[k] = sy, seg_sz[k] = sz0;
-      ++k;
-    }
-    if (cnt1 > 0) {
-      seg_off[k] = o + cnt0, seg_beg[k] = beg1;
-      seg_sx[k] = sx1, seg_sy[k] = sy1, seg_sz[k] = sz1;
-    }
-    const int total = last & 0xffffff;
-    if (lane == 0) seg_off[last >> 24] = total;  // sentinel
-    __syncthreads();
-    int sl = 0;
-    for (int g = lane; g < total; g += 64) {
-      while (seg_off[sl + 1] <= g) ++sl;  // the sentinel total > g ends it
-      const int j = seg_beg[sl] + g - seg_off[sl];
-      const double4 aj = spos[j];
-      const double lx = seg_sx[sl], ly = seg_sy[sl], lz = seg_sz[sl];
-      for (int h = 0; h < n; ++h) {
-        const int i = b0 + h;
-        if (j <= i) continue;
-        const double4 ai = spos[i];
-        const double d[3] = {aj.x + (lx - ai.x), aj.y + (ly - ai.y), aj.z + (lz - ai.z)};
-        if (d[0] * d[0] + d[1] * d[1] + d[2] * d[2] >= r2max) continue;
-        double sd[3];
-        to_frac_rt(P, d, sd);
-        cnt += floor(sd[0] + 0.5) == 0.0 && floor(sd[1] + 0.5) == 0.0 && floor(sd[2] + 0.5) == 0.0;
+      const int mine = ((int)(cnt0 > 0) + (int)(cnt1 > 0)) * (1 << 24) + cnt0 + cnt1;
+      int incl = mine;
+      for (int off = 1; off < 64; off <<= 1) {
+        const int t = __shfl_up(incl, off, 64);
+        if (lane >= off) incl += t;
       }
+      const int excl = incl - mine, last = __shfl(incl, 63, 64);
+      int k = excl >> 24;
+      const int o = excl & 0xffffff;
+      if (cnt0 > 0) {
+        seg_off[k] = o, seg_beg[k] = beg0;
+        seg_sx[k] = sx, seg_sy[k] = sy, seg_sz[k] = sz0;
+        ++k;
+      }
+      if (cnt1 > 0) {
+        seg_off[k] = o + cnt0, seg_beg[k] = beg1;
+        seg_sx[k] = sx1, seg_sy[k] = sy1, seg_sz[k] = sz1;
+      }
+      const int total = last & 0xffffff;
+      if (lane == 0) seg_off[last >> 24] = total;  // sentinel
+      __syncthreads();
+      // kCountAhead candidates per lane in flight: their loads first
+      int sl = 0;
+      for (int g0 = lane; g0 - lane < total; g0 += 64 * kCountAhead) {
+        double4 aj[kCountAhead];
+        double lx[kCountAhead], ly[kCountAhead], lz[kCountAhead];
+        int jj[kCountAhead];
+#pragma unroll
+        for (int v = 0; v < kCountAhead; ++v) {
+          const int g = min(g0 + 64 * v, total - 1);  // clamped: always a valid index
+          while (seg_off[sl + 1] <= g) ++sl;  // the sentinel total > g ends it
+          jj[v] = g0 + 64 * v < total ? seg_beg[sl] + g - seg_off[sl] : -1;
+          aj[v] = spos[seg_beg[sl] + g - seg_off[sl]];
+          lx[v] = seg_sx[sl], ly[v] = seg_sy[sl], lz[v] = seg_sz[sl];
+        }
+#pragma unroll
+        for (int v = 0; v < kCountAhead; ++v) {
+          const int j = jj[v];  // -1 past the end: below every home index
+#pragma unroll
+          for (int u = 0; u < kPassAtoms; ++u) {
+            const double d[3] = {aj[v].x + (lx[v] - hx[u]), aj[v].y + (ly[v] - hy[u]),
+                                 aj[v].z + (lz[v] - hz[u])};
+            bool in = u < ni && j > i0 + u && d[0] * d[0] + d[1] * d[1] + d[2] * d[2] < r2max;
+            if constexpr (IMG) {
+              if (in) {
+                double sd[3];
+                to_frac_rt(P, d, sd);
+                in = floor(sd[0] + 0.5) == 0.0 && floor(sd[1] + 0.5) == 0.0 &&
+                     floor(sd[2] + 0.5) == 0.0;
+              }
+            }
+            cnt += in;
+          }
+        }
+      }
+      __syncthreads();  // the tables are rewritten by the next round
     }
-    __syncthreads();  // the tables are rewritten by the next round
   }
   for (int off = 32; off > 0; off >>= 1) cnt += __shfl_down(cnt, off, 64);
   if (lane == 0 && cnt) atomicAdd(count, (unsigned long long)cnt);
@@ -1013,6 +1046,7 @@ __global__ __launch_bounds__(64 * WPB) void k_spread_mfma(DevParams P, GridDims 
                                                           const int *__restrict__ start,
                                                           double *__restrict__ q1,
                                                           unsigned *__restrict__ q1b) {
+  chain_issue_priority();
   constexpr int BS = 64 * WPB;
   // per wavefront: records of kSFlush atoms and a queue of sorted indices
   __shared__ double WX[WPB * kSFlush * kWXs];
@@ -1138,6 +1172,240 @@ __global__ __launch_bounds__(64 * WPB) void k_spread_mfma(DevParams P, GridDims 
   }
 }
 
+// ---- spreading from per-atom placements (grids with M_x >= 16 + NU and
+// M_y, M_z >= 4 + NU, every axis below 1024 points)
+//
+// k_spread_place computes once per atom, in cell-sorted order, what every
+// brick the atom reaches needs: the lowest support point m - NU/2 + 1 (mod M)
+// per axis, packed 10 bits each, and the fractional offsets t with the
+// charge.  k_spread_bricks then tests a candidate with three integer
+// comparisons and builds the MFMA operands of a flush directly from each
+// atom's NU weights per axis: the lane for row r (A) or column (y, z) (B)
+// picks weight i = rel + NU - 1 - r of its atom, zero outside [0, NU), where
+// rel in [1 - NU, ext - 1] is the support's first point relative to the
+// brick (negative where the support wraps in from below).  On these grids
+// one such offset per axis covers every point of the brick the support
+// reaches, so no point needs two weights of one atom.  The arithmetic, the
+// atoms of each MFMA and their order are those of k_spread_mfma.
+constexpr int kSPFlush = 16;               // atoms per flush
+constexpr int kSPAhead = 4;                // candidate placements in flight per lane
+constexpr int kSPQueue = kSPFlush + 64;    // queued atoms: < kSPFlush + one batch of 64
+
+template <bool TRI>
+__global__ __launch_bounds__(256) void k_spread_place(DevParams P, GridDims g,
+                                                      const double4 *__restrict__ spos,
+                                                      unsigned *__restrict__ sbase,
+                                                      double4 *__restrict__ st) {
+  chain_issue_priority();
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= P.N) return;
+  const double4 at = spos[i];
+  const double r[3] = {at.x, at.y, at.z};
+  double sf[3], t[3];
+  to_frac<TRI>(P, r, sf);
+  const int dims[3] = {g.x, g.y, g.z};
+  unsigned packed = 0;
+#pragma unroll
+  for (int ax = 0; ax < 3; ++ax) {
+    double sv = sf[ax];
+    sv = sv - floor(sv);
+    const double tt = dims[ax] * sv, m = floor(tt);
+    t[ax] = tt - m;
+    int v = ((int)m - P.nu / 2 + 1) % dims[ax];
+    v += v < 0 ? dims[ax] : 0;
+    packed |= (unsigned)v << (10 * ax);
+  }
+  sbase[i] = packed;
+  st[i] = make_double4(t[0], t[1], t[2], at.w);
+}
+
+template <int NU, int WPB>
+__global__ __launch_bounds__(64 * WPB) void k_spread_bricks(DevParams P, GridDims g,
+                                                            const unsigned *__restrict__ sbase,
+                                                            const double4 *__restrict__ st,
+                                                            const int *__restrict__ start,
+                                                            double *__restrict__ q1,
+                                                            unsigned *__restrict__ q1b) {
+  chain_issue_priority();
+  constexpr int BS = 64 * WPB, NW = 3 * NU;
+  constexpr int WSZ = WPB * (kSPFlush * NW > 256 ? kSPFlush * NW : 256);
+  __shared__ double W[WSZ];                  // weights of one flush's atoms; the tile sum
+  __shared__ double4 queue[WPB * kSPQueue];  // (t, q) of the queued atoms
+  __shared__ unsigned qrel[WPB * kSPQueue];  // their offsets rel + NU - 1, 8 bits per axis
+  __shared__ int rbeg[kSRanges], roff[kSRanges + 1 + kSJump];  // total repeated past the end
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int nbz = (g.z + kSZ - 1) / kSZ, nby = (g.y + kSY - 1) / kSY;
+  int b = blockIdx.x;
+  const int gz0 = (b % nbz) * kSZ;
+  b /= nbz;
+  const int gy0 = (b % nby) * kSY, gx0 = (b / nby) * kSX;
+  int lox, lenx, loy, leny, loz, lenz;
+  cell_span(gx0, kSX, NU / 2, g.x, P.nc[0], lox, lenx);
+  cell_span(gy0, kSY, NU / 2, g.y, P.nc[1], loy, leny);
+  cell_span(gz0, kSZ, NU / 2, g.z, P.nc[2], loz, lenz);
+  const int ncol = lenx * leny;
+  const int dims[3] = {g.x, g.y, g.z}, org[3] = {gx0, gy0, gz0}, ext[3] = {kSX, kSY, kSZ};
+  const unsigned long long below = (1ull << lane) - 1ull;
+  double *const Wv = W + wv * (WSZ / WPB);
+  double4 *const Qw = queue + wv * kSPQueue;
+  unsigned *const Rw = qrel + wv * kSPQueue;
+  const int row = lane & 15, kq = lane >> 4;
+  int npend = 0;  // queued atoms of this wavefront (uniform)
+  sp_f64x4 acc = {0.0, 0.0, 0.0, 0.0};
+  // weights of queued atoms [0, n) into Wv, then Q += WX^T (WY x WZ) by
+  // (n + 3) / 4 MFMAs (atoms n .. of the last group contribute zeros)
+  auto flush = [&](int n) {
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (int r0 = 0; r0 < 3 * kSPFlush; r0 += 64) {  // job: (axis, atom)
+      const int j = r0 + lane, a = j % kSPFlush, ax = j / kSPFlush;
+      if (j < 3 * kSPFlush && a < n) {
+        const double4 v = Qw[a];
+        const double t = ax == 0 ? v.x : ax == 1 ? v.y : v.z, sc = ax == 1 ? v.w : 1.0;
+        double w[NU];
+        bspline<NU, false>(P.Q, t, w, nullptr);
+#pragma unroll
+        for (int i = 0; i < NU; ++i) Wv[a * NW + ax * NU + i] = sc * w[i];
+      }
+    }
+    __builtin_amdgcn_wave_barrier();
+    // all four groups' operands first (their LDS reads overlap), then the MFMAs
+    double av[kSPFlush / 4], bv[kSPFlush / 4];
+#pragma unroll
+    for (int gi = 0; gi < kSPFlush / 4; ++gi) {
+      const int a = 4 * gi + kq;
+      const unsigned pr = Rw[a];
+      const int ix = (int)(pr & 255u) - row, iy = (int)((pr >> 8) & 255u) - (row >> 2),
+                iz = (int)((pr >> 16) & 255u) - (row & 3);
+      const double *wa = Wv + a * NW;
+      const bool ok = a < n;
+      const double wx = wa[min(max(ix, 0), NU - 1)];
+      const double wy = wa[NU + min(max(iy, 0), NU - 1)];
+      const double wz = wa[2 * NU + min(max(iz, 0), NU - 1)];
+      // (LDS past the queued atoms holds stale values: both operands zero)
+      av[gi] = ok && (unsigned)ix < (unsigned)NU ? wx : 0.0;
+      bv[gi] = ok && (unsigned)iy < (unsigned)NU && (unsigned)iz < (unsigned)NU ? wy * wz : 0.0;
+    }
+#pragma unroll
+    for (int gi = 0; gi < kSPFlush / 4; ++gi)
+      if (4 * gi < n) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[gi], bv[gi], acc, 0, 0, 0);
+    __builtin_amdgcn_wave_barrier();
+  };
+  for (int c0 = 0; c0 < ncol; c0 += kSRanges / 2) {
+    const int nc_b = min(ncol - c0, kSRanges / 2);
+    __syncthreads();  // previous batch done with the table
+    for (int cc = tid; cc < nc_b; cc += BS) {
+      int col = c0 + cc;
+      int x = wrapi(lox + col / leny, P.nc[0]), y = wrapi(loy + col % leny, P.nc[1]);
+      int base = (x * P.nc[1] + y) * P.nc[2];
+      int z0 = wrapi(loz, P.nc[2]), z1 = z0 + lenz;
+      int e1 = min(z1, P.nc[2]), e2 = z1 > P.nc[2] ? z1 - P.nc[2] : 0;
+      rbeg[2 * cc] = start[base + z0];
+      roff[2 * cc + 1] = start[base + e1] - start[base + z0];
+      rbeg[2 * cc + 1] = start[base];
+      roff[2 * cc + 2] = start[base + e2] - start[base];
+    }
+    __syncthreads();
+    const int nr = 2 * nc_b;
+    if (wv == 0) {  // inclusive prefix of roff[1..nr]: 8 entries per lane
+      int loc[8], run = 0;
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        int k = 8 * lane + u;
+        run += k < nr ? roff[k + 1] : 0;
+        loc[u] = run;
+      }
+      int incl = run;
+      for (int off = 1; off < 64; off <<= 1) {
+        int t = __shfl_up(incl, off, 64);
+        if (lane >= off) incl += t;
+      }
+      const int excl = incl - run;
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        int k = 8 * lane + u;
+        if (k < nr) roff[k + 1] = excl + loc[u];
+      }
+      if (lane == 0) roff[0] = 0;
+      const int tot = __shfl(incl, 63, 64);
+      if (lane < kSJump) roff[nr + 1 + lane] = tot;  // the jumps of spread_index stop here
+    }
+    __syncthreads();
+    const int total = roff[nr];
+    // wavefront wv takes candidates 64 (WPB r + wv) + lane, r = 0, 1, ...,
+    // kSPAhead at a time: their placements are fetched together, then tested
+    // in candidate order
+    int lo = 0;
+    for (int k0 = wv * 64 + lane; k0 - lane < total; k0 += kSPAhead * BS) {
+      unsigned bc[kSPAhead];
+      double4 tc[kSPAhead];
+      bool vc[kSPAhead];
+#pragma unroll
+      for (int u = 0; u < kSPAhead; ++u) {
+        const int ic = spread_index(roff, rbeg, nr, k0 + u * BS, total, lo);
+        vc[u] = ic >= 0;
+        bc[u] = vc[u] ? sbase[ic] : 0u;
+        tc[u] = vc[u] ? st[ic] : make_double4(0.0, 0.0, 0.0, 0.0);
+      }
+#pragma unroll
+      for (int u = 0; u < kSPAhead; ++u) {
+        if (k0 - lane + u * BS >= total) break;  // wavefront-uniform
+        bool reach = vc[u];
+        unsigned rel = 0;
+#pragma unroll
+        for (int ax = 0; ax < 3; ++ax) {
+          int d = (int)((bc[u] >> (10 * ax)) & 1023u) - org[ax];
+          d += d < 0 ? dims[ax] : 0;
+          reach = reach && (d < ext[ax] || d > dims[ax] - NU);
+          rel |= (unsigned)((d < ext[ax] ? d : d - dims[ax]) + NU - 1) << (8 * ax);
+        }
+        const unsigned long long bm = __ballot(reach);
+        if (reach) {
+          const int slot = npend + __popcll(bm & below);
+          Qw[slot] = tc[u];
+          Rw[slot] = rel;
+        }
+        npend += __popcll(bm);
+        while (npend >= kSPFlush) {
+          flush(kSPFlush);
+          // move the leftover queue entries [kSPFlush, npend) to the front
+          const bool mv = kSPFlush + lane < npend;
+          const double4 rest = Qw[min(kSPFlush + lane, kSPQueue - 1)];
+          const unsigned rrel = Rw[min(kSPFlush + lane, kSPQueue - 1)];
+          __builtin_amdgcn_wave_barrier();
+          if (mv) Qw[lane] = rest, Rw[lane] = rrel;
+          npend -= kSPFlush;
+          __builtin_amdgcn_wave_barrier();
+        }
+      }
+    }
+  }
+  if (npend > 0) flush(npend);  // the last, partial group of atoms
+  // ---- sum the wavefronts' tiles in a fixed order and store
+  if constexpr (WPB > 1) {
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < 4; ++i) W[(wv * 4 + i) * 64 + lane] = acc[i];
+    __syncthreads();
+  }
+  if (wv == 0) {
+    // f64 16x16x4 result layout: register i of lane l holds row 4 i + l / 16,
+    // column l % 16
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      double v = acc[i];
+      for (int w = 1; w < WPB; ++w) v += W[(w * 4 + i) * 64 + lane];
+      const int x = gx0 + 4 * i + (lane >> 4), col = lane & 15;
+      const int y = gy0 + (col >> 2), z = gz0 + (col & 3);
+      if (x < g.x && y < g.y && z < g.z) {
+        const long id = ((long)x * g.y + y) * g.z + z;
+        q1[id] = v;
+        if (q1b) q1b[id] = bf16_split(v);
+      }
+    }
+  }
+}
+
 // ---------------------------------------------------------------- restrict
 // restrict_ (forcefield1.c:401-477): q^l_m = sum_n J_nx J_ny J_nz
 // q^{l-1}_{2m+n}, n in [-NU/2, NU/2]^3, periodic, as separable passes like
@@ -1154,6 +1422,7 @@ __global__ __launch_bounds__(kRBS) void k_restrict(DevParams P, GridDims f, Grid
                                                   const double *__restrict__ qf,
                                                   double *__restrict__ qc,
                                                   unsigned *__restrict__ qcb) {
+  chain_issue_priority();
   // coarse brick of kRB^3 points; fine indices 2m + n - H, n in [0, NU]
   constexpr int H = NU / 2, FW = 2 * (kRB - 1) + NU + 1;
   __shared__ double fs[FW * FW * FW];
@@ -1268,6 +1537,7 @@ __global__ __launch_bounds__(kTopBS) void k_top_small(DevParams P, GridDims g,
                                                       const double *__restrict__ khat,
                                                       const double *__restrict__ tw,
                                                       double *__restrict__ eL, int asis) {
+  chain_issue_priority();
   extern __shared__ double top_lds[];  // 4 n doubles: two complex buffers
   long n = g.size();
   double *re = top_lds, *im = top_lds + n, *re2 = top_lds + 2 * n, *im2 = top_lds + 3 * n;
@@ -1293,6 +1563,7 @@ __global__ __launch_bounds__(256) void k_top_pack(DevParams P, GridDims g,
                                                   const double *__restrict__ qL,
                                                   double *__restrict__ re,
                                                   double *__restrict__ im, int asis) {
+  chain_issue_priority();
   long e = (long)blockIdx.x * 256 + threadIdx.x;
   if (e >= g.size()) return;
   int z = e % g.z, y = (e / g.z) % g.y, x = e / ((long)g.y * g.z);
@@ -1307,6 +1578,7 @@ __global__ __launch_bounds__(256) void k_dft_axis(GridDims g, int axis, double s
                                                   double *__restrict__ oim,
                                                   const double *__restrict__ tw,
                                                   const double *__restrict__ mul) {
+  chain_issue_priority();
   long e = (long)blockIdx.x * 256 + threadIdx.x;
   if (e >= g.size()) return;
   int M = axis == 0 ? g.x : axis == 1 ? g.y : g.z;
@@ -1329,6 +1601,7 @@ __global__ __launch_bounds__(256) void k_dft_axis(GridDims g, int axis, double s
 
 __global__ __launch_bounds__(256) void k_top_out(long n, const double *__restrict__ re,
                                                  double *__restrict__ eL) {
+  chain_issue_priority();
   long e = (long)blockIdx.x * 256 + threadIdx.x;
   if (e < n) eL[e] = re[e] / (double)n;
 }
@@ -1416,6 +1689,7 @@ __global__ __launch_bounds__(kFftBS) void k_fft_pass(DevParams P, GridDims g, Ff
                                                      const double *__restrict__ khat,
                                                      const double *__restrict__ qL,
                                                      double *__restrict__ eL, int asis) {
+  chain_issue_priority();
   extern __shared__ double2 fft_lds[];
   const int M = A.M, B = A.B;
   const long inner = axis == 2 ? 1 : axis == 1 ? (long)g.z : (long)g.y * g.z;
@@ -1569,6 +1843,7 @@ __global__ __launch_bounds__(kGridBS) void k_grid_energy(DevParams P, GridDims g
                                                          const double *__restrict__ q1,
                                                          const double *__restrict__ e1,
                                                          double *__restrict__ epart, int asis) {
+  chain_issue_priority();
   long n = g.size();
   double s = 0.0;
   for (long e = (long)blockIdx.x * kGridBS + threadIdx.x; e < n; e += (long)gridDim.x * kGridBS) {
@@ -1657,6 +1932,7 @@ __device__ void finalize_tail(const DevParams &P, const FinalizeArgs &f, double 
 
 constexpr int kFinBS = 1024;
 __global__ __launch_bounds__(kFinBS) void k_finalize(DevParams P, FinalizeArgs f) {
+  chain_issue_priority();
   double s = 0, c = 0;
   for (int k = threadIdx.x; k < f.np; k += kFinBS) neumaier(s, c, f.ep[k]);
   finalize_tail<kFinBS>(P, f, s, c);
@@ -1737,6 +2013,7 @@ __global__ __launch_bounds__(256) void k_gather(DevParams P, GridDims g,
                                                 const double *__restrict__ e1,
                                                 const double *__restrict__ scal,
                                                 double *__restrict__ F, double *__restrict__ phi) {
+  chain_issue_priority();
   int i = blockIdx.x * 256 + threadIdx.x;
   if (i < P.N) gather_atom<NU, ADD, TRI>(P, g, spos, sidx, e1, scal, F, phi, i);
 }
@@ -1853,6 +2130,7 @@ __global__ __launch_bounds__(256) void k_gather_tile(DevParams P, GridDims g, Ti
                                                      const double *__restrict__ scal,
                                                      double *__restrict__ F,
                                                      double *__restrict__ phi) {
+  chain_issue_priority();
   extern __shared__ double reg[];
   __shared__ int seg_off[kTileMax * kTileMax + 1], seg_beg[kTileMax * kTileMax];
   const int b = blockIdx.x, t = threadIdx.x;
@@ -1929,6 +2207,7 @@ __global__ __launch_bounds__(256) void k_combine(DevParams P, const double *__re
                                                  double *__restrict__ F,
                                                  double *__restrict__ phi, double *__restrict__ ep2,
                                                  FinalizeArgs fin) {
+  chain_issue_priority();
   __shared__ double ss[256], cc[256];
   __shared__ int last;
   const int t = threadIdx.x, b = blockIdx.x;
@@ -2050,12 +2329,37 @@ void launch_pair_fp32(const DevParams &P, const double4 *spos, const int *sidx, 
 
 void launch_pair_count(const DevParams &P, const double4 *spos, const int *start, const int4 *cols,
                        int ncol, double r2max, unsigned long long *count, hipStream_t s) {
-  k_pair_count<<<P.ncell, 64, 0, s>>>(P, spos, start, cols, ncol, r2max, count);
+  const double w = 2.0 * std::sqrt(r2max) * (1.0 + 1e-12);  // rounding of d / L stays below 1/2
+  if (!P.tri && w < P.box[0] && w < P.box[1] && w < P.box[2])
+    k_pair_count<false><<<P.ncell, 64, 0, s>>>(P, spos, start, cols, ncol, r2max, count);
+  else
+    k_pair_count<true><<<P.ncell, 64, 0, s>>>(P, spos, start, cols, ncol, r2max, count);
+}
+
+bool spread_by_placements(const DevParams &P, GridDims g) {
+  return g.x >= kSX + P.nu && g.y >= kSY + P.nu && g.z >= kSZ + P.nu && g.x < 1024 &&
+         g.y < 1024 && g.z < 1024;
 }
 
 void launch_spread(const DevParams &P, GridDims g, const double4 *spos, const int *start,
-                   double *q1, hipStream_t s, unsigned *q1b) {
+                   double *q1, hipStream_t s, unsigned *q1b, unsigned *sbase, double4 *st) {
   unsigned nb = (unsigned)(((g.x + kSX - 1) / kSX) * ((g.y + kSY - 1) / kSY) * ((g.z + kSZ - 1) / kSZ));
+  if (sbase && st && spread_by_placements(P, g)) {
+    if (P.tri)
+      k_spread_place<true><<<nblk(P.N, 256), 256, 0, s>>>(P, g, spos, sbase, st);
+    else
+      k_spread_place<false><<<nblk(P.N, 256), 256, 0, s>>>(P, g, spos, sbase, st);
+    if (nb >= kSpreadFine) {
+#define CALL(NU) k_spread_bricks<NU, 1><<<nb, 64, 0, s>>>(P, g, sbase, st, start, q1, q1b)
+      MSM_NU_SWITCH(P.nu, CALL)
+#undef CALL
+    } else {
+#define CALL(NU) k_spread_bricks<NU, 4><<<nb, 256, 0, s>>>(P, g, sbase, st, start, q1, q1b)
+      MSM_NU_SWITCH(P.nu, CALL)
+#undef CALL
+    }
+    return;
+  }
   if (nb >= kSpreadFine) {
 #define CALL(NU)                                                    \
   if (P.tri) k_spread_mfma<NU, 1, true><<<nb, 64, 0, s>>>(P, g, spos, start, q1, q1b); \

@@ -100,6 +100,7 @@ struct BRow<MSM_G2G_BF16X3, NCH> {
 
 template <int PREC, int NCH>
 __global__ __launch_bounds__(64) void k_g2g_tiles(G2GArgs args) {
+  chain_issue_priority();
   extern __shared__ __align__(16) unsigned char lds_raw[];
   constexpr int KC = Cfg<PREC>::KC, KPL = Cfg<PREC>::KPL, PAD = Cfg<PREC>::PAD;
   long task = blockIdx.x;
@@ -202,6 +203,7 @@ constexpr int kSlabU = 16;  // slab elements per thread and batch (loads in flig
 
 template <int NCH, int T>
 __global__ __launch_bounds__(64 * kRowWaves) void k_g2g_rows(G2GArgs args) {
+  chain_issue_priority();
   extern __shared__ __align__(16) unsigned char lds_raw[];
   constexpr int KC = 32, KPL = 8, PAD = 8, BS = 64 * kRowWaves;
   long task = blockIdx.x;
@@ -336,6 +338,7 @@ __global__ __launch_bounds__(256) void k_prolong_sum(DevParams P, GridDims g, Gr
                                                      double *__restrict__ el,
                                                      const double *__restrict__ q1,
                                                      double *__restrict__ epart) {
+  chain_issue_priority();
   constexpr int H = NU / 2, CW = H + 4, B = kPB;
   __shared__ double cs[CW * CW * CW];
   __shared__ double t1[CW * CW * B];  // [cx][cy][fz]
