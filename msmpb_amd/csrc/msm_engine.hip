@@ -107,7 +107,7 @@ struct Engine {
   unsigned *d_sbase = nullptr;
   double4 *d_st = nullptr;
   int4 *d_sten = nullptr;
-  unsigned long long *d_count = nullptr;
+  unsigned *d_count = nullptr;  // pair counts per pair-search cell
   double *d_ql[msm::kMaxLevel + 1] = {}, *d_el[msm::kMaxLevel + 1] = {};
   unsigned *d_qb[msm::kMaxLevel + 1] = {};  // q^l as bf16 hi/lo pairs (l < L)
   double *d_khat[msm::kMaxLevel + 1] = {};
@@ -623,7 +623,7 @@ int create_engine(FFX *x, const msm::Plan &plan) {
   HIP_TRY(hipMemset(E->d_done, 0, sizeof(unsigned)));
   TRY(dalloc(&E->d_wpos, N));
   TRY(dalloc(&E->d_spos, N));
-  TRY(dalloc(&E->d_count, 1));
+  TRY(dalloc(&E->d_count, E->P.ncell));
   if (msm::spread_by_placements(E->P, E->gd[1])) {
     TRY(dalloc(&E->d_sbase, N));
     TRY(dalloc(&E->d_st, N));
@@ -931,12 +931,14 @@ int count_pairs_dev(FFX *x, const double *d_pos, const double *d_q, double margi
     cols = d_cols, ncol = (int)c.size();
   }
   stage_prepare(E, d_pos, d_q, s);
-  HIP_TRY(hipMemsetAsync(E.d_count, 0, sizeof(unsigned long long), s));
-  msm::launch_pair_count(E.P, E.d_spos, E.d_start, cols, ncol, R * R, E.d_count, s);
+  unsigned *cc = E.d_count;
+  msm::launch_pair_count(E.P, E.d_spos, E.d_start, cols, ncol, R * R, cc, s);
   msm::launch_fill_int(E.d_flags, 1, 0, s);  // no energy reduction follows to clear it
-  unsigned long long c = 0;
-  HIP_TRY(hipMemcpyAsync(&c, E.d_count, sizeof c, hipMemcpyDeviceToHost, s));
+  std::vector<unsigned> h(E.P.ncell);
+  HIP_TRY(hipMemcpyAsync(h.data(), cc, h.size() * sizeof(unsigned), hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
+  unsigned long long c = 0;
+  for (unsigned v : h) c += v;
   if (d_cols) hipFree(d_cols);
   *count = (long long)c;
   return MSM_OK;

@@ -97,10 +97,11 @@ void launch_pair_fp64(const DevParams &P, const double4 *spos, const int *sidx, 
 void launch_pair_fp32(const DevParams &P, const double4 *spos, const int *sidx, const int *start,
                       const int4 *cols, int ncol, double *F, double *phi, double *epart,
                       hipStream_t s);
-// Pairs i < j whose nearest image lies within sqrt(r2max), added to *count;
-// cols are cell columns built for that radius.
+// Pairs i < j whose nearest image lies within sqrt(r2max), counted per home
+// cell into cell_count (ncell entries); cols are cell columns built for
+// that radius.
 void launch_pair_count(const DevParams &P, const double4 *spos, const int *start, const int4 *cols,
-                       int ncol, double r2max, unsigned long long *count, hipStream_t s);
+                       int ncol, double r2max, unsigned *cell_count, hipStream_t s);
 
 // B-spline spreading of charges onto the level-1 grid, one workgroup per
 // grid brick (every grid point written exactly once).

@@ -794,12 +794,15 @@ __global__ __launch_bounds__(64) void k_pair_count(DevParams P, const double4 *_
                                                    const int *__restrict__ start,
                                                    const int4 *__restrict__ cols, int ncol,
                                                    double r2max,
-                                                   unsigned long long *__restrict__ count) {
+                                                   unsigned *__restrict__ cell_count) {
   __shared__ int seg_off[kSegMax + 1], seg_beg[kSegMax];
   __shared__ double seg_sx[kSegMax], seg_sy[kSegMax], seg_sz[kSegMax];
   const int c = blockIdx.x, lane = threadIdx.x;
   const int b0 = start[c], n = start[c + 1] - b0;
-  if (n == 0) return;  // wavefront-uniform
+  if (n == 0) {  // wavefront-uniform
+    if (lane == 0) cell_count[c] = 0;
+    return;
+  }
   const int cz = c % P.nc[2], cy = (c / P.nc[2]) % P.nc[1], cx = c / (P.nc[1] * P.nc[2]);
   unsigned cnt = 0;
   // home atoms kPassAtoms at a time, held in registers (wavefront-uniform)
@@ -873,9 +876,10 @@ __global__ __launch_bounds__(64) void k_pair_count(DevParams P, const double4 *_
           const int j = jj[v];  // -1 past the end: below every home index
 #pragma unroll
           for (int u = 0; u < kPassAtoms; ++u) {
+            if (u >= ni) break;  // wavefront-uniform
             const double d[3] = {aj[v].x + (lx[v] - hx[u]), aj[v].y + (ly[v] - hy[u]),
                                  aj[v].z + (lz[v] - hz[u])};
-            bool in = u < ni && j > i0 + u && d[0] * d[0] + d[1] * d[1] + d[2] * d[2] < r2max;
+            bool in = j > i0 + u && d[0] * d[0] + d[1] * d[1] + d[2] * d[2] < r2max;
             if constexpr (IMG) {
               if (in) {
                 double sd[3];
@@ -892,7 +896,7 @@ __global__ __launch_bounds__(64) void k_pair_count(DevParams P, const double4 *_
     }
   }
   for (int off = 32; off > 0; off >>= 1) cnt += __shfl_down(cnt, off, 64);
-  if (lane == 0 && cnt) atomicAdd(count, (unsigned long long)cnt);
+  if (lane == 0) cell_count[c] = cnt;  // one count per cell: no contended atomic
 }
 
 // ---------------------------------------------------------------- spread
@@ -2328,12 +2332,12 @@ void launch_pair_fp32(const DevParams &P, const double4 *spos, const int *sidx, 
 }
 
 void launch_pair_count(const DevParams &P, const double4 *spos, const int *start, const int4 *cols,
-                       int ncol, double r2max, unsigned long long *count, hipStream_t s) {
+                       int ncol, double r2max, unsigned *cell_count, hipStream_t s) {
   const double w = 2.0 * std::sqrt(r2max) * (1.0 + 1e-12);  // rounding of d / L stays below 1/2
   if (!P.tri && w < P.box[0] && w < P.box[1] && w < P.box[2])
-    k_pair_count<false><<<P.ncell, 64, 0, s>>>(P, spos, start, cols, ncol, r2max, count);
+    k_pair_count<false><<<P.ncell, 64, 0, s>>>(P, spos, start, cols, ncol, r2max, cell_count);
   else
-    k_pair_count<true><<<P.ncell, 64, 0, s>>>(P, spos, start, cols, ncol, r2max, count);
+    k_pair_count<true><<<P.ncell, 64, 0, s>>>(P, spos, start, cols, ncol, r2max, cell_count);
 }
 
 bool spread_by_placements(const DevParams &P, GridDims g) {
