@@ -694,7 +694,11 @@ int stage_grids(Engine &E, int precision, int layout, Timing &T, hipStream_t s) 
     HIP_TRY(hipEventRecord(E.ev_q1, s));
     HIP_TRY(hipStreamWaitEvent(E.stream3, E.ev_q1, 0));
     t = T.open(E.stream3);
-    msm::launch_g2g_tiles(pr, g2g_levels(E.g2g[pr], 0, 1), E.g2g_lds[pr], E.stream3);
+    // beside four or more levels the coarse chain outlasts this pass, which
+    // then leaves issue priority to the pair kernel
+    msm::G2GArgs a1 = g2g_levels(E.g2g[pr], 0, 1);
+    a1.urgent = L <= 3;
+    msm::launch_g2g_tiles(pr, a1, E.g2g_lds[pr], E.stream3);
     T.close(t, "grid2grid", E.stream3);
     HIP_TRY(hipEventRecord(E.ev_g1, E.stream3));
   }
@@ -706,7 +710,9 @@ int stage_grids(Engine &E, int precision, int layout, Timing &T, hipStream_t s) 
   }
   if (L >= 3) {  // stencil passes of the levels 2 .. L-1 in one launch
     t = T.open(s);
-    msm::launch_g2g_tiles(pr, g2g_levels(E.g2g[pr], 1, L - 1), E.g2g_lds[pr], s);
+    msm::G2GArgs ac = g2g_levels(E.g2g[pr], 1, L - 1);
+    ac.urgent = 1;
+    msm::launch_g2g_tiles(pr, ac, E.g2g_lds[pr], s);
     T.close(t, "grid2grid", s);
   }
   t = T.open(s);

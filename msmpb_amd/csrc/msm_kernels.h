@@ -12,6 +12,9 @@ namespace msm {
 // wavefronts take instruction issue ahead of the pair kernel's on a shared
 // SIMD (wave priority 3; the pair kernel keeps the default 0).
 __device__ __forceinline__ void chain_issue_priority() { __builtin_amdgcn_s_setprio(3); }
+__device__ __forceinline__ void chain_issue_priority(bool on) {
+  if (on) __builtin_amdgcn_s_setprio(3);
+}
 
 constexpr int kMaxNu = 10;
 
@@ -168,6 +171,7 @@ struct G2GArgs {
   G2GLevel lv[9];
   int nlev;
   long ntasks;
+  int urgent;  // on the chain's critical path: wave priority raised
 };
 // Chooses tiles, K chunks and kx groups for about target_waves wavefronts.
 void plan_g2g_level(int prec, GridDims g, GridDims s, int target_waves, G2GLevel &L);

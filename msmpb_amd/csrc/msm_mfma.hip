@@ -100,7 +100,7 @@ struct BRow<MSM_G2G_BF16X3, NCH> {
 
 template <int PREC, int NCH>
 __global__ __launch_bounds__(64) void k_g2g_tiles(G2GArgs args) {
-  chain_issue_priority();
+  chain_issue_priority(args.urgent);
   extern __shared__ __align__(16) unsigned char lds_raw[];
   constexpr int KC = Cfg<PREC>::KC, KPL = Cfg<PREC>::KPL, PAD = Cfg<PREC>::PAD;
   long task = blockIdx.x;
@@ -203,7 +203,7 @@ constexpr int kSlabU = 16;  // slab elements per thread and batch (loads in flig
 
 template <int NCH, int T>
 __global__ __launch_bounds__(64 * kRowWaves) void k_g2g_rows(G2GArgs args) {
-  chain_issue_priority();
+  chain_issue_priority(args.urgent);
   extern __shared__ __align__(16) unsigned char lds_raw[];
   constexpr int KC = 32, KPL = 8, PAD = 8, BS = 64 * kRowWaves;
   long task = blockIdx.x;
