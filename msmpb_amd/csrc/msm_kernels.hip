@@ -513,28 +513,47 @@ __device__ __forceinline__ void pair_run(const DevParams &P, const typename Pair
   }
 }
 
-// Per-lane select without branches on the array (keeps it in registers).
-__device__ __forceinline__ float lane_pick(float a, float b, int up) {
-  const int x = __float_as_int(a), y = __float_as_int(b);
-  return __int_as_float((y & up) | (x & ~up));
-}
-__device__ __forceinline__ double lane_pick(double a, double b, long long up) {
-  const long long x = __double_as_longlong(a), y = __double_as_longlong(b);
-  return __longlong_as_double((y & up) | (x & ~up));
-}
+// Halving exchanges.  Each sums two values a, b over a pair of lanes: the
+// lane of the pair whose selecting bit is clear receives a summed over both
+// lanes, the other b summed over both.  Lanes 32 apart (v_permlane32_swap:
+// lanes 32-63 of a change places with lanes 0-31 of b) and 16 apart
+// (v_permlane16_swap, in each half) need no selects; the rest go through
+// DPP row moves whose lane pairing is an involution: row_ror:8 (lane ^ 8),
+// row_half_mirror (lane ^ 7), quad_perm (lane ^ 2, lane ^ 1).
+constexpr int kDppRor8 = 0x128, kDppHalfMirror = 0x141, kDppXor2 = 0x4E, kDppXor1 = 0xB1;
 
-// One halving exchange: lanes with bit m set keep values [H, 2H), the others
-// [0, H), each summed with the partner lane's (lane ^ m) copy.
-template <int H, typename X, int NV>
-__device__ __forceinline__ void halve(X (&v)[NV], int lane, int m) {
-  using I = typename std::conditional<sizeof(X) == 8, long long, int>::type;
-  const I up = (lane & m) ? (I)-1 : (I)0;
-#pragma unroll
-  for (int k = 0; k < H; ++k) {
-    const X keep = lane_pick(v[k], v[k + H], up);
-    const X give = lane_pick(v[k + H], v[k], up);
-    v[k] = keep + __shfl_xor(give, m, 64);
-  }
+template <bool S32>
+__device__ __forceinline__ unsigned swap_lanes(unsigned &a, unsigned b) {
+  const auto r = S32 ? __builtin_amdgcn_permlane32_swap(a, b, false, false)
+                     : __builtin_amdgcn_permlane16_swap(a, b, false, false);
+  a = r[0];
+  return r[1];
+}
+template <bool S32>
+__device__ __forceinline__ float pair_sum_swap(float a, float b) {
+  unsigned x = __float_as_uint(a);
+  const unsigned y = swap_lanes<S32>(x, __float_as_uint(b));
+  return __uint_as_float(x) + __uint_as_float(y);
+}
+template <bool S32>
+__device__ __forceinline__ double pair_sum_swap(double a, double b) {
+  const unsigned long long ua = __double_as_longlong(a), ub = __double_as_longlong(b);
+  unsigned alo = (unsigned)ua, ahi = (unsigned)(ua >> 32);
+  const unsigned blo = swap_lanes<S32>(alo, (unsigned)ub);
+  const unsigned bhi = swap_lanes<S32>(ahi, (unsigned)(ub >> 32));
+  return __longlong_as_double((long long)(((unsigned long long)ahi << 32) | alo)) +
+         __longlong_as_double((long long)(((unsigned long long)bhi << 32) | blo));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_move(double x) {
+  const unsigned long long u = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_mov_dpp((int)u, CTRL, 0xF, 0xF, true);
+  const int hi = __builtin_amdgcn_mov_dpp((int)(u >> 32), CTRL, 0xF, 0xF, true);
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+template <int CTRL>
+__device__ __forceinline__ double pair_sum_dpp(double a, double b, bool up) {
+  return (up ? b : a) + dpp_move<CTRL>(up ? a : b);
 }
 
 // Sum over the wavefront of the run's partials s[g][d], scattered: lane l
@@ -544,22 +563,26 @@ __device__ __forceinline__ void halve(X (&v)[NV], int lane, int m) {
 template <bool MIXED, bool W>
 __device__ __forceinline__ double wave_sum_scatter(typename PairPrec<MIXED>::R (&s)[kPassAtoms][4],
                                                    int lane) {
-  using R = typename PairPrec<MIXED>::R;
   constexpr int V = W ? 32 : 16;
-  R f[32];
+  double d[V / 2];
 #pragma unroll
-  for (int k = 0; k < 32; ++k) f[k] = s[k >> 2][k & 3];
-  halve<V / 2>(f, lane, 32);
-  double d[16];
+  for (int k = 0; k < V / 2; ++k)  // lane bit 5 selects value k + V/2
+    d[k] = (double)pair_sum_swap<true>(s[k >> 2][k & 3], s[(k + V / 2) >> 2][(k + V / 2) & 3]);
 #pragma unroll
-  for (int k = 0; k < V / 2; ++k) d[k] = (double)f[k];
-  halve<V / 4>(d, lane, 16);
-  halve<V / 8>(d, lane, 8);
-  halve<V / 16>(d, lane, 4);
-  if constexpr (W) halve<1>(d, lane, 2);
-  double r = d[0];
-  if constexpr (!W) r += __shfl_xor(r, 2, 64);
-  r += __shfl_xor(r, 1, 64);
+  for (int k = 0; k < V / 4; ++k) d[k] = pair_sum_swap<false>(d[k], d[k + V / 4]);  // bit 4
+  const bool b3 = lane & 8, b2 = lane & 4, b1 = lane & 2;
+#pragma unroll
+  for (int k = 0; k < V / 8; ++k) d[k] = pair_sum_dpp<kDppRor8>(d[k], d[k + V / 8], b3);
+#pragma unroll
+  for (int k = 0; k < V / 16; ++k) d[k] = pair_sum_dpp<kDppHalfMirror>(d[k], d[k + V / 16], b2);
+  double r;
+  if constexpr (W) {
+    r = pair_sum_dpp<kDppXor2>(d[0], d[1], b1);
+  } else {
+    r = d[0];
+    r += dpp_move<kDppXor2>(r);
+  }
+  r += dpp_move<kDppXor1>(r);
   return r;
 }
 
