@@ -1565,23 +1565,25 @@ __global__ __launch_bounds__(kTopBS) void k_top_small(DevParams P, GridDims g,
                                                       const double *__restrict__ tw,
                                                       double *__restrict__ eL, int asis) {
   chain_issue_priority();
-  extern __shared__ double top_lds[];  // 4 n doubles: two complex buffers
+  extern __shared__ double top_lds[];  // 4 n doubles: two complex buffers; the twiddles
   long n = g.size();
   double *re = top_lds, *im = top_lds + n, *re2 = top_lds + 2 * n, *im2 = top_lds + 3 * n;
+  double *const twl = top_lds + 4 * n;  // read once per DFT term: kept in LDS
+  for (int k = threadIdx.x; k < 2 * (g.x + g.y + g.z); k += kTopBS) twl[k] = tw[k];
   for (long e = threadIdx.x; e < n; e += kTopBS) {
     int z = e % g.z, y = (e / g.z) % g.y, x = e / ((long)g.y * g.z);
     re[e] = top_input(g, qL, x, y, z, asis, P.nu);
     im[e] = 0.0;
   }
   __syncthreads();
-  dft_pass_lds(g, 2, -1.0, re, im, re2, im2, tw);
-  dft_pass_lds(g, 1, -1.0, re2, im2, re, im, tw);
-  dft_pass_lds(g, 0, -1.0, re, im, re2, im2, tw);
+  dft_pass_lds(g, 2, -1.0, re, im, re2, im2, twl);
+  dft_pass_lds(g, 1, -1.0, re2, im2, re, im, twl);
+  dft_pass_lds(g, 0, -1.0, re, im, re2, im2, twl);
   for (long e = threadIdx.x; e < n; e += kTopBS) re2[e] *= khat[e], im2[e] *= khat[e];
   __syncthreads();
-  dft_pass_lds(g, 0, 1.0, re2, im2, re, im, tw);
-  dft_pass_lds(g, 1, 1.0, re, im, re2, im2, tw);
-  dft_pass_lds(g, 2, 1.0, re2, im2, re, im, tw);
+  dft_pass_lds(g, 0, 1.0, re2, im2, re, im, twl);
+  dft_pass_lds(g, 1, 1.0, re, im, re2, im2, twl);
+  dft_pass_lds(g, 2, 1.0, re2, im2, re, im, twl);
   for (long e = threadIdx.x; e < n; e += kTopBS) eL[e] = re[e] / (double)n;
 }
 
@@ -2431,8 +2433,9 @@ bool fft_axis_plan(int M, FftAxis &A) {
 void launch_top(const DevParams &P, GridDims g, const double *qL, const double *khat,
                 const double *tw, double *work, double *eL, int asis, hipStream_t s) {
   long n = g.size();
-  if (n <= kTopSmall) {
-    k_top_small<<<1, kTopBS, 4 * n * sizeof(double), s>>>(P, g, qL, khat, tw, eL, asis);
+  const long small_lds = (4 * n + 2 * (g.x + g.y + g.z)) * (long)sizeof(double);
+  if (n <= kTopSmall && small_lds <= 65536) {
+    k_top_small<<<1, kTopBS, (size_t)small_lds, s>>>(P, g, qL, khat, tw, eL, asis);
     return;
   }
   FftAxis ax[3];

@@ -326,8 +326,27 @@ __global__ __launch_bounds__(64 * kRowWaves) void k_g2g_rows(G2GArgs args) {
 // of the stencil pass in a fixed order and stores every point once.  With
 // q1 != null (the finest level) it also writes the brick's sum of q1 e1,
 // the grid-energy partial.
-__device__ __forceinline__ int floor_half(int a) { return a >> 1; }
-__device__ __forceinline__ int ceil_half(int a) { return -((-a) >> 1); }
+// Prolongation at fine index v >= 0 (forcefield1.c:530-567): the sum over
+// the coarse m with |v - 2m| <= H of J_{v-2m+H} c(m), in ascending m.  With
+// m = v/2 + d the weight of offset d is J_{H-2d} (v even) or J_{H+1-2d} (v
+// odd): compile-time entries of J chosen by the parity, where an index into
+// J that varies from lane to lane costs a memory load per tap.  Offsets
+// whose weight is zero for the parity of v add exact zeros.
+template <int NU, typename F>
+__device__ __forceinline__ double prolong_taps(const double *J, int v, F c) {
+  constexpr int H = NU / 2;
+  const int j = v >> 1;
+  const bool odd = v & 1;
+  double acc = 0.0;
+#pragma unroll
+  for (int d = -(H / 2); d <= (H + 1) / 2; ++d) {
+    const int i0 = H - 2 * d, i1 = H + 1 - 2 * d;
+    const double w0 = (i0 >= 0 && i0 <= NU) ? J[i0] : 0.0;
+    const double w1 = (i1 >= 0 && i1 <= NU) ? J[i1] : 0.0;
+    acc += (odd ? w1 : w0) * c(j + d);
+  }
+  return acc;
+}
 
 constexpr int kPB = 8;  // fine brick edge of the prolongation
 
@@ -360,18 +379,12 @@ __global__ __launch_bounds__(256) void k_prolong_sum(DevParams P, GridDims g, Gr
   __syncthreads();
   for (int k = tid; k < CW * CW * B; k += 256) {  // along z
     const int fz = k % B, cxy = k / B, v = fz0 + fz;
-    double acc = 0.0;
-    for (int m = ceil_half(v - H); m <= floor_half(v + H); ++m)
-      acc += P.J[v - 2 * m + H] * cs[cxy * CW + (m - cz0)];
-    t1[k] = acc;
+    t1[k] = prolong_taps<NU>(P.J, v, [&](int m) { return cs[cxy * CW + (m - cz0)]; });
   }
   __syncthreads();
   for (int k = tid; k < CW * B * B; k += 256) {  // along y
     const int fz = k % B, fy = (k / B) % B, cx = k / (B * B), v = fy0 + fy;
-    double acc = 0.0;
-    for (int m = ceil_half(v - H); m <= floor_half(v + H); ++m)
-      acc += P.J[v - 2 * m + H] * t1[(cx * CW + (m - cy0)) * B + fz];
-    t2[k] = acc;
+    t2[k] = prolong_taps<NU>(P.J, v, [&](int m) { return t1[(cx * CW + (m - cy0)) * B + fz]; });
   }
   __syncthreads();
   const long n = g.size();
@@ -380,9 +393,7 @@ __global__ __launch_bounds__(256) void k_prolong_sum(DevParams P, GridDims g, Gr
     const int fz = k % B, fy = (k / B) % B, fx = k / (B * B);
     const int x = fx0 + fx, y = fy0 + fy, z = fz0 + fz;
     if (x >= g.x || y >= g.y || z >= g.z) continue;
-    double acc = 0.0;
-    for (int m = ceil_half(x - H); m <= floor_half(x + H); ++m)
-      acc += P.J[x - 2 * m + H] * t2[((m - cx0) * B + fy) * B + fz];
+    double acc = prolong_taps<NU>(P.J, x, [&](int m) { return t2[((m - cx0) * B + fy) * B + fz]; });
     const long id = ((long)x * g.y + y) * g.z + z;
     for (int gi = 0; gi < G; ++gi) acc += part[(long)gi * n + id];
     el[id] = acc;
