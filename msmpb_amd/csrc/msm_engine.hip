@@ -642,22 +642,26 @@ int create_engine(FFX *x, const msm::Plan &plan) {
 // ---- the evaluation chain ------------------------------------------------
 // Binning + counting sort of atoms into cells (replaces the stored
 // neighbour list of forcefield0.c:21-164).
-void stage_prepare(Engine &E, const double *pos, const double *q, hipStream_t s) {
+// fork (optional): bound to the completion of the last binning kernel, for
+// the pair stream to wait on.
+void stage_prepare(Engine &E, const double *pos, const double *q, hipStream_t s,
+                   hipEvent_t fork = nullptr) {
   // d_cnt is zero here: set at allocation, cleared again by every cell sort
   msm::launch_bin(E.P, pos, q, E.d_cell_of, E.d_rank, E.d_cnt, E.d_wpos, E.d_qpart, E.d_flags, s);
   msm::launch_scan(E.P, E.d_cnt, E.d_start, E.d_qpart, msm::bin_blocks(E.P.N), E.d_scal, s);
   msm::launch_scatter(E.P, E.d_cell_of, E.d_rank, E.d_wpos, E.d_start, E.d_spos, E.d_sidx,
                       E.d_scell, s);
-  msm::launch_cell_sort(E.P, E.d_start, E.d_cnt, E.d_spos, E.d_sidx, s);
+  msm::launch_cell_sort(E.P, E.d_start, E.d_cnt, E.d_spos, E.d_sidx, s, fork);
 }
 
-void stage_pairs(Engine &E, int precision, double *F, double *phi, hipStream_t s) {
+void stage_pairs(Engine &E, int precision, double *F, double *phi, hipStream_t s,
+                 hipEvent_t done = nullptr) {
   if (precision == MSM_PREC_MIXED)
     msm::launch_pair_fp32(E.P, E.d_spos, E.d_sidx, E.d_start, E.d_sten, E.nsten, F, phi,
-                          E.d_epair, s);
+                          E.d_epair, s, done);
   else
     msm::launch_pair_fp64(E.P, E.d_spos, E.d_sidx, E.d_start, E.d_sten, E.nsten, F, phi,
-                          E.d_epair, s);
+                          E.d_epair, s, done);
 }
 
 // The levels [first, last) of the level-stencil launch description, as a
@@ -687,20 +691,20 @@ int stage_grids(Engine &E, int precision, int layout, Timing &T, hipStream_t s) 
   int t = T.open(s);
   // the BF16 stencils read q^l as bf16 hi/lo pairs, written with q^l
   const bool split = pr == msm::MSM_G2G_BF16X3;
+  // ev_q1 (q^1 ready) rides on the spreading kernel's dispatch
   msm::launch_spread(E.P, E.gd[1], E.d_spos, E.d_start, E.d_ql[1], s,
-                     split && L >= 2 ? E.d_qb[1] : nullptr, E.d_sbase, E.d_st);
+                     split && L >= 2 ? E.d_qb[1] : nullptr, E.d_sbase, E.d_st,
+                     L >= 2 ? E.ev_q1 : nullptr);
   T.close(t, "anterpolation", s);
   if (L >= 2) {
-    HIP_TRY(hipEventRecord(E.ev_q1, s));
     HIP_TRY(hipStreamWaitEvent(E.stream3, E.ev_q1, 0));
     t = T.open(E.stream3);
     // beside four or more levels the coarse chain outlasts this pass, which
     // then leaves issue priority to the pair kernel
     msm::G2GArgs a1 = g2g_levels(E.g2g[pr], 0, 1);
     a1.urgent = L <= 3;
-    msm::launch_g2g_tiles(pr, a1, E.g2g_lds[pr], E.stream3);
+    msm::launch_g2g_tiles(pr, a1, E.g2g_lds[pr], E.stream3, E.ev_g1);
     T.close(t, "grid2grid", E.stream3);
-    HIP_TRY(hipEventRecord(E.ev_g1, E.stream3));
   }
   for (int l = 2; l <= L; ++l) {
     t = T.open(s);
@@ -778,14 +782,13 @@ int check_built(FF *ff) {
 
 // Pairs on the pair stream, concurrently with the grid chain; the caller
 // joins before the reduction that adds the pair results (k_combine or
-// k_finalize).
-int fork_pairs(Engine &E, int precision, double *F, double *phi, hipStream_t s, Timing &T) {
-  HIP_TRY(hipEventRecord(E.ev_fork, s));
+// k_finalize).  The binning bound ev_fork to its last kernel (stage_prepare
+// with E.ev_fork); ev_join rides on the pair kernel.
+int fork_pairs(Engine &E, int precision, double *F, double *phi, Timing &T) {
   HIP_TRY(hipStreamWaitEvent(E.stream2, E.ev_fork, 0));
   int t = T.open(E.stream2);
-  stage_pairs(E, precision, F, phi, E.stream2);
+  stage_pairs(E, precision, F, phi, E.stream2, E.ev_join);
   T.close(t, "partcl2partcl", E.stream2);
-  HIP_TRY(hipEventRecord(E.ev_join, E.stream2));
   return MSM_OK;
 }
 
@@ -794,13 +797,13 @@ int eval_chain(FFX *x, const double *pos, const double *q, double *out, double *
                hipStream_t s, Timing &T) {
   Engine &E = *x->eng;
   int t = T.open(s);
-  stage_prepare(E, pos, q, s);
+  stage_prepare(E, pos, q, s, E.ev_fork);
   T.close(t, "nlist", s);
   // the pair kernel runs beside the whole grid chain including the
   // interpolation (the chain's kernels are sized -- registers, LDS -- to fit
   // next to its resident wavefronts); it writes the short-range parts of F
   // and phi to its own buffers, added after the join
-  TRY(fork_pairs(E, x->precision, F ? E.d_Fp : nullptr, phi ? E.d_phip : nullptr, s, T));
+  TRY(fork_pairs(E, x->precision, F ? E.d_Fp : nullptr, phi ? E.d_phip : nullptr, T));
   TRY(stage_grids(E, x->precision, x->layout, T, s));
   stage_grid_energy(E, x->layout, s);
   if (F || phi) {
@@ -1482,8 +1485,8 @@ int msm_forward_device(FF *ff, const double *d_pos, const double *d_q, double *d
   // with state kept for the adjoint the pair kernel also writes its
   // per-atom sums (the short-range parts of -dE/dr and dE/dq)
   const bool kp = keep != MSM_KEEP_NONE;
-  stage_prepare(E, d_pos, d_q, s);
-  TRY(fork_pairs(E, x->precision, kp ? E.d_Fp : nullptr, kp ? E.d_phip : nullptr, s, T));
+  stage_prepare(E, d_pos, d_q, s, E.ev_fork);
+  TRY(fork_pairs(E, x->precision, kp ? E.d_Fp : nullptr, kp ? E.d_phip : nullptr, T));
   TRY(stage_grids(E, x->precision, x->layout, T, s));
   stage_grid_energy(E, x->layout, s);
   HIP_TRY(hipStreamWaitEvent(s, E.ev_join, 0));

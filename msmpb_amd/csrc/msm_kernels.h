@@ -2,7 +2,10 @@
 // evaluation chain (implemented in msm_kernels.hip).
 #pragma once
 
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
+
+#include <cstdint>
 
 #include "msm_plan.h"  // kMaxLevel
 
@@ -67,6 +70,19 @@ struct FftAxis {
 bool fft_axis_plan(int M, FftAxis &A);
 
 // ---- launchers (stream-ordered, no synchronisation) ----
+// `done` (optional, where a launcher takes it): an event bound to the
+// completion of the launcher's last kernel, for another stream to wait on.
+// It rides on that kernel's dispatch (hipExtLaunchKernel), where a separate
+// hipEventRecord adds a marker packet that delays the next kernel of the
+// recording stream by several microseconds.
+template <typename K, typename... A>
+void launch_done(K kernel, dim3 grid, dim3 block, size_t lds, hipStream_t s, hipEvent_t done,
+                 A... args) {
+  if (done)
+    hipExtLaunchKernelGGL(kernel, grid, block, (std::uint32_t)lds, s, nullptr, done, 0u, args...);
+  else
+    kernel<<<grid, block, lds, s>>>(args...);
+}
 void launch_fill_int(int *p, long n, int v, hipStream_t s);
 void launch_fill_dbl(double *p, long n, double v, hipStream_t s);
 
@@ -86,7 +102,7 @@ void launch_scatter(const DevParams &P, const int *cell_of, const int *rank, con
 // Orders atoms inside each cell by original index (deterministic results)
 // and clears cnt for the next binning.
 void launch_cell_sort(const DevParams &P, const int *start, int *cnt, double4 *spos, int *sidx,
-                      hipStream_t s);
+                      hipStream_t s, hipEvent_t done = nullptr);
 
 // Short-range pairs, full neighbour shell, one wavefront per home cell;
 // cols: (dx, dy, dzlo, dzhi) neighbour cell columns, dzhi - dzlo < nc[2].
@@ -96,10 +112,10 @@ void launch_cell_sort(const DevParams &P, const int *start, int *cnt, double4 *s
 int pair_blocks(int N);
 void launch_pair_fp64(const DevParams &P, const double4 *spos, const int *sidx, const int *start,
                       const int4 *cols, int ncol, double *F, double *phi, double *epart,
-                      hipStream_t s);
+                      hipStream_t s, hipEvent_t done = nullptr);
 void launch_pair_fp32(const DevParams &P, const double4 *spos, const int *sidx, const int *start,
                       const int4 *cols, int ncol, double *F, double *phi, double *epart,
-                      hipStream_t s);
+                      hipStream_t s, hipEvent_t done = nullptr);
 // Pairs i < j whose nearest image lies within sqrt(r2max), counted per home
 // cell into cell_count (ncell entries); cols are cell columns built for
 // that radius.
@@ -113,7 +129,7 @@ void launch_pair_count(const DevParams &P, const double4 *spos, const int *start
 // spread_by_placements(P, g) holds.
 void launch_spread(const DevParams &P, GridDims g, const double4 *spos, const int *start,
                    double *q1, hipStream_t s, unsigned *q1b = nullptr, unsigned *sbase = nullptr,
-                   double4 *st = nullptr);
+                   double4 *st = nullptr, hipEvent_t done = nullptr);
 bool spread_by_placements(const DevParams &P, GridDims g);
 // q^l = R q^{l-1} (full weighting with the two-scale stencil).
 // qcb (optional): q^l also as bf16_split pairs.
@@ -185,7 +201,7 @@ void g2g_bfrag_fill(int prec, const G2GLevel &L, const double *khat, void *host_
 size_t g2g_lds_bytes(int prec, const G2GLevel &L);
 int g2g_max_nch(int prec);
 // All levels' stencil correlations in one launch (one wavefront per task).
-void launch_g2g_tiles(int prec, G2GArgs args, size_t lds, hipStream_t s);
+void launch_g2g_tiles(int prec, G2GArgs args, size_t lds, hipStream_t s, hipEvent_t done = nullptr);
 // e^l = sum of the G partials + prolongation of e^{l+1}; with q1 != null
 // also one partial of sum q1 e1 per workgroup into epart (prolong_blocks(g)).
 int prolong_blocks(GridDims g);

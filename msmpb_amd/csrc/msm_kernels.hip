@@ -2306,11 +2306,11 @@ void launch_scatter(const DevParams &P, const int *cell_of, const int *rank, con
 }
 
 void launch_cell_sort(const DevParams &P, const int *start, int *cnt, double4 *spos, int *sidx,
-                      hipStream_t s) {
+                      hipStream_t s, hipEvent_t done) {
   if (P.N <= 6L * P.ncell)  // a few atoms per cell: 16 lanes each
-    k_cell_sort<16><<<nblk(P.ncell, 16), 256, 0, s>>>(P, start, cnt, spos, sidx);
+    launch_done(k_cell_sort<16>, nblk(P.ncell, 16), 256, 0, s, done, P, start, cnt, spos, sidx);
   else
-    k_cell_sort<64><<<nblk(P.ncell, 4), 256, 0, s>>>(P, start, cnt, spos, sidx);
+    launch_done(k_cell_sort<64>, nblk(P.ncell, 4), 256, 0, s, done, P, start, cnt, spos, sidx);
 }
 
 int pair_blocks(int N) { return (int)nblk(N, kPairBS); }
@@ -2326,12 +2326,14 @@ int pair_blocks(int N) { return (int)nblk(N, kPairBS); }
 
 void launch_pair_fp64(const DevParams &P, const double4 *spos, const int *sidx, const int *start,
                       const int4 *cols, int ncol, double *F, double *phi, double *epart,
-                      hipStream_t s) {
-#define CALL(NU)                                                                            \
-  if (P.tri)                                                                                \
-    k_pair_cell<false, NU, true><<<P.ncell, 64, 0, s>>>(P, spos, sidx, start, cols, ncol, F, phi, epart); \
-  else                                                                                      \
-    k_pair_cell<false, NU, false><<<P.ncell, 64, 0, s>>>(P, spos, sidx, start, cols, ncol, F, phi, epart)
+                      hipStream_t s, hipEvent_t done) {
+#define CALL(NU)                                                                             \
+  if (P.tri)                                                                                 \
+    launch_done(k_pair_cell<false, NU, true>, P.ncell, 64, 0, s, done, P, spos, sidx, start, \
+                cols, ncol, F, phi, epart);                                                  \
+  else                                                                                       \
+    launch_done(k_pair_cell<false, NU, false>, P.ncell, 64, 0, s, done, P, spos, sidx, start, \
+                cols, ncol, F, phi, epart)
   if (P.ewald) {
     CALL(0);
   } else {
@@ -2342,12 +2344,14 @@ void launch_pair_fp64(const DevParams &P, const double4 *spos, const int *sidx, 
 
 void launch_pair_fp32(const DevParams &P, const double4 *spos, const int *sidx, const int *start,
                       const int4 *cols, int ncol, double *F, double *phi, double *epart,
-                      hipStream_t s) {
+                      hipStream_t s, hipEvent_t done) {
 #define CALL(NU)                                                                            \
   if (P.tri)                                                                                \
-    k_pair_cell<true, NU, true><<<P.ncell, 64, 0, s>>>(P, spos, sidx, start, cols, ncol, F, phi, epart); \
+    launch_done(k_pair_cell<true, NU, true>, P.ncell, 64, 0, s, done, P, spos, sidx, start, \
+                cols, ncol, F, phi, epart);                                                 \
   else                                                                                      \
-    k_pair_cell<true, NU, false><<<P.ncell, 64, 0, s>>>(P, spos, sidx, start, cols, ncol, F, phi, epart)
+    launch_done(k_pair_cell<true, NU, false>, P.ncell, 64, 0, s, done, P, spos, sidx, start, \
+                cols, ncol, F, phi, epart)
   if (P.ewald) {
     CALL(0);
   } else {
@@ -2371,7 +2375,8 @@ bool spread_by_placements(const DevParams &P, GridDims g) {
 }
 
 void launch_spread(const DevParams &P, GridDims g, const double4 *spos, const int *start,
-                   double *q1, hipStream_t s, unsigned *q1b, unsigned *sbase, double4 *st) {
+                   double *q1, hipStream_t s, unsigned *q1b, unsigned *sbase, double4 *st,
+                   hipEvent_t done) {
   unsigned nb = (unsigned)(((g.x + kSX - 1) / kSX) * ((g.y + kSY - 1) / kSY) * ((g.z + kSZ - 1) / kSZ));
   if (sbase && st && spread_by_placements(P, g)) {
     if (P.tri)
@@ -2379,11 +2384,11 @@ void launch_spread(const DevParams &P, GridDims g, const double4 *spos, const in
     else
       k_spread_place<false><<<nblk(P.N, 256), 256, 0, s>>>(P, g, spos, sbase, st);
     if (nb >= kSpreadFine) {
-#define CALL(NU) k_spread_bricks<NU, 1><<<nb, 64, 0, s>>>(P, g, sbase, st, start, q1, q1b)
+#define CALL(NU) launch_done(k_spread_bricks<NU, 1>, nb, 64, 0, s, done, P, g, sbase, st, start, q1, q1b)
       MSM_NU_SWITCH(P.nu, CALL)
 #undef CALL
     } else {
-#define CALL(NU) k_spread_bricks<NU, 4><<<nb, 256, 0, s>>>(P, g, sbase, st, start, q1, q1b)
+#define CALL(NU) launch_done(k_spread_bricks<NU, 4>, nb, 256, 0, s, done, P, g, sbase, st, start, q1, q1b)
       MSM_NU_SWITCH(P.nu, CALL)
 #undef CALL
     }
@@ -2391,14 +2396,14 @@ void launch_spread(const DevParams &P, GridDims g, const double4 *spos, const in
   }
   if (nb >= kSpreadFine) {
 #define CALL(NU)                                                    \
-  if (P.tri) k_spread_mfma<NU, 1, true><<<nb, 64, 0, s>>>(P, g, spos, start, q1, q1b); \
-  else k_spread_mfma<NU, 1, false><<<nb, 64, 0, s>>>(P, g, spos, start, q1, q1b)
+  if (P.tri) launch_done(k_spread_mfma<NU, 1, true>, nb, 64, 0, s, done, P, g, spos, start, q1, q1b); \
+  else launch_done(k_spread_mfma<NU, 1, false>, nb, 64, 0, s, done, P, g, spos, start, q1, q1b)
     MSM_NU_SWITCH(P.nu, CALL)
 #undef CALL
   } else {
 #define CALL(NU)                                                      \
-  if (P.tri) k_spread_mfma<NU, 4, true><<<nb, 256, 0, s>>>(P, g, spos, start, q1, q1b); \
-  else k_spread_mfma<NU, 4, false><<<nb, 256, 0, s>>>(P, g, spos, start, q1, q1b)
+  if (P.tri) launch_done(k_spread_mfma<NU, 4, true>, nb, 256, 0, s, done, P, g, spos, start, q1, q1b); \
+  else launch_done(k_spread_mfma<NU, 4, false>, nb, 256, 0, s, done, P, g, spos, start, q1, q1b)
     MSM_NU_SWITCH(P.nu, CALL)
 #undef CALL
   }

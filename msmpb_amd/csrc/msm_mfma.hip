@@ -575,26 +575,27 @@ size_t g2g_lds_bytes(int prec, const G2GLevel &L) {
 }
 
 template <int PREC, int NCH>
-static void g2g_launch(const G2GArgs &args, size_t lds, hipStream_t s) {
-  k_g2g_tiles<PREC, NCH><<<(unsigned)args.ntasks, 64, lds, s>>>(args);
+static void g2g_launch(const G2GArgs &args, size_t lds, hipStream_t s, hipEvent_t done) {
+  launch_done(k_g2g_tiles<PREC, NCH>, (unsigned)args.ntasks, 64, lds, s, done, args);
 }
 
 int g2g_max_nch(int prec) { return prec == MSM_G2G_FP64 ? 16 : 4; }
 
-void launch_g2g_tiles(int prec, G2GArgs args, size_t lds, hipStream_t s) {
+void launch_g2g_tiles(int prec, G2GArgs args, size_t lds, hipStream_t s, hipEvent_t done) {
   if (args.ntasks <= 0) return;
   // all levels of one launch share the chunk count (the plan equalises it)
   int nch = args.lv[0].nch;
   if (prec == MSM_G2G_FP64) {
     switch (nch) {
-#define C(N) case N: g2g_launch<MSM_G2G_FP64, N>(args, lds, s); break;
+#define C(N) case N: g2g_launch<MSM_G2G_FP64, N>(args, lds, s, done); break;
       C(1) C(2) C(3) C(4) C(5) C(6) C(7) C(8) C(9) C(10) C(11) C(12) C(13) C(14) C(15) C(16)
 #undef C
     }
   } else {
     const int T = args.lv[0].tpw;
 #define R(N, TT) \
-  if (nch == N && T == TT) k_g2g_rows<N, TT><<<(unsigned)args.ntasks, 64 * kRowWaves, lds, s>>>(args);
+  if (nch == N && T == TT)                                                                      \
+    launch_done(k_g2g_rows<N, TT>, (unsigned)args.ntasks, 64 * kRowWaves, lds, s, done, args);
 #define RT(N) R(N, 1) R(N, 2) R(N, 3) R(N, 4) R(N, 5)
     RT(1) RT(2) RT(3) RT(4)
 #undef RT
