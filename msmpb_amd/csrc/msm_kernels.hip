@@ -1910,6 +1910,33 @@ __device__ __forceinline__ void csum_add(double &s, double &c, double s2, double
   c += c2 + err;
 }
 
+// Compensated partial sums (s, c) over the wavefront: a butterfly in a
+// fixed order, then lane 0's copy is used (deterministic run to run).
+__device__ __forceinline__ void wave_csum(double &s, double &c) {
+#pragma unroll
+  for (int m = 1; m < 64; m <<= 1) {
+    const double s2 = __shfl_xor(s, m, 64), c2 = __shfl_xor(c, m, 64);
+    csum_add(s, c, s2, c2);
+  }
+}
+
+// ... and over a workgroup of BS threads: the wavefronts' sums in order;
+// thread 0 receives the total (the others' values are unspecified).
+template <int BS>
+__device__ __forceinline__ void block_csum(double &s, double &c) {
+  constexpr int NW = BS / 64;
+  __shared__ double ws[NW], wc[NW];
+  wave_csum(s, c);
+  const int t = threadIdx.x;
+  if ((t & 63) == 0) ws[t >> 6] = s, wc[t >> 6] = c;
+  __syncthreads();
+  if (t == 0) {
+    s = ws[0], c = wc[0];
+    for (int w = 1; w < NW; ++w) csum_add(s, c, ws[w], wc[w]);
+  }
+  __syncthreads();  // ws, wc free for the next call
+}
+
 // Compensated sums of the per-cell pair partials and per-block grid partials:
 // each of 256 threads sums a fixed strided subset, then a fixed binary tree
 // combines the (sum, carry) pairs, so the result does not depend on
@@ -1920,24 +1947,13 @@ __device__ __forceinline__ void csum_add(double &s, double &c, double s2, double
 // grid partials and writes out.
 template <int BS>
 __device__ void finalize_tail(const DevParams &P, const FinalizeArgs &f, double sv, double cv) {
-  __shared__ double ss[2][BS], cc[2][BS];
   int t = threadIdx.x;
-  ss[0][t] = sv, cc[0][t] = cv;
   double s = 0, c = 0;
   for (int k = t; k < f.ng; k += BS) neumaier(s, c, f.eg[k]);
-  ss[1][t] = s, cc[1][t] = c;
-  __syncthreads();
-  for (int h = BS / 2; h > 0; h >>= 1) {
-    if (t < h)
-      for (int w = 0; w < 2; ++w) {
-        double a = ss[w][t], b = cc[w][t];
-        csum_add(a, b, ss[w][t + h], cc[w][t + h]);
-        ss[w][t] = a, cc[w][t] = b;
-      }
-    __syncthreads();
-  }
+  block_csum<BS>(sv, cv);
+  block_csum<BS>(s, c);
   if (t != 0) return;
-  double epair = ss[0][0] + cc[0][0], egrid = 0.5 * (ss[1][0] + cc[1][0]);
+  double epair = sv + cv, egrid = 0.5 * (s + c);
   double qs = f.scal[0], q2 = f.scal[1];
   // constant part (forcefield1.c:114-130)
   double econst = 0.5 * q2 * P.coeff1 - 0.5 * (qs * qs) * P.coeff2 - 0.5 * q2 * P.gamma0 * P.inv_a0;
@@ -2237,7 +2253,6 @@ __global__ __launch_bounds__(256) void k_combine(DevParams P, const double *__re
                                                  double *__restrict__ phi, double *__restrict__ ep2,
                                                  FinalizeArgs fin) {
   chain_issue_priority();
-  __shared__ double ss[256], cc[256];
   __shared__ int last;
   const int t = threadIdx.x, b = blockIdx.x;
   const long i = (long)b * 256 + t;
@@ -2250,19 +2265,10 @@ __global__ __launch_bounds__(256) void k_combine(DevParams P, const double *__re
   const int k0 = b * chunk, k1 = min(fin.np, k0 + chunk);
   double sv = 0.0, cv = 0.0;
   for (int k = k0 + t; k < k1; k += 256) neumaier(sv, cv, fin.ep[k]);
-  ss[t] = sv, cc[t] = cv;
-  __syncthreads();
-  for (int h = 128; h > 0; h >>= 1) {
-    if (t < h) {
-      double x = ss[t], y = cc[t];
-      csum_add(x, y, ss[t + h], cc[t + h]);
-      ss[t] = x, cc[t] = y;
-    }
-    __syncthreads();
-  }
+  block_csum<256>(sv, cv);
   if (t == 0) {
-    ep2[2 * b] = ss[0];
-    ep2[2 * b + 1] = cc[0];
+    ep2[2 * b] = sv;
+    ep2[2 * b + 1] = cv;
     __threadfence();
     last = atomicAdd(fin.done, 1u) == gridDim.x - 1;
   }
