@@ -74,9 +74,9 @@ def algorithmic(stage, m, P, nu):
     if stage == "grid2grid":  # 2 * sum_l prod(M_l) prod(s_l), l < L
         f = sum(2 * np.prod(m.grid_dims(l)) * np.prod(m.stencil_dims(l))
                 for l in range(1, m.levels))
-        return f, "flop", "k_g2g_tiles<" + ("1" if mixed == "true" else "0")
+        return f, "flop", "k_g2g_rows<" if mixed == "true" else "k_g2g_tiles<0"
     if stage == "anterpolation":  # N (2 nu^3 + 6 nu^2) FLOP
-        return N * (2 * nu ** 3 + 6 * nu ** 2), "flop", f"k_spread_mfma<{nu},"
+        return N * (2 * nu ** 3 + 6 * nu ** 2), "flop", f"k_spread_bricks<{nu},"
     if stage == "interpolation":  # N (8 nu^3 + 6 nu^2) FLOP
         return N * (8 * nu ** 3 + 6 * nu ** 2), "flop", f"k_gather_tile<{nu},"
     return None
@@ -258,9 +258,13 @@ def rms_rel(Fa, Fb):
 
 
 def roofline(stages, m, pairs, nu):
-    """Dominant kernel of the evaluation: achieved rate from its HIP-event
-    duration against the gfx950 peak, with its calibrated HBM traffic."""
-    dom = max(stages, key=stages.get)
+    """Dominant kernel of the evaluation -- the pair kernel: the largest GPU
+    time per evaluation (profiles/water96k_kernel_stats.md) and the only
+    kernel on its stream, so that the HIP events around it time it alone
+    (the chain's stage events on the caller's stream also span the waits
+    for the other streams) -- its achieved rate from that duration against
+    the gfx950 peak, with its calibrated HBM traffic."""
+    dom = "partcl2partcl"
     alg = algorithmic(dom, m, pairs, nu)
     if alg is None or stages[dom] <= 0:
         return None

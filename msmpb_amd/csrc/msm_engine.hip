@@ -651,7 +651,10 @@ void stage_prepare(Engine &E, const double *pos, const double *q, hipStream_t s,
   msm::launch_scan(E.P, E.d_cnt, E.d_start, E.d_qpart, msm::bin_blocks(E.P.N), E.d_scal, s);
   msm::launch_scatter(E.P, E.d_cell_of, E.d_rank, E.d_wpos, E.d_start, E.d_spos, E.d_sidx,
                       E.d_scell, s);
-  msm::launch_cell_sort(E.P, E.d_start, E.d_cnt, E.d_spos, E.d_sidx, s, fork);
+  // the cell sort also writes the atoms' spreading placements
+  const bool place = msm::spread_by_placements(E.P, E.gd[1]);
+  msm::launch_cell_sort(E.P, E.d_start, E.d_cnt, E.d_spos, E.d_sidx, s, fork, E.gd[1],
+                        place ? E.d_sbase : nullptr, place ? E.d_st : nullptr);
 }
 
 void stage_pairs(Engine &E, int precision, double *F, double *phi, hipStream_t s,
@@ -694,7 +697,7 @@ int stage_grids(Engine &E, int precision, int layout, Timing &T, hipStream_t s) 
   // ev_q1 (q^1 ready) rides on the spreading kernel's dispatch
   msm::launch_spread(E.P, E.gd[1], E.d_spos, E.d_start, E.d_ql[1], s,
                      split && L >= 2 ? E.d_qb[1] : nullptr, E.d_sbase, E.d_st,
-                     L >= 2 ? E.ev_q1 : nullptr);
+                     L >= 2 ? E.ev_q1 : nullptr, /*placed=*/true);
   T.close(t, "anterpolation", s);
   if (L >= 2) {
     HIP_TRY(hipStreamWaitEvent(E.stream3, E.ev_q1, 0));

@@ -100,9 +100,12 @@ void launch_scan(const DevParams &P, const int *cnt, int *start, const double *q
 void launch_scatter(const DevParams &P, const int *cell_of, const int *rank, const double4 *wpos,
                     const int *start, double4 *spos, int *sidx, int *scell, hipStream_t s);
 // Orders atoms inside each cell by original index (deterministic results)
-// and clears cnt for the next binning.
+// and clears cnt for the next binning.  With sbase, st (N entries each) it
+// also writes every atom's spreading placement on the grid g1 (as the
+// placement pass of launch_spread does).
 void launch_cell_sort(const DevParams &P, const int *start, int *cnt, double4 *spos, int *sidx,
-                      hipStream_t s, hipEvent_t done = nullptr);
+                      hipStream_t s, hipEvent_t done = nullptr, GridDims g1 = GridDims{0, 0, 0},
+                      unsigned *sbase = nullptr, double4 *st = nullptr);
 
 // Short-range pairs, full neighbour shell, one wavefront per home cell;
 // cols: (dx, dy, dzlo, dzhi) neighbour cell columns, dzhi - dzlo < nc[2].
@@ -125,11 +128,12 @@ void launch_pair_count(const DevParams &P, const double4 *spos, const int *start
 // B-spline spreading of charges onto the level-1 grid, one workgroup per
 // grid brick (every grid point written exactly once).
 // q1b (optional): q^1 also as bf16_split pairs.  sbase, st (optional, N
-// entries each): per-atom placements, computed first and used where
+// entries each): per-atom placements, computed first (placed: already
+// written by the cell sort of the same evaluation) and used where
 // spread_by_placements(P, g) holds.
 void launch_spread(const DevParams &P, GridDims g, const double4 *spos, const int *start,
                    double *q1, hipStream_t s, unsigned *q1b = nullptr, unsigned *sbase = nullptr,
-                   double4 *st = nullptr, hipEvent_t done = nullptr);
+                   double4 *st = nullptr, hipEvent_t done = nullptr, bool placed = false);
 bool spread_by_placements(const DevParams &P, GridDims g);
 // q^l = R q^{l-1} (full weighting with the two-scale stencil).
 // qcb (optional): q^l also as bf16_split pairs.

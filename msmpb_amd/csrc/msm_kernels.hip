@@ -296,11 +296,42 @@ __global__ __launch_bounds__(256) void k_scatter(int N, const int *__restrict__ 
 // W lanes per cell (W = 16 when cells hold a few atoms, else 64): ranks the
 // cell's atoms by original index (so the order, and every sum built on it, is
 // deterministic) and moves them in place.
-template <int W>
+// Placement of a sorted atom on the level-1 grid for the spreading
+// (anterpolate, forcefield1.c:350-368): fractional offsets t in [0, 1) and
+// the index of the lowest support point m - NU/2 + 1 per axis, reduced
+// modulo the grid and packed 10 bits per axis, with the charge next to t.
+template <bool TRI>
+__device__ __forceinline__ void place_atom(const DevParams &P, GridDims g, const double4 &at,
+                                           unsigned &packed, double4 &tq) {
+  const double r[3] = {at.x, at.y, at.z};
+  double sf[3], t[3];
+  to_frac<TRI>(P, r, sf);
+  const int dims[3] = {g.x, g.y, g.z};
+  packed = 0;
+#pragma unroll
+  for (int ax = 0; ax < 3; ++ax) {
+    double sv = sf[ax];
+    sv = sv - floor(sv);
+    const double tt = dims[ax] * sv, m = floor(tt);
+    t[ax] = tt - m;
+    int v = ((int)m - P.nu / 2 + 1) % dims[ax];
+    v += v < 0 ? dims[ax] : 0;
+    packed |= (unsigned)v << (10 * ax);
+  }
+  tq = make_double4(t[0], t[1], t[2], at.w);
+}
+
+// With sbase != null it also writes each atom's spreading placement
+// (place_atom): the spreading kernel then follows the binning on the same
+// stream directly, and is dispatched before the pair kernel on its own
+// stream can take the CUs.
+template <int W, bool TRI>
 __global__ __launch_bounds__(256) void k_cell_sort(DevParams P, const int *__restrict__ start,
                                                    int *__restrict__ cnt,
                                                    double4 *__restrict__ spos,
-                                                   int *__restrict__ sidx) {
+                                                   int *__restrict__ sidx, GridDims g1,
+                                                   unsigned *__restrict__ sbase,
+                                                   double4 *__restrict__ st) {
   const int c = blockIdx.x * (256 / W) + threadIdx.x / W;
   const int lane = threadIdx.x % W;
   if (c >= P.ncell) return;
@@ -314,6 +345,13 @@ __global__ __launch_bounds__(256) void k_cell_sort(DevParams P, const int *__res
     if (lane < n) {
       sidx[b + rank] = key;
       spos[b + rank] = v;
+      if (sbase) {
+        unsigned pk;
+        double4 tq;
+        place_atom<TRI>(P, g1, v, pk, tq);
+        sbase[b + rank] = pk;
+        st[b + rank] = tq;
+      }
     }
   } else if (lane == 0) {  // crowded cell: insertion sort by one lane
     for (int k = b + 1; k < b + n; ++k) {
@@ -328,6 +366,14 @@ __global__ __launch_bounds__(256) void k_cell_sort(DevParams P, const int *__res
       sidx[m + 1] = key;
       spos[m + 1] = v;
     }
+    if (sbase)
+      for (int k = b; k < b + n; ++k) {
+        unsigned pk;
+        double4 tq;
+        place_atom<TRI>(P, g1, spos[k], pk, tq);
+        sbase[k] = pk;
+        st[k] = tq;
+      }
   }
 }
 
@@ -1226,24 +1272,7 @@ __global__ __launch_bounds__(256) void k_spread_place(DevParams P, GridDims g,
   chain_issue_priority();
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= P.N) return;
-  const double4 at = spos[i];
-  const double r[3] = {at.x, at.y, at.z};
-  double sf[3], t[3];
-  to_frac<TRI>(P, r, sf);
-  const int dims[3] = {g.x, g.y, g.z};
-  unsigned packed = 0;
-#pragma unroll
-  for (int ax = 0; ax < 3; ++ax) {
-    double sv = sf[ax];
-    sv = sv - floor(sv);
-    const double tt = dims[ax] * sv, m = floor(tt);
-    t[ax] = tt - m;
-    int v = ((int)m - P.nu / 2 + 1) % dims[ax];
-    v += v < 0 ? dims[ax] : 0;
-    packed |= (unsigned)v << (10 * ax);
-  }
-  sbase[i] = packed;
-  st[i] = make_double4(t[0], t[1], t[2], at.w);
+  place_atom<TRI>(P, g, spos[i], sbase[i], st[i]);
 }
 
 template <int NU, int WPB>
@@ -2312,11 +2341,16 @@ void launch_scatter(const DevParams &P, const int *cell_of, const int *rank, con
 }
 
 void launch_cell_sort(const DevParams &P, const int *start, int *cnt, double4 *spos, int *sidx,
-                      hipStream_t s, hipEvent_t done) {
-  if (P.N <= 6L * P.ncell)  // a few atoms per cell: 16 lanes each
-    launch_done(k_cell_sort<16>, nblk(P.ncell, 16), 256, 0, s, done, P, start, cnt, spos, sidx);
-  else
-    launch_done(k_cell_sort<64>, nblk(P.ncell, 4), 256, 0, s, done, P, start, cnt, spos, sidx);
+                      hipStream_t s, hipEvent_t done, GridDims g1, unsigned *sbase, double4 *st) {
+#define A P, start, cnt, spos, sidx, g1, sbase, st
+  if (P.N <= 6L * P.ncell) {  // a few atoms per cell: 16 lanes each
+    if (P.tri) launch_done(k_cell_sort<16, true>, nblk(P.ncell, 16), 256, 0, s, done, A);
+    else launch_done(k_cell_sort<16, false>, nblk(P.ncell, 16), 256, 0, s, done, A);
+  } else {
+    if (P.tri) launch_done(k_cell_sort<64, true>, nblk(P.ncell, 4), 256, 0, s, done, A);
+    else launch_done(k_cell_sort<64, false>, nblk(P.ncell, 4), 256, 0, s, done, A);
+  }
+#undef A
 }
 
 int pair_blocks(int N) { return (int)nblk(N, kPairBS); }
@@ -2382,13 +2416,15 @@ bool spread_by_placements(const DevParams &P, GridDims g) {
 
 void launch_spread(const DevParams &P, GridDims g, const double4 *spos, const int *start,
                    double *q1, hipStream_t s, unsigned *q1b, unsigned *sbase, double4 *st,
-                   hipEvent_t done) {
+                   hipEvent_t done, bool placed) {
   unsigned nb = (unsigned)(((g.x + kSX - 1) / kSX) * ((g.y + kSY - 1) / kSY) * ((g.z + kSZ - 1) / kSZ));
   if (sbase && st && spread_by_placements(P, g)) {
-    if (P.tri)
-      k_spread_place<true><<<nblk(P.N, 256), 256, 0, s>>>(P, g, spos, sbase, st);
-    else
-      k_spread_place<false><<<nblk(P.N, 256), 256, 0, s>>>(P, g, spos, sbase, st);
+    if (!placed) {
+      if (P.tri)
+        k_spread_place<true><<<nblk(P.N, 256), 256, 0, s>>>(P, g, spos, sbase, st);
+      else
+        k_spread_place<false><<<nblk(P.N, 256), 256, 0, s>>>(P, g, spos, sbase, st);
+    }
     if (nb >= kSpreadFine) {
 #define CALL(NU) launch_done(k_spread_bricks<NU, 1>, nb, 64, 0, s, done, P, g, sbase, st, start, q1, q1b)
       MSM_NU_SWITCH(P.nu, CALL)
