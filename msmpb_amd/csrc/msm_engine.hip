@@ -667,6 +667,15 @@ void stage_pairs(Engine &E, int precision, double *F, double *phi, hipStream_t s
                           E.d_epair, s, done);
 }
 
+// LDS of one stencil launch: its own levels' slabs only (a workgroup that
+// asks for more than the pair kernel leaves free on a CU waits until pair
+// workgroups retire there).
+size_t g2g_launch_lds(int pr, const msm::G2GArgs &a) {
+  size_t m = 0;
+  for (int k = 0; k < a.nlev; ++k) m = std::max(m, msm::g2g_lds_bytes(pr, a.lv[k]));
+  return m;
+}
+
 // The levels [first, last) of the level-stencil launch description, as a
 // launch of their own (task numbering rebased to the first of them).
 msm::G2GArgs g2g_levels(const msm::G2GArgs &A, int first, int last) {
@@ -706,7 +715,7 @@ int stage_grids(Engine &E, int precision, int layout, Timing &T, hipStream_t s) 
     // then leaves issue priority to the pair kernel
     msm::G2GArgs a1 = g2g_levels(E.g2g[pr], 0, 1);
     a1.urgent = L <= 3;
-    msm::launch_g2g_tiles(pr, a1, E.g2g_lds[pr], E.stream3, E.ev_g1);
+    msm::launch_g2g_tiles(pr, a1, g2g_launch_lds(pr, a1), E.stream3, E.ev_g1);
     T.close(t, "grid2grid", E.stream3);
   }
   for (int l = 2; l <= L; ++l) {
@@ -719,7 +728,7 @@ int stage_grids(Engine &E, int precision, int layout, Timing &T, hipStream_t s) 
     t = T.open(s);
     msm::G2GArgs ac = g2g_levels(E.g2g[pr], 1, L - 1);
     ac.urgent = 1;
-    msm::launch_g2g_tiles(pr, ac, E.g2g_lds[pr], s);
+    msm::launch_g2g_tiles(pr, ac, g2g_launch_lds(pr, ac), s);
     T.close(t, "grid2grid", s);
   }
   t = T.open(s);
