@@ -1466,9 +1466,12 @@ __global__ __launch_bounds__(64 * WPB) void k_spread_bricks(DevParams P, GridDim
 // restrict_ (forcefield1.c:401-477): q^l_m = sum_n J_nx J_ny J_nz
 // q^{l-1}_{2m+n}, n in [-NU/2, NU/2]^3, periodic, as separable passes like
 // the reference's z -> y -> x (:442-472).  A workgroup owns a 2^3 coarse
-// brick: it loads the (NU+3)^3 fine values the brick reads into LDS, then
-// contracts z, y and x in turn.  The small brick keeps the LDS footprint
-// (7.4 KB at NU = 6) low enough to run beside the pair kernel.
+// brick: the z pass reads the fine values it needs straight from global
+// memory (each thread's NU + 1 loads per value in flight together), then
+// contracts y and x through LDS.  1.6 KB of LDS at NU = 6, so that several
+// of these one-wavefront workgroups fit on a CU beside the pair kernel (a
+// staged (NU+3)^3 tile, 7.4 KB, let one run per CU there: the 1728 bricks
+// of the 24^3 level took seven rounds).
 constexpr int kRB = 2;
 constexpr int kRBS = 64;  // one wavefront per brick: the work is small, and few
                           // wavefronts find slots quickly beside the pair kernel
@@ -1481,7 +1484,8 @@ __global__ __launch_bounds__(kRBS) void k_restrict(DevParams P, GridDims f, Grid
   chain_issue_priority();
   // coarse brick of kRB^3 points; fine indices 2m + n - H, n in [0, NU]
   constexpr int H = NU / 2, FW = 2 * (kRB - 1) + NU + 1;
-  __shared__ double fs[FW * FW * FW];
+  constexpr int NZ = FW * FW * kRB;        // z-pass outputs
+  constexpr int PER = (NZ + kRBS - 1) / kRBS;
   __shared__ double t1[FW * FW * kRB];    // [fx][fy][cz]
   __shared__ double t2[FW * kRB * kRB];   // [fx][cy][cz]
   const int tid = threadIdx.x;
@@ -1491,18 +1495,23 @@ __global__ __launch_bounds__(kRBS) void k_restrict(DevParams P, GridDims f, Grid
   b /= nbz;
   const int cy0 = (b % nby) * kRB, cx0 = (b / nby) * kRB;
   const int fx0 = 2 * cx0 - H, fy0 = 2 * cy0 - H, fz0 = 2 * cz0 - H;
-  for (int k = tid; k < FW * FW * FW; k += kRBS) {
-    const int a = k / (FW * FW), bb = (k / FW) % FW, cc = k % FW;
-    fs[k] = qf[((long)wrapi(fx0 + a, f.x) * f.y + wrapi(fy0 + bb, f.y)) * f.z + wrapi(fz0 + cc, f.z)];
-  }
-  __syncthreads();
-  for (int k = tid; k < FW * FW * kRB; k += kRBS) {  // along z
-    const int cz = k % kRB, fxy = k / kRB;
-    const double *row = fs + fxy * FW + 2 * cz;
-    double acc = 0.0;
+  {  // along z, from global memory: all of this thread's loads first
+    double v[PER][NU + 1];
 #pragma unroll
-    for (int n = 0; n <= NU; ++n) acc += P.J[n] * row[n];
-    t1[k] = acc;
+    for (int u = 0; u < PER; ++u) {
+      const int k = min(tid + u * kRBS, NZ - 1);  // past the end: a valid address, not stored
+      const int cz = k % kRB, fxy = k / kRB, a = fxy / FW, bb = fxy % FW;
+      const double *col = qf + ((long)wrapi(fx0 + a, f.x) * f.y + wrapi(fy0 + bb, f.y)) * f.z;
+#pragma unroll
+      for (int n = 0; n <= NU; ++n) v[u][n] = col[wrapi(fz0 + 2 * cz + n, f.z)];
+    }
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      double acc = 0.0;
+#pragma unroll
+      for (int n = 0; n <= NU; ++n) acc += P.J[n] * v[u][n];
+      if (tid + u * kRBS < NZ) t1[tid + u * kRBS] = acc;
+    }
   }
   __syncthreads();
   for (int k = tid; k < FW * kRB * kRB; k += kRBS) {  // along y
