@@ -1405,6 +1405,10 @@ int FF_set_layout_compat(FF *ff, int layout) {
   if (layout == MSM_LAYOUT_ASIS && X(ff)->method == MSM_METHOD_PME)
     return fail(MSM_ERR_ARG, "the as-is index layout does not apply to the PME method");
   FFX *x = X(ff);
+  if (layout == MSM_LAYOUT_ASIS && x->built && x->eng && !msm::reference_images_hold(x->eng->plan))
+    return fail(MSM_ERR_ARG,
+                "FF_set_layout_compat: the as-is layout keeps the reference's image rules, which "
+                "need the cutoff below half the smallest face-to-face width of a triclinic box");
   x->layout = layout;
   if (x->built && x->eng && x->eng->plan.method == msm::kMsm) {
     // the layout also selects the level kernels' image range: regenerate

@@ -264,14 +264,18 @@ void build_level_kernel(Plan &p, int l, const double as[3]) {
 }
 
 bool box_dependent(Plan &p, std::string &err, bool stencils = true) {
-  // the reference's image-range vector: column norms of A^-1
-  double as[3] = {col_norm(p.Ai, 0), col_norm(p.Ai, 1), col_norm(p.Ai, 2)};
+  // image-range vector: the reference's column norms of A^-1, or with the
+  // complete image range max(row, column) norm (Plan::kas)
   kappa_image_bound(p);
   // constant-energy coefficients (forcefield0.c:612-643): own images of
-  // each charge inside the cutoff, and the neutralising background
+  // each charge inside the cutoff, and the neutralising background.  A
+  // lattice vector A k inside a0 has |k_a| < a0 |row a of A^-1|; the column
+  // norms the reference bounds k with can miss some where the cutoff
+  // reaches a face-to-face width of a triclinic box (none lie inside a0
+  // below that width, so both ranges give the same sum there)
   double a0 = p.aCut[0];
   int lim[3];
-  for (int a = 0; a < 3; ++a) lim[a] = (int)std::ceil(a0 * as[a] - 1.0);
+  for (int a = 0; a < 3; ++a) lim[a] = (int)std::ceil(a0 * p.kas[a] - 1.0);
   p.coeff1 = 0;
   for (int x = -lim[0]; x <= lim[0]; ++x)
     for (int y = -lim[1]; y <= lim[1]; ++y)
@@ -296,24 +300,40 @@ bool box_dependent(Plan &p, std::string &err, bool stencils = true) {
   return true;
 }
 
-// Box shape: orthorhombic (diagonal A) or triclinic.  A triclinic box needs
-// the cutoff below half its smallest face-to-face width 1 / |row a of A^-1|:
-// there the reference's pair rule (the image nearest in fractional
-// coordinates, neighborlist and partcl2partcl of forcefield0.c and
-// forcefield1.c) and the sum over every image inside the cutoff coincide.
+// Box shape: orthorhombic (diagonal A) or triclinic.  Below half the
+// smallest face-to-face width w = 1 / |row a of A^-1| the reference's pair
+// rule (the image nearest in fractional coordinates, then the images
+// p in [-plim, plim]^3 around it with plim from the column norms of A^-1;
+// neighborlist and partcl2partcl of forcefield0.c and forcefield1.c) and
+// the sum over every image inside the cutoff coincide: an image within
+// a0 < w/2 has every |s_a| < 1/2.  At larger cutoffs in a triclinic box the
+// two part: a pair whose fractionally nearest image lies beyond the cutoff
+// is not listed although another image may lie inside it, and column norms
+// below the row norms leave images outside the plim box.  The MSM method
+// with the complete image range (complete_images, the default layout) sums
+// every image inside the cutoff at any cutoff -- the pair kernel's cell
+// columns, lattice shifts and self-image constant hold every image for any
+// box -- so only the reference's own rules (the as-is layout) and the PME
+// comparator keep the bound.
+static double min_width(const Plan &p) {
+  double wmin = INFINITY;
+  for (int a = 0; a < 3; ++a) wmin = std::fmin(wmin, 1.0 / row_norm(p.Ai, a));
+  return wmin;
+}
+
 bool check_box(Plan &p, std::string &err) {
   p.ortho = true;
   for (int i = 0; i < 3; ++i)
     for (int j = 0; j < 3; ++j)
       if (i != j && p.A[i][j] != 0.0) p.ortho = false;
-  if (p.ortho) return true;
-  double wmin = INFINITY;
-  for (int a = 0; a < 3; ++a) wmin = std::fmin(wmin, 1.0 / row_norm(p.Ai, a));
+  if (p.ortho || (p.method == kMsm && p.complete_images)) return true;
+  const double wmin = min_width(p);
   if (!(p.aCut[0] < 0.5 * wmin)) {
     char buf[256];
     std::snprintf(buf, sizeof buf,
                   "triclinic box: the cutoff %g must be below half the smallest face-to-face "
-                  "width (%g)", p.aCut[0], wmin);
+                  "width (%g) for %s", p.aCut[0], wmin,
+                  p.method == kMsm ? "the reference's image rules (as-is layout)" : "the PME method");
     err = buf;
     return false;
   }
@@ -431,6 +451,8 @@ bool build_pme(const BuildRequest &r, Plan &p, std::string &err) {
 }
 
 }  // namespace
+
+bool reference_images_hold(const Plan &p) { return p.ortho || p.aCut[0] < 0.5 * min_width(p); }
 
 double invert3(const double a[3][3], double ai[3][3]) {
   double adj[3][3];

@@ -80,6 +80,11 @@ bool rebuild_plan(Plan &plan, const double edges[3][3], std::string &err, bool s
 // Sets plan.kas from the box and plan.complete_images.
 void kappa_image_bound(Plan &plan);
 
+// True where the reference's own image rules (the as-is layout) hold every
+// image inside the cutoff: an orthorhombic box, or a cutoff below half the
+// smallest face-to-face width.
+bool reference_images_hold(const Plan &plan);
+
 // Series coefficients b_0..b_6 of t^2 psi(t) of the top-level kernel.
 void top_series(int nu, double b[7]);
 

@@ -21,6 +21,14 @@
  * images (a sheared CsCl cell at L = 2 loses 1e-3 of its energy).
  * compat = 0 sums over max(row, column) norm: identical terms wherever the
  * HEAD's range already holds every image, the complete kernel otherwise.
+ * The same holds for the pair term and the own-image constant coeff1 once
+ * the cutoff reaches half a face-to-face width of a triclinic box: the
+ * reference lists a pair by its fractionally nearest image and sums the
+ * images within plim (column norms) of it (forcefield0.c:109-160,
+ * forcefield1.c:240-321, forcefield0.c:612-643); compat = 0 lists every
+ * pair with any image inside the cutoff and bounds the images by
+ * max(row, column) norm, compat = 1 keeps the reference's rules.  Below half
+ * the smallest width, and in orthorhombic boxes, the two coincide.
  */
 #include "msm_oracle.h"
 
@@ -91,6 +99,14 @@ static double rownorm(const double m[3][3], int r) {
 }
 static double colnorm(const double m[3][3], int c) {
   return sqrt(m[0][c] * m[0][c] + m[1][c] * m[1][c] + m[2][c] * m[2][c]);
+}
+
+/* Image-range vector of the pair term and coeff1: the reference's column
+ * norms of A^-1 (compat), or max(row, column) norm, which bounds every
+ * lattice shift inside a radius r by |p_a| <= r |row a of A^-1| + 1/2. */
+static double image_norm(const struct orc *o, int a) {
+  double c = colnorm(o->Ai, a);
+  return o->compat ? c : fmax(c, rownorm(o->Ai, a));
 }
 
 /* _FF_get_errEst (forcefield0.c:553-582), with C = 1. */
@@ -447,7 +463,7 @@ orc_t *orc_build(int N, const double *q, const double *pos, const double *edges,
   /* FF_rebuild: constant-energy coefficients (forcefield0.c:612-643) */
   double as[3] = {colnorm(o->Ai, 0), colnorm(o->Ai, 1), colnorm(o->Ai, 2)};
   double lim[3];
-  for (int a = 0; a < 3; a++) lim[a] = ceil(a0 * as[a] - 1.);
+  for (int a = 0; a < 3; a++) lim[a] = ceil(a0 * image_norm(o, a) - 1.);
   o->coeff1 = 0.;
   for (double x = -lim[0]; x <= lim[0]; x++)
     for (double y = -lim[1]; y <= lim[1]; y++)
@@ -489,10 +505,16 @@ void orc_free(orc_t *o) {
 /* ---- neighbour list (forcefield0.c:21-164): all pairs i<j whose nearest
  * image lies within a0 + margin, stored CSR per i.  Pairs are found with a
  * cell grid of width >= a0 + margin (own enumeration; the reference hashes
- * into ceil(N^(1/3))^3 cells), which yields the same pair set. */
+ * into ceil(N^(1/3))^3 cells), which yields the same pair set.  pair_count
+ * is that set's size; with compat = 0 the list also takes the pairs whose
+ * nearest image lies outside but another image inside (a triclinic box
+ * narrower than twice the radius; along such an axis the grid has one
+ * cell, so every such pair is a candidate). */
 int orc_build_pairs(orc_t *o, const double *pos, double margin) {
   int N = o->N;
   double ac = o->aCut[0] + margin, ac2 = ac * ac;
+  double plim[3]; /* images around the nearest one that can lie inside ac */
+  for (int a = 0; a < 3; a++) plim[a] = ceil(ac * image_norm(o, a) - 0.5);
   int nc[3];
   for (int a = 0; a < 3; a++) {
     double w = 1. / rownorm(o->Ai, a);
@@ -552,8 +574,24 @@ int orc_build_pairs(orc_t *o, const double *pos, double margin) {
               double v = r[a] - (o->A[a][0] * p[0] + o->A[a][1] * p[1] + o->A[a][2] * p[2]);
               d2 += v * v;
             }
-            if (d2 < ac2) {
-              pairs++;
+            int listed = d2 < ac2;
+            pairs += listed;
+            if (!o->compat && !listed) { /* another image inside ac */
+              double v0[3];
+              for (int a = 0; a < 3; a++)
+                v0[a] = r[a] - (o->A[a][0] * p[0] + o->A[a][1] * p[1] + o->A[a][2] * p[2]);
+              for (double px = -plim[0]; px <= plim[0] && !listed; px++)
+                for (double py = -plim[1]; py <= plim[1] && !listed; py++)
+                  for (double pz = -plim[2]; pz <= plim[2] && !listed; pz++) {
+                    double e2 = 0.;
+                    for (int a = 0; a < 3; a++) {
+                      double v = v0[a] + (o->A[a][0] * px + o->A[a][1] * py + o->A[a][2] * pz);
+                      e2 += v * v;
+                    }
+                    listed = e2 < ac2;
+                  }
+            }
+            if (listed) {
               if (len == cap) { cap *= 2; o->plist = (int *)realloc(o->plist, sizeof(int) * cap); }
               o->plist[len++] = j;
             }
@@ -572,9 +610,8 @@ static double short_range(orc_t *o, const double *pos, const double *q, double *
                           double *phi) {
   int N = o->N, nu = o->nu;
   double a0 = o->aCut[0], a02 = a0 * a0;
-  double as[3] = {colnorm(o->Ai, 0), colnorm(o->Ai, 1), colnorm(o->Ai, 2)};
   double plim[3];
-  for (int a = 0; a < 3; a++) plim[a] = ceil(a0 * as[a] - 0.5);
+  for (int a = 0; a < 3; a++) plim[a] = ceil(a0 * image_norm(o, a) - 0.5);
   double a03 = pow(a0, 3);
   for (int i = 0; i < N; i++) { F[3 * i] = F[3 * i + 1] = F[3 * i + 2] = 0.; if (phi) phi[i] = 0.; }
   double energy = 0.;

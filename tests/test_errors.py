@@ -41,13 +41,14 @@ def test_build_rejects_parameters_with_reason(kw, msg, water):
 
 
 def test_build_rejects_skewed_box_beyond_its_domain(water):
-    """Triclinic boxes need the cutoff below half the smallest face-to-face
-    width (14.4 A box, cutoff 8 A)."""
+    """The reference's own image rules (as-is layout) need a triclinic box's
+    cutoff below half its smallest face-to-face width (14.4 A box, cutoff
+    8 A); the default layout sums every image and takes it."""
     s = water
     e = s.edges.copy()
     e[0, 1] = 1.0
     with pytest.raises(MSMError, match="triclinic box: the cutoff 8 must be below half"):
-        MSM(s.pos, s.q, e, nu=4, levels=2, top_grid=4, cutoff=8.0)
+        MSM(s.pos, s.q, e, nu=4, levels=2, top_grid=4, cutoff=8.0, layout="asis")
 
 
 @pytest.mark.parametrize("kw", [dict(precision="fp32"), dict(method="ewald"), dict(layout="padded")])

@@ -2,22 +2,28 @@
 
 The reference accepts any box matrix (neighborlist's skew handling,
 forcefield0.c:61-107; fractional coordinates throughout forcefield1.c).
-This build supports triclinic boxes whose cutoff is below half the smallest
-face-to-face width, where the reference's pair rule (the image nearest in
-fractional coordinates) and the sum over all images inside the cutoff agree;
-outside it FF_build refuses the box.
+Below half the smallest face-to-face width its pair rule (the image nearest
+in fractional coordinates, then the images within plim of it) is the sum
+over every image inside the cutoff.  Beyond it the reference's rules drop
+images in a sheared cell (a pair is listed by its fractionally nearest
+image only, and plim and the own-image range come from the column norms
+of A^-1); this build's default layout sums every image inside the cutoff
+at any cutoff, and the as-is layout (the reference's rules) and the PME
+comparator refuse such boxes.
 
 Pins of the oracles (CPU): the PME and MSM oracles reproduce the
 closed-form Madelung energy of a CsCl crystal written in sheared cells (the
 same lattice, a' = a, b' = a + b, c' = b + c), MSM within its a-priori
-error estimate at L = 1, 2 and 3; the MSM oracle's forces and potentials
-are central differences of its energy in a skewed box.  The reference
-HEAD's image range of the level kernels (column norms of A^-1,
-forcefield0.c:618-620) drops images in the sheared cell: the as-is oracle
-misses the Madelung energy by far more than the estimate at L >= 2.
+error estimate at L = 1, 2 and 3 and at cutoffs up to 1.1 times the
+smallest width; the MSM oracle's forces and potentials are central
+differences of its energy in skewed boxes, below and beyond half the
+width.  The reference HEAD's image range of the level kernels (column norms
+of A^-1, forcefield0.c:618-620) drops images in the sheared cell: the as-is
+oracle misses the Madelung energy by far more than the estimate at L >= 2.
 The GPU (marked gpu): MSM and PME energies, forces and potentials against
-the oracles in skewed boxes (FP64 to 1e-10, mixed to 1e-5), the Madelung
-energy on the GPU, pair counts, and FF_rebuild into a changed skewed box.
+the oracles in skewed boxes (FP64 to 1e-10, mixed to 1e-5; MSM also with a
+cutoff beyond half the smallest width), the Madelung energy on the GPU,
+pair counts, and FF_rebuild into a changed skewed box.
 """
 import ctypes
 
@@ -47,13 +53,13 @@ U2 = np.array([[1, 1, 0], [0, 1, 1], [0, 0, 1]])  # a' = a, b' = a + b, c' = b +
 ALPHA, DNN = 1.76267477307099, np.sqrt(3) / 2     # CsCl Madelung constant, nearest distance
 
 
-def _random_skewed(n=3000, seed=5, charged=0.0):
+def _random_skewed(n=3000, seed=5, charged=0.0, box=SKEW):
     rng = np.random.default_rng(seed)
     s = rng.uniform(0, 1, (n, 3))
-    pos = s @ SKEW.T  # r = A s
+    pos = s @ box.T  # r = A s
     q = rng.normal(size=n)
     q -= q.mean() - charged / n
-    return pos, q, SKEW.copy()
+    return pos, q, box.copy()
 
 
 def test_domain_is_half_the_smallest_width():
@@ -100,9 +106,25 @@ def test_msm_oracle_madelung_in_sheared_cell(n, nu, L, M, frac, oracle_mod):
         assert rel_err(Ea, exact) > 100 * rel_err(E, exact)
 
 
-def test_msm_oracle_forces_are_derivatives_in_skewed_box(oracle_mod):
-    pos, q, A = _random_skewed(n=40, seed=9, charged=0.2)
-    o = oracle_mod.Oracle(pos, q, A, nu=6, L=2, Mtop=(6, 6, 7), a0=8.0)
+# cutoffs beyond half the smallest width: every image inside the cutoff
+MSM_MADELUNG_BEYOND = [(4, 10, 1, 48, 0.7), (4, 10, 1, 48, 1.1), (4, 8, 2, 12, 0.8),
+                       (6, 8, 2, 16, 0.9)]
+
+
+@pytest.mark.parametrize("n,nu,L,M,frac", MSM_MADELUNG_BEYOND)
+def test_msm_oracle_madelung_beyond_half_width(n, nu, L, M, frac, oracle_mod):
+    pos, q, A = _cscl_sheared(U2, n)
+    a0 = frac * _widths(A).min()
+    o = oracle_mod.Oracle(pos, q, A, nu=nu, L=L, Mtop=M, a0=a0)
+    rel_est = o.deltaF() / force_scale(q, A)
+    E, F, phi, _ = o.energy(pos)
+    _madelung_check(E, F, phi, q, A, rel_est)
+
+
+@pytest.mark.parametrize("box,a0", [(SKEW, 8.0), (SKEW / 3, 7.0)])
+def test_msm_oracle_forces_are_derivatives_in_skewed_box(box, a0, oracle_mod):
+    pos, q, A = _random_skewed(n=40, seed=9, charged=0.2, box=box)
+    o = oracle_mod.Oracle(pos, q, A, nu=6, L=2, Mtop=(6, 6, 7), a0=a0)
     E, F, phi, _ = o.energy(pos)
     h, hq = 1e-5, 1e-6
     for i in (1, 17, 33):
@@ -119,15 +141,30 @@ def test_msm_oracle_forces_are_derivatives_in_skewed_box(oracle_mod):
         assert abs(fdq - phi[i]) < 1e-6 * max(1.0, abs(phi[i]))
 
 
-@pytest.mark.parametrize("method", ["msm", "pme"])
-def test_cutoff_beyond_half_width_refused(method):
+@pytest.mark.parametrize("method,layout", [("msm", "asis"), ("pme", "fixed")])
+def test_cutoff_beyond_half_width_refused(method, layout):
+    """The reference's own image rules (as-is layout) and the PME comparator
+    keep the cutoff below half the smallest width of a triclinic box."""
     from msmpb_amd import MSM
     from msmpb_amd.msm import MSMError
     pos, q, A = _random_skewed(n=200)
     a0 = 0.51 * _widths(A).min()
     kw = dict(nu=6, levels=2, top_grid=8) if method == "msm" else dict(nu=6, top_grid=32)
     with pytest.raises(MSMError, match="triclinic box: the cutoff"):
-        MSM(pos, q, A, cutoff=a0, method=method, **kw)
+        MSM(pos, q, A, cutoff=a0, method=method, layout=layout, **kw)
+
+
+def test_cutoff_beyond_half_width_accepted():
+    """The default layout (every image inside the cutoff) takes any cutoff:
+    the host plan accepts the box (without a GPU the build stops later, at
+    the device)."""
+    from msmpb_amd import MSM
+    from msmpb_amd.msm import MSMError
+    pos, q, A = _random_skewed(n=200)
+    try:
+        MSM(pos, q, A, cutoff=0.7 * _widths(A).min(), nu=6, levels=2, top_grid=8).close()
+    except MSMError as e:
+        assert "triclinic" not in str(e), str(e)
 
 
 # ---------------------------------------------------------------- GPU
@@ -144,6 +181,10 @@ CASES = {  # name: (system kwargs, method, MSM/PME parameters)
     "msm_L3_nu8_charged": (dict(seed=6, charged=0.4), "msm",
                            dict(nu=8, levels=3, top_grid=(4, 4, 5), cutoff=9.0)),
     "pme_nu6": (dict(seed=7), "pme", dict(nu=6, top_grid=(32, 36, 40), cutoff=9.0)),
+    # cutoff 0.7 of the smallest face-to-face width (14.8 A): several images
+    # of a neighbour, and of an atom itself, inside it
+    "msm_L2_nu6_beyond_half": (dict(n=800, seed=10, box=SKEW / 2), "msm",
+                               dict(nu=6, levels=2, top_grid=(6, 6, 7), cutoff=10.4)),
 }
 
 
@@ -177,6 +218,25 @@ def test_triclinic_matches_oracle(case, oracle_mod, torch_cuda):
                 assert rel_err(E, Eo) < 1e-5 and rms_rel_force(F, Fo) < 1e-5
         finally:
             m.close()
+
+
+@pytest.mark.gpu
+def test_beyond_half_width_refuses_asis_switch(oracle_mod, torch_cuda):
+    """A built FF whose cutoff exceeds half the smallest width keeps its
+    layout when asked for the as-is one (the reference's image rules), and
+    still evaluates as before."""
+    from msmpb_amd import MSM, _lib
+    sk, _, kw = CASES["msm_L2_nu6_beyond_half"]
+    pos, q, A = _random_skewed(**sk)
+    print("[triclinic] as-is switch refused beyond half the width", flush=True)
+    Eo = _oracle(oracle_mod, "msm", pos, q, A, kw).energy(pos, want_phi=False)[0]
+    m = MSM(pos, q, A, **kw)
+    try:
+        assert _lib.lib().FF_set_layout_compat(m._ff, _lib.MSM_LAYOUT_ASIS) != 0
+        assert "face-to-face" in _lib.last_error()
+        assert rel_err(m.energy(pos)[0], Eo) < TOL
+    finally:
+        m.close()
 
 
 @pytest.mark.gpu
