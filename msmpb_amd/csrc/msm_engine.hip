@@ -99,8 +99,6 @@ struct Engine {
   int *d_cell_of = nullptr, *d_cnt = nullptr, *d_start = nullptr, *d_sidx = nullptr;
   int *d_rank = nullptr;
   double *d_Fp = nullptr, *d_phip = nullptr;  // short-range parts of F and phi
-  double *d_ep2 = nullptr;                    // per-workgroup pair-energy partials
-  unsigned *d_done = nullptr;  // workgroup counter of the interpolation's energy reduction
   int *d_scell = nullptr, *d_flags = nullptr;
   double4 *d_wpos = nullptr, *d_spos = nullptr;
   // per-atom spreading placements (packed support origin; t and q), sorted order
@@ -136,7 +134,7 @@ struct Engine {
     T.clear();
     void *ptrs[] = {d_pos,  d_q,    d_F,    d_phi,   d_out,  d_qpart, d_scal,  d_epair,
                     d_egrid, d_cell_of, d_cnt, d_start, d_sidx, d_scell, d_flags, d_wpos,
-                    d_rank, d_done, d_Fp, d_phip, d_ep2,
+                    d_rank, d_Fp, d_phip,
                     d_spos, d_sten, d_count, d_qsplit, d_tw, d_topwork, d_stab, d_sscr,
                     d_sbase, d_st};
     for (void *p : ptrs)
@@ -160,8 +158,7 @@ struct Engine {
     }
     d_pos = d_q = d_F = d_phi = d_out = d_qpart = d_scal = d_epair = d_egrid = nullptr;
     d_cell_of = d_cnt = d_start = d_sidx = d_scell = d_flags = d_rank = nullptr;
-    d_done = nullptr;
-    d_Fp = d_phip = d_ep2 = nullptr;
+    d_Fp = d_phip = nullptr;
     d_sbase = nullptr;
     d_st = nullptr;
     d_wpos = d_spos = nullptr;
@@ -610,8 +607,6 @@ int create_engine(FFX *x, const msm::Plan &plan) {
   TRY(dalloc(&E->d_rank, N));
   TRY(dalloc(&E->d_Fp, 3 * N));
   TRY(dalloc(&E->d_phip, N));
-  TRY(dalloc(&E->d_ep2, 2 * msm::combine_blocks((int)N)));
-  TRY(dalloc(&E->d_done, 1));
   TRY(dalloc(&E->d_cnt, E->P.ncell));
   TRY(dalloc(&E->d_start, E->P.ncell + 1));
   TRY(dalloc(&E->d_sidx, N));
@@ -620,7 +615,6 @@ int create_engine(FFX *x, const msm::Plan &plan) {
   // counters that each evaluation leaves cleared for the next one
   HIP_TRY(hipMemset(E->d_cnt, 0, E->P.ncell * sizeof(int)));
   HIP_TRY(hipMemset(E->d_flags, 0, 4 * sizeof(int)));
-  HIP_TRY(hipMemset(E->d_done, 0, sizeof(unsigned)));
   TRY(dalloc(&E->d_wpos, N));
   TRY(dalloc(&E->d_spos, N));
   TRY(dalloc(&E->d_count, E->P.ncell));
@@ -759,7 +753,7 @@ int grid_energy_parts(const Engine &E) {
 
 msm::FinalizeArgs finalize_args(Engine &E, double *user) {
   return msm::FinalizeArgs{E.d_epair, E.P.ncell, E.d_egrid, grid_energy_parts(E), E.d_scal,
-                           E.d_flags, E.d_out, E.d_done, user == E.d_out ? nullptr : user};
+                           E.d_flags, E.d_out, user == E.d_out ? nullptr : user};
 }
 
 // Grid-energy partials for L = 1 (the level-1 prolongation writes them
@@ -777,7 +771,7 @@ void stage_grid_energy(Engine &E, int layout, hipStream_t s) {
 // from the reduction kernel itself.
 void stage_energy(Engine &E, double *F, double *phi, double *out, hipStream_t s) {
   if (F || phi)
-    msm::launch_combine(E.P, E.d_Fp, E.d_phip, F, phi, E.d_ep2, finalize_args(E, out), s);
+    msm::launch_combine(E.P, E.d_Fp, E.d_phip, F, phi, finalize_args(E, out), s);
   else
     msm::launch_finalize(E.P, finalize_args(E, out), s);
 }
@@ -1539,7 +1533,7 @@ int msm_adjoint_device(FF *ff, const double *d_pos, const double *d_q, double *d
       msm::launch_gather(E.P, E.gd[1], E.d_spos, E.d_sidx, E.d_start, E.d_el[1], E.d_scal,
                          d_force, d_phi, false, s);
       msm::launch_combine(E.P, d_force ? E.d_Fp : nullptr, d_phi ? E.d_phip : nullptr, d_force,
-                          d_phi, E.d_ep2, finalize_args(E, E.d_out), s);
+                          d_phi, finalize_args(E, E.d_out), s);
     }
   } else {
     stage_pairs(E, x->precision, d_force, d_phi, s);  // the pair kernel writes, gather adds

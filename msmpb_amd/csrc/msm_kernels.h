@@ -219,8 +219,7 @@ int grid_blocks(long n);
 void launch_grid_energy(const DevParams &P, GridDims g, const double *q1, const double *e1,
                         double *epart, int asis, hipStream_t s);
 // Final energy reduction: out = {E, constant part, pair part, grid part,
-// non-finite input count}; flags[0] is cleared.  `done` is a zero-initialised
-// counter used when the reduction rides on the interpolation launch.
+// non-finite input count}; flags[0] is cleared.
 struct FinalizeArgs {
   const double *ep;
   int np;
@@ -229,7 +228,6 @@ struct FinalizeArgs {
   const double *scal;
   int *flags;
   double *out;  // >= 5 doubles; null: no reduction
-  unsigned *done;
   double *user;  // optional caller buffer that also receives out[0..3]
 };
 // Interpolation: F (=, or += with add) q E and phi (=, +=) grid potential
@@ -239,11 +237,9 @@ void launch_gather(const DevParams &P, GridDims g, const double4 *spos, const in
                    const int *start, const double *e1, const double *scal, double *F,
                    double *phi, bool add, hipStream_t s);
 void launch_finalize(const DevParams &P, const FinalizeArgs &fin, hipStream_t s);
-// F += Fp, phi += phip (either may be null) and the energy reduction; ep2
-// holds 2 doubles per workgroup (combine_blocks(N)).
-int combine_blocks(int N);
+// F += Fp, phi += phip (either may be null) and the energy reduction.
 void launch_combine(const DevParams &P, const double *Fp, const double *phip, double *F,
-                    double *phi, double *ep2, const FinalizeArgs &fin, hipStream_t s);
+                    double *phi, const FinalizeArgs &fin, hipStream_t s);
 
 
 // ---- box-dependent stencils on the GPU (FF_rebuild) ----

@@ -1975,24 +1975,74 @@ __device__ __forceinline__ void block_csum(double &s, double &c) {
   __syncthreads();  // ws, wc free for the next call
 }
 
-// Compensated sums of the per-cell pair partials and per-block grid partials:
-// each of 256 threads sums a fixed strided subset, then a fixed binary tree
-// combines the (sum, carry) pairs, so the result does not depend on
-// scheduling.  out = {E, constant part, pair part, grid part, flags}; the
-// non-finite-input counter is copied to out[4] and cleared for the next
-// evaluation.
-// Tree-combines the BS threads' compensated pair sums (sv, cv) with the
-// grid partials and writes out.
+// Compensated sums of the pair and the grid partials over the threads of a
+// BS-thread workgroup: thread t takes p[t], p[t + BS], ... of each in that
+// order.  The partials sit in memory written by other kernels, so every
+// load of a round (UP pair and UG grid partials per thread) is issued
+// before any is summed: one memory latency per round instead of one per
+// value.
+template <int BS, int UP, int UG>
+__device__ __forceinline__ void strided_csum2(const double *__restrict__ ep, int np,
+                                              const double *__restrict__ eg, int ng, double &sp,
+                                              double &cp, double &sg, double &cg) {
+  const int t = threadIdx.x;
+  for (int r = 0; t + r * UP * BS < np || t + r * UG * BS < ng; ++r) {  // rounds
+    const int kp = t + r * UP * BS, kg = t + r * UG * BS;
+    double v[UP], w[UG];
+#pragma unroll
+    for (int u = 0; u < UP; ++u) v[u] = kp + u * BS < np ? ep[kp + u * BS] : 0.0;
+#pragma unroll
+    for (int u = 0; u < UG; ++u) w[u] = kg + u * BS < ng ? eg[kg + u * BS] : 0.0;
+#pragma unroll
+    for (int u = 0; u < UP; ++u) neumaier(sp, cp, v[u]);  // adding 0.0 changes neither
+#pragma unroll
+    for (int u = 0; u < UG; ++u) neumaier(sg, cg, w[u]);
+  }
+}
+
+// Two compensated sums (s1, c1), (s2, c2) over a workgroup at once (the
+// butterflies of block_csum interleaved, one barrier); thread 0 receives
+// the totals.
 template <int BS>
-__device__ void finalize_tail(const DevParams &P, const FinalizeArgs &f, double sv, double cv) {
-  int t = threadIdx.x;
-  double s = 0, c = 0;
-  for (int k = t; k < f.ng; k += BS) neumaier(s, c, f.eg[k]);
-  block_csum<BS>(sv, cv);
-  block_csum<BS>(s, c);
+__device__ __forceinline__ void block_csum2(double &s1, double &c1, double &s2, double &c2) {
+  constexpr int NW = BS / 64;
+  __shared__ double ws[4][NW];
+#pragma unroll
+  for (int m = 1; m < 64; m <<= 1) {
+    const double a = __shfl_xor(s1, m, 64), b = __shfl_xor(c1, m, 64);
+    const double d = __shfl_xor(s2, m, 64), e = __shfl_xor(c2, m, 64);
+    csum_add(s1, c1, a, b);
+    csum_add(s2, c2, d, e);
+  }
+  const int t = threadIdx.x;
+  if ((t & 63) == 0) ws[0][t >> 6] = s1, ws[1][t >> 6] = c1, ws[2][t >> 6] = s2, ws[3][t >> 6] = c2;
+  __syncthreads();
+  if (t == 0) {
+    s1 = ws[0][0], c1 = ws[1][0], s2 = ws[2][0], c2 = ws[3][0];
+    for (int w = 1; w < NW; ++w) csum_add(s1, c1, ws[0][w], ws[1][w]), csum_add(s2, c2, ws[2][w], ws[3][w]);
+  }
+}
+
+// The energy of an evaluation by one workgroup: compensated sums of the
+// per-cell pair partials and the per-block grid partials, each thread over
+// a fixed strided subset, then the wavefronts and the wavefront totals in a
+// fixed order (the result does not depend on scheduling), and the constant
+// terms.  out = {E, constant part, pair part, grid part, flags}; the
+// non-finite-input counter is copied to out[4] and cleared for the next
+// evaluation.  One workgroup, so no other workgroup's results need to be
+// made visible to it (a last-workgroup-done reduction needs a device-scope
+// release per workgroup, which on this GPU writes back and invalidates L2).
+template <int BS>
+__device__ void energy_block(const DevParams &P, const FinalizeArgs &f) {
+  const int t = threadIdx.x;
+  double qs = 0.0, q2 = 0.0;
+  int flags = 0;
+  if (t == 0) qs = f.scal[0], q2 = f.scal[1], flags = *f.flags;  // in flight with the rest
+  double sv = 0, cv = 0, s = 0, c = 0;
+  strided_csum2<BS, 32, 4>(f.ep, f.np, f.eg, f.ng, sv, cv, s, c);
+  block_csum2<BS>(sv, cv, s, c);
   if (t != 0) return;
   double epair = sv + cv, egrid = 0.5 * (s + c);
-  double qs = f.scal[0], q2 = f.scal[1];
   // constant part (forcefield1.c:114-130)
   double econst = 0.5 * q2 * P.coeff1 - 0.5 * (qs * qs) * P.coeff2 - 0.5 * q2 * P.gamma0 * P.inv_a0;
   s = 0, c = 0;
@@ -2003,7 +2053,7 @@ __device__ void finalize_tail(const DevParams &P, const FinalizeArgs &f, double 
   f.out[1] = econst;
   f.out[2] = epair;
   f.out[3] = egrid;
-  f.out[4] = (double)*f.flags;
+  f.out[4] = (double)flags;
   if (f.user) {
     f.user[0] = f.out[0];
     f.user[1] = econst;
@@ -2016,9 +2066,7 @@ __device__ void finalize_tail(const DevParams &P, const FinalizeArgs &f, double 
 constexpr int kFinBS = 1024;
 __global__ __launch_bounds__(kFinBS) void k_finalize(DevParams P, FinalizeArgs f) {
   chain_issue_priority();
-  double s = 0, c = 0;
-  for (int k = threadIdx.x; k < f.np; k += kFinBS) neumaier(s, c, f.ep[k]);
-  finalize_tail<kFinBS>(P, f, s, c);
+  energy_block<kFinBS>(P, f);
 }
 
 // interpolate (forcefield1.c:570-607) + F += q E (:228-231), and the
@@ -2281,43 +2329,23 @@ __global__ __launch_bounds__(256) void k_gather_tile(DevParams P, GridDims g, Ti
 }
 
 // F += Fp and phi += phip (the short-range parts, written by the pair kernel
-// beside the interpolation), then the energy reduction: every workgroup
-// sums a fixed slice of the per-cell pair partials into a compensated
-// (sum, carry) pair, and the last workgroup to finish combines those and the
-// grid partials in a fixed order.
-__global__ __launch_bounds__(256) void k_combine(DevParams P, const double *__restrict__ Fp,
-                                                 const double *__restrict__ phip,
-                                                 double *__restrict__ F,
-                                                 double *__restrict__ phi, double *__restrict__ ep2,
-                                                 FinalizeArgs fin) {
+// beside the interpolation) by workgroups 1 ..; workgroup 0 reduces the
+// energy meanwhile (energy_block).
+__global__ __launch_bounds__(kFinBS) void k_combine(DevParams P, const double *__restrict__ Fp,
+                                                    const double *__restrict__ phip,
+                                                    double *__restrict__ F,
+                                                    double *__restrict__ phi, FinalizeArgs fin) {
   chain_issue_priority();
-  __shared__ int last;
-  const int t = threadIdx.x, b = blockIdx.x;
-  const long i = (long)b * 256 + t;
+  if (blockIdx.x == 0) {
+    energy_block<kFinBS>(P, fin);
+    return;
+  }
+  const long i = (long)(blockIdx.x - 1) * kFinBS + threadIdx.x;
   if (i < P.N) {
     if (F)
       for (int d = 0; d < 3; ++d) F[3 * i + d] += Fp[3 * i + d];
     if (phi) phi[i] += phip[i];
   }
-  const int chunk = (fin.np + gridDim.x - 1) / gridDim.x;
-  const int k0 = b * chunk, k1 = min(fin.np, k0 + chunk);
-  double sv = 0.0, cv = 0.0;
-  for (int k = k0 + t; k < k1; k += 256) neumaier(sv, cv, fin.ep[k]);
-  block_csum<256>(sv, cv);
-  if (t == 0) {
-    ep2[2 * b] = sv;
-    ep2[2 * b + 1] = cv;
-    __threadfence();
-    last = atomicAdd(fin.done, 1u) == gridDim.x - 1;
-  }
-  __syncthreads();
-  if (!last) return;
-  __threadfence();
-  if (t == 0) *fin.done = 0u;
-  // the pair partials are now the gridDim.x (sum, carry) pairs of ep2
-  sv = 0.0, cv = 0.0;
-  for (int k = t; k < (int)gridDim.x; k += 256) csum_add(sv, cv, ep2[2 * k], ep2[2 * k + 1]);
-  finalize_tail<256>(P, fin, sv, cv);
 }
 
 inline unsigned nblk(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
@@ -2579,11 +2607,11 @@ void launch_gather(const DevParams &P, GridDims g, const double4 *spos, const in
 #undef GA
 }
 
-int combine_blocks(int N) { return (int)nblk(N, 256); }
+
 
 void launch_combine(const DevParams &P, const double *Fp, const double *phip, double *F,
-                    double *phi, double *ep2, const FinalizeArgs &fin, hipStream_t s) {
-  k_combine<<<combine_blocks(P.N), 256, 0, s>>>(P, Fp, phip, F, phi, ep2, fin);
+                    double *phi, const FinalizeArgs &fin, hipStream_t s) {
+  k_combine<<<1 + nblk(P.N, kFinBS), kFinBS, 0, s>>>(P, Fp, phip, F, phi, fin);
 }
 
 void launch_finalize(const DevParams &P, const FinalizeArgs &fin, hipStream_t s) {
