@@ -97,9 +97,12 @@ struct Engine {
   double *d_out = nullptr, *d_qpart = nullptr, *d_scal = nullptr;
   double *d_epair = nullptr, *d_egrid = nullptr;
   int *d_cell_of = nullptr, *d_cnt = nullptr, *d_start = nullptr, *d_sidx = nullptr;
-  int *d_rank = nullptr;
+  // cell members by arrival (ncell rows of bcap), the overflow list and its
+  // counts (binning -> scan -> placement)
+  int *d_bucket = nullptr, *d_ovf = nullptr, *d_ovfn = nullptr;
+  int bcap = 0;
   double *d_Fp = nullptr, *d_phip = nullptr;  // short-range parts of F and phi
-  int *d_scell = nullptr, *d_flags = nullptr;
+  int *d_flags = nullptr;
   double4 *d_wpos = nullptr, *d_spos = nullptr;
   // per-atom spreading placements (packed support origin; t and q), sorted order
   unsigned *d_sbase = nullptr;
@@ -133,8 +136,8 @@ struct Engine {
   void release() {
     T.clear();
     void *ptrs[] = {d_pos,  d_q,    d_F,    d_phi,   d_out,  d_qpart, d_scal,  d_epair,
-                    d_egrid, d_cell_of, d_cnt, d_start, d_sidx, d_scell, d_flags, d_wpos,
-                    d_rank, d_Fp, d_phip,
+                    d_egrid, d_cell_of, d_cnt, d_start, d_sidx, d_flags, d_wpos,
+                    d_bucket, d_ovf, d_ovfn, d_Fp, d_phip,
                     d_spos, d_sten, d_count, d_qsplit, d_tw, d_topwork, d_stab, d_sscr,
                     d_sbase, d_st};
     for (void *p : ptrs)
@@ -157,7 +160,7 @@ struct Engine {
       d_ksplit[l] = nullptr;
     }
     d_pos = d_q = d_F = d_phi = d_out = d_qpart = d_scal = d_epair = d_egrid = nullptr;
-    d_cell_of = d_cnt = d_start = d_sidx = d_scell = d_flags = d_rank = nullptr;
+    d_cell_of = d_cnt = d_start = d_sidx = d_flags = d_bucket = d_ovf = d_ovfn = nullptr;
     d_Fp = d_phip = nullptr;
     d_sbase = nullptr;
     d_st = nullptr;
@@ -604,17 +607,22 @@ int create_engine(FFX *x, const msm::Plan &plan) {
   TRY(dalloc(&E->d_egrid, std::max<long>(msm::grid_blocks(E->gd[1].size()),
                                           msm::prolong_blocks(E->gd[1]))));
   TRY(dalloc(&E->d_cell_of, N));
-  TRY(dalloc(&E->d_rank, N));
+  // bucket rows of several times the mean occupancy (a cell past its row
+  // still works, through the overflow list, only more slowly)
+  E->bcap = 16 * (int)std::ceil((4.0 * (double)N / E->P.ncell + 8.0) / 16.0);
+  TRY(dalloc(&E->d_bucket, (size_t)E->P.ncell * E->bcap));
+  TRY(dalloc(&E->d_ovf, N));
+  TRY(dalloc(&E->d_ovfn, 2));
   TRY(dalloc(&E->d_Fp, 3 * N));
   TRY(dalloc(&E->d_phip, N));
   TRY(dalloc(&E->d_cnt, E->P.ncell));
   TRY(dalloc(&E->d_start, E->P.ncell + 1));
   TRY(dalloc(&E->d_sidx, N));
-  TRY(dalloc(&E->d_scell, N));
   TRY(dalloc(&E->d_flags, 4));
   // counters that each evaluation leaves cleared for the next one
   HIP_TRY(hipMemset(E->d_cnt, 0, E->P.ncell * sizeof(int)));
   HIP_TRY(hipMemset(E->d_flags, 0, 4 * sizeof(int)));
+  HIP_TRY(hipMemset(E->d_ovfn, 0, 2 * sizeof(int)));
   TRY(dalloc(&E->d_wpos, N));
   TRY(dalloc(&E->d_spos, N));
   TRY(dalloc(&E->d_count, E->P.ncell));
@@ -640,15 +648,17 @@ int create_engine(FFX *x, const msm::Plan &plan) {
 // the pair stream to wait on.
 void stage_prepare(Engine &E, const double *pos, const double *q, hipStream_t s,
                    hipEvent_t fork = nullptr) {
-  // d_cnt is zero here: set at allocation, cleared again by every cell sort
-  msm::launch_bin(E.P, pos, q, E.d_cell_of, E.d_rank, E.d_cnt, E.d_wpos, E.d_qpart, E.d_flags, s);
-  msm::launch_scan(E.P, E.d_cnt, E.d_start, E.d_qpart, msm::bin_blocks(E.P.N), E.d_scal, s);
-  msm::launch_scatter(E.P, E.d_cell_of, E.d_rank, E.d_wpos, E.d_start, E.d_spos, E.d_sidx,
-                      E.d_scell, s);
-  // the cell sort also writes the atoms' spreading placements
+  // d_cnt and d_ovfn[0] are zero here: set at allocation, cleared again by
+  // every scan
+  msm::launch_bin(E.P, pos, q, E.d_cell_of, E.d_bucket, E.bcap, E.d_ovfn, E.d_ovf, E.d_cnt,
+                  E.d_wpos, E.d_qpart, E.d_flags, s);
+  msm::launch_scan(E.P, E.d_cnt, E.d_start, E.d_qpart, msm::bin_blocks(E.P.N), E.d_scal,
+                   E.d_ovfn, s);
+  // the placement pass also writes the atoms' spreading placements
   const bool place = msm::spread_by_placements(E.P, E.gd[1]);
-  msm::launch_cell_sort(E.P, E.d_start, E.d_cnt, E.d_spos, E.d_sidx, s, fork, E.gd[1],
-                        place ? E.d_sbase : nullptr, place ? E.d_st : nullptr);
+  msm::launch_place(E.P, E.d_cell_of, E.d_wpos, E.d_start, E.d_bucket, E.bcap, E.d_ovfn,
+                    E.d_ovf, E.d_spos, E.d_sidx, s, fork, E.gd[1],
+                    place ? E.d_sbase : nullptr, place ? E.d_st : nullptr);
 }
 
 void stage_pairs(Engine &E, int precision, double *F, double *phi, hipStream_t s,

@@ -87,25 +87,30 @@ void launch_fill_int(int *p, long n, int v, hipStream_t s);
 void launch_fill_dbl(double *p, long n, double v, hipStream_t s);
 
 // Binning: wrap positions into the box, count atoms per cell (cnt must be
-// zero on entry; rank = arrival order within the cell), partial sums of q
-// and q^2 per block (2 doubles per block into qpart), non-finite inputs
-// counted in flags[0].
+// zero on entry) and record each atom in its cell's bucket row of cap
+// entries (bucket: ncell x cap) at its arrival rank, or past the row on the
+// overflow list ovf (N entries; ovfn[0] counts them, zero on entry); partial
+// sums of q and q^2 per block (2 doubles per block into qpart), non-finite
+// inputs counted in flags[0].
 int bin_blocks(int N);
-void launch_bin(const DevParams &P, const double *pos, const double *q, int *cell_of, int *rank,
-                int *cnt, double4 *wpos, double *qpart, int *flags, hipStream_t s);
-// Exclusive scan of cnt into start[0..ncell] and reduction of qpart into
-// scal[0] = sum q, scal[1] = sum q^2.
-void launch_scan(const DevParams &P, const int *cnt, int *start, const double *qpart, int nqpart,
-                 double *scal, hipStream_t s);
-void launch_scatter(const DevParams &P, const int *cell_of, const int *rank, const double4 *wpos,
-                    const int *start, double4 *spos, int *sidx, int *scell, hipStream_t s);
-// Orders atoms inside each cell by original index (deterministic results)
-// and clears cnt for the next binning.  With sbase, st (N entries each) it
-// also writes every atom's spreading placement on the grid g1 (as the
-// placement pass of launch_spread does).
-void launch_cell_sort(const DevParams &P, const int *start, int *cnt, double4 *spos, int *sidx,
-                      hipStream_t s, hipEvent_t done = nullptr, GridDims g1 = GridDims{0, 0, 0},
-                      unsigned *sbase = nullptr, double4 *st = nullptr);
+void launch_bin(const DevParams &P, const double *pos, const double *q, int *cell_of, int *bucket,
+                int cap, int *ovfn, int *ovf, int *cnt, double4 *wpos, double *qpart, int *flags,
+                hipStream_t s);
+// Exclusive scan of cnt into start[0..ncell], cnt and ovfn[0] cleared for
+// the next binning (the overflow count moved to ovfn[1]), and reduction of
+// qpart into scal[0] = sum q, scal[1] = sum q^2.
+void launch_scan(const DevParams &P, int *cnt, int *start, const double *qpart, int nqpart,
+                 double *scal, int *ovfn, hipStream_t s);
+// The cell-sorted order: sorted positions and charges (spos) and original
+// indices (sidx), atoms ordered inside each cell by original index
+// (deterministic results).  With sbase, st (N entries each) it also writes
+// every atom's spreading placement on the grid g1 (as the placement pass of
+// launch_spread does).
+void launch_place(const DevParams &P, const int *cell_of, const double4 *wpos, const int *start,
+                  const int *bucket, int cap, const int *ovfn, const int *ovf, double4 *spos,
+                  int *sidx, hipStream_t s, hipEvent_t done = nullptr,
+                  GridDims g1 = GridDims{0, 0, 0}, unsigned *sbase = nullptr,
+                  double4 *st = nullptr);
 
 // Short-range pairs, full neighbour shell, one wavefront per home cell;
 // cols: (dx, dy, dzlo, dzhi) neighbour cell columns, dzhi - dzlo < nc[2].
@@ -129,7 +134,7 @@ void launch_pair_count(const DevParams &P, const double4 *spos, const int *start
 // grid brick (every grid point written exactly once).
 // q1b (optional): q^1 also as bf16_split pairs.  sbase, st (optional, N
 // entries each): per-atom placements, computed first (placed: already
-// written by the cell sort of the same evaluation) and used where
+// written by the placement pass of the binning) and used where
 // spread_by_placements(P, g) holds.
 void launch_spread(const DevParams &P, GridDims g, const double4 *spos, const int *start,
                    double *q1, hipStream_t s, unsigned *q1b = nullptr, unsigned *sbase = nullptr,

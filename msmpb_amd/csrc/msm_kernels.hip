@@ -145,9 +145,15 @@ __device__ __forceinline__ void lattice_rt(const DevParams &P, int kx, int ky, i
   else lattice<false>(P, kx, ky, kz, x, y, z);
 }
 
+// Cell of every atom, its wrapped position, and the cell's members: the
+// counting atomic gives the atom's arrival rank r in its cell, and the atom
+// is recorded in the cell's bucket row (cap entries) at r, or, past the row,
+// on the overflow list (ovfn[0] entries).  Also the per-workgroup charge sums.
 __global__ __launch_bounds__(kBinBS) void k_bin(DevParams P, const double *__restrict__ pos,
                                                 const double *__restrict__ q,
-                                                int *__restrict__ cell_of, int *__restrict__ rank,
+                                                int *__restrict__ cell_of, int *__restrict__ bucket,
+                                                int cap, int *__restrict__ ovfn,
+                                                int *__restrict__ ovf,
                                                 int *__restrict__ cnt, double4 *__restrict__ wpos,
                                                 double *__restrict__ qpart, int *__restrict__ flags) {
   int i = blockIdx.x * kBinBS + threadIdx.x;
@@ -186,8 +192,10 @@ __global__ __launch_bounds__(kBinBS) void k_bin(DevParams P, const double *__res
     if (!ok) atomicAdd(flags, 1);
     int cell = (c[0] * P.nc[1] + c[1]) * P.nc[2] + c[2];
     cell_of[i] = cell;
-    rank[i] = atomicAdd(cnt + cell, 1);  // arrival order; the cell sort fixes the order
     wpos[i] = make_double4(rw[0], rw[1], rw[2], qi);
+    const int rk = atomicAdd(cnt + cell, 1);  // arrival order; k_place fixes the order
+    if (rk < cap) bucket[(long)cell * cap + rk] = i;
+    else ovf[atomicAdd(ovfn, 1)] = i;
   }
   double s1 = block_sum<kBinBS>(qi);
   __syncthreads();
@@ -205,14 +213,16 @@ constexpr int kScanBS = 1024;
 // 4t .. 4t + 3 of every tile of the pass (coalesced, all loads in flight
 // together), the per-thread sums are scanned per tile across the workgroup
 // (wavefront shuffles, then the wavefront totals), and the prefixes are
-// written back four at a time.  The counts are left in place (the cell sort
-// clears them for the next evaluation).  Also reduces the per-block charge
-// sums of the binning pass.
+// written back four at a time.  The counts are cleared as they are read,
+// ready for the next evaluation's binning, and so is the overflow count
+// (moved to ovfn[1] for k_place).  Also reduces the per-block charge sums
+// of the binning pass.
 constexpr int kScanTiles = 8;
-__global__ __launch_bounds__(kScanBS) void k_scan(int n, const int *__restrict__ cnt,
+__global__ __launch_bounds__(kScanBS) void k_scan(int n, int *__restrict__ cnt,
                                                   int *__restrict__ start,
                                                   const double *__restrict__ qpart, int nq,
-                                                  double *__restrict__ scal) {
+                                                  double *__restrict__ scal,
+                                                  int *__restrict__ ovfn) {
   constexpr int NW = kScanBS / 64, TILE = 4 * kScanBS;
   __shared__ int wtot[kScanTiles][NW];
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
@@ -227,11 +237,13 @@ __global__ __launch_bounds__(kScanBS) void k_scan(int n, const int *__restrict__
       const int i = base + j * TILE + 4 * t;
       if (i + 3 < n) {
         v[j] = *reinterpret_cast<const int4 *>(cnt + i);
+        *reinterpret_cast<int4 *>(cnt + i) = make_int4(0, 0, 0, 0);
       } else {
         v[j].x = i < n ? cnt[i] : 0;
         v[j].y = i + 1 < n ? cnt[i + 1] : 0;
         v[j].z = i + 2 < n ? cnt[i + 2] : 0;
         v[j].w = 0;
+        for (int k = i; k < min(i + 3, n); ++k) cnt[k] = 0;
       }
       sum[j] = v[j].x + v[j].y + v[j].z + v[j].w;
       incl[j] = sum[j];
@@ -269,7 +281,11 @@ __global__ __launch_bounds__(kScanBS) void k_scan(int n, const int *__restrict__
     }
     __syncthreads();  // wtot is rewritten by the next pass
   }
-  if (t == 0) start[n] = carry;
+  if (t == 0) {
+    start[n] = carry;
+    ovfn[1] = ovfn[0];
+    ovfn[0] = 0;
+  }
   // charge sums for the constant energy terms
   const double a = block_sum<kScanBS>(q1);
   __syncthreads();
@@ -277,25 +293,6 @@ __global__ __launch_bounds__(kScanBS) void k_scan(int n, const int *__restrict__
   if (t == 0) scal[0] = a, scal[1] = c2;
 }
 
-__global__ __launch_bounds__(256) void k_scatter(int N, const int *__restrict__ cell_of,
-                                                 const int *__restrict__ rank,
-                                                 const double4 *__restrict__ wpos,
-                                                 const int *__restrict__ start,
-                                                 double4 *__restrict__ spos,
-                                                 int *__restrict__ sidx,
-                                                 int *__restrict__ scell) {
-  int i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= N) return;
-  int c = cell_of[i];
-  int slot = start[c] + rank[i];
-  spos[slot] = wpos[i];
-  sidx[slot] = i;
-  scell[slot] = c;
-}
-
-// W lanes per cell (W = 16 when cells hold a few atoms, else 64): ranks the
-// cell's atoms by original index (so the order, and every sum built on it, is
-// deterministic) and moves them in place.
 // Placement of a sorted atom on the level-1 grid for the spreading
 // (anterpolate, forcefield1.c:350-368): fractional offsets t in [0, 1) and
 // the index of the lowest support point m - NU/2 + 1 per axis, reduced
@@ -321,59 +318,56 @@ __device__ __forceinline__ void place_atom(const DevParams &P, GridDims g, const
   tq = make_double4(t[0], t[1], t[2], at.w);
 }
 
-// With sbase != null it also writes each atom's spreading placement
-// (place_atom): the spreading kernel then follows the binning on the same
-// stream directly, and is dispatched before the pair kernel on its own
-// stream can take the CUs.
-template <int W, bool TRI>
-__global__ __launch_bounds__(256) void k_cell_sort(DevParams P, const int *__restrict__ start,
-                                                   int *__restrict__ cnt,
-                                                   double4 *__restrict__ spos,
-                                                   int *__restrict__ sidx, GridDims g1,
-                                                   unsigned *__restrict__ sbase,
-                                                   double4 *__restrict__ st) {
-  const int c = blockIdx.x * (256 / W) + threadIdx.x / W;
-  const int lane = threadIdx.x % W;
-  if (c >= P.ncell) return;
-  if (lane == 0) cnt[c] = 0;  // ready for the next evaluation's binning
+// The cell-sorted order, one thread per atom: atom i's place in its cell c
+// is the number of c's atoms with a smaller original index (so the order,
+// and every sum built on it, is deterministic), counted over c's bucket row
+// and, for a cell that overflowed it, the overflow list (a scan of the whole
+// list per such atom: rows hold several times the mean occupancy, so
+// overflows are rare).  Writes the sorted position and charge, the original
+// index and, with sbase != null, the spreading placement (place_atom): the
+// spreading kernel then follows the binning on the same stream directly,
+// and is dispatched before the pair kernel on its own stream can take the
+// CUs.
+constexpr int kPlaceAhead = 8;  // bucket entries in flight per thread
+template <bool TRI>
+__global__ __launch_bounds__(256) void k_place(DevParams P, const int *__restrict__ cell_of,
+                                               const double4 *__restrict__ wpos,
+                                               const int *__restrict__ start,
+                                               const int *__restrict__ bucket, int cap,
+                                               const int *__restrict__ ovfn,
+                                               const int *__restrict__ ovf,
+                                               double4 *__restrict__ spos, int *__restrict__ sidx,
+                                               GridDims g1, unsigned *__restrict__ sbase,
+                                               double4 *__restrict__ st) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= P.N) return;
+  const double4 v = wpos[i];
+  const int c = cell_of[i];
+  const int *row = bucket + (long)c * cap;
   const int b = start[c], n = start[c + 1] - b;
-  if (n <= W) {
-    const int key = lane < n ? sidx[b + lane] : 0x7fffffff;
-    const double4 v = lane < n ? spos[b + lane] : make_double4(0, 0, 0, 0);
-    int rank = 0;
-    for (int j = 0; j < n; ++j) rank += __shfl(key, j, W) < key;
-    if (lane < n) {
-      sidx[b + rank] = key;
-      spos[b + rank] = v;
-      if (sbase) {
-        unsigned pk;
-        double4 tq;
-        place_atom<TRI>(P, g1, v, pk, tq);
-        sbase[b + rank] = pk;
-        st[b + rank] = tq;
-      }
+  int rank = 0;
+  for (int k0 = 0; k0 < min(n, cap); k0 += kPlaceAhead) {
+    int e[kPlaceAhead];
+#pragma unroll
+    for (int u = 0; u < kPlaceAhead; ++u) e[u] = k0 + u < min(n, cap) ? row[k0 + u] : i;
+#pragma unroll
+    for (int u = 0; u < kPlaceAhead; ++u) rank += e[u] < i;
+  }
+  if (n > cap) {
+    const int nv = ovfn[1];
+    for (int k = 0; k < nv; ++k) {
+      const int j = ovf[k];
+      rank += j < i && cell_of[j] == c;
     }
-  } else if (lane == 0) {  // crowded cell: insertion sort by one lane
-    for (int k = b + 1; k < b + n; ++k) {
-      const int key = sidx[k];
-      const double4 v = spos[k];
-      int m = k - 1;
-      while (m >= b && sidx[m] > key) {
-        sidx[m + 1] = sidx[m];
-        spos[m + 1] = spos[m];
-        --m;
-      }
-      sidx[m + 1] = key;
-      spos[m + 1] = v;
-    }
-    if (sbase)
-      for (int k = b; k < b + n; ++k) {
-        unsigned pk;
-        double4 tq;
-        place_atom<TRI>(P, g1, spos[k], pk, tq);
-        sbase[k] = pk;
-        st[k] = tq;
-      }
+  }
+  spos[b + rank] = v;
+  sidx[b + rank] = i;
+  if (sbase) {
+    unsigned pk;
+    double4 tq;
+    place_atom<TRI>(P, g1, v, pk, tq);
+    sbase[b + rank] = pk;
+    st[b + rank] = tq;
   }
 }
 
@@ -2362,31 +2356,25 @@ void launch_fill_dbl(double *p, long n, double v, hipStream_t s) {
 
 int bin_blocks(int N) { return (int)nblk(N, kBinBS); }
 
-void launch_bin(const DevParams &P, const double *pos, const double *q, int *cell_of, int *rank,
-                int *cnt, double4 *wpos, double *qpart, int *flags, hipStream_t s) {
-  k_bin<<<bin_blocks(P.N), kBinBS, 0, s>>>(P, pos, q, cell_of, rank, cnt, wpos, qpart, flags);
+void launch_bin(const DevParams &P, const double *pos, const double *q, int *cell_of, int *bucket,
+                int cap, int *ovfn, int *ovf, int *cnt, double4 *wpos, double *qpart, int *flags,
+                hipStream_t s) {
+  k_bin<<<bin_blocks(P.N), kBinBS, 0, s>>>(P, pos, q, cell_of, bucket, cap, ovfn, ovf, cnt, wpos,
+                                           qpart, flags);
 }
 
-void launch_scan(const DevParams &P, const int *cnt, int *start, const double *qpart, int nqpart,
-                 double *scal, hipStream_t s) {
-  k_scan<<<1, kScanBS, 0, s>>>(P.ncell, cnt, start, qpart, nqpart, scal);
+void launch_scan(const DevParams &P, int *cnt, int *start, const double *qpart, int nqpart,
+                 double *scal, int *ovfn, hipStream_t s) {
+  k_scan<<<1, kScanBS, 0, s>>>(P.ncell, cnt, start, qpart, nqpart, scal, ovfn);
 }
 
-void launch_scatter(const DevParams &P, const int *cell_of, const int *rank, const double4 *wpos,
-                    const int *start, double4 *spos, int *sidx, int *scell, hipStream_t s) {
-  k_scatter<<<nblk(P.N, 256), 256, 0, s>>>(P.N, cell_of, rank, wpos, start, spos, sidx, scell);
-}
-
-void launch_cell_sort(const DevParams &P, const int *start, int *cnt, double4 *spos, int *sidx,
-                      hipStream_t s, hipEvent_t done, GridDims g1, unsigned *sbase, double4 *st) {
-#define A P, start, cnt, spos, sidx, g1, sbase, st
-  if (P.N <= 6L * P.ncell) {  // a few atoms per cell: 16 lanes each
-    if (P.tri) launch_done(k_cell_sort<16, true>, nblk(P.ncell, 16), 256, 0, s, done, A);
-    else launch_done(k_cell_sort<16, false>, nblk(P.ncell, 16), 256, 0, s, done, A);
-  } else {
-    if (P.tri) launch_done(k_cell_sort<64, true>, nblk(P.ncell, 4), 256, 0, s, done, A);
-    else launch_done(k_cell_sort<64, false>, nblk(P.ncell, 4), 256, 0, s, done, A);
-  }
+void launch_place(const DevParams &P, const int *cell_of, const double4 *wpos, const int *start,
+                  const int *bucket, int cap, const int *ovfn, const int *ovf, double4 *spos,
+                  int *sidx, hipStream_t s, hipEvent_t done, GridDims g1, unsigned *sbase,
+                  double4 *st) {
+#define A P, cell_of, wpos, start, bucket, cap, ovfn, ovf, spos, sidx, g1, sbase, st
+  if (P.tri) launch_done(k_place<true>, nblk(P.N, 256), 256, 0, s, done, A);
+  else launch_done(k_place<false>, nblk(P.N, 256), 256, 0, s, done, A);
 #undef A
 }
 
