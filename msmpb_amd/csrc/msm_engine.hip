@@ -615,12 +615,14 @@ int create_engine(FFX *x, const msm::Plan &plan) {
   TRY(dalloc(&E->d_ovfn, 2));
   TRY(dalloc(&E->d_Fp, 3 * N));
   TRY(dalloc(&E->d_phip, N));
-  TRY(dalloc(&E->d_cnt, E->P.ncell));
-  TRY(dalloc(&E->d_start, E->P.ncell + 1));
+  // the scan reads and writes whole int4 (padding of cnt zero)
+  const size_t ncell4 = ((size_t)E->P.ncell + 3) / 4 * 4;
+  TRY(dalloc(&E->d_cnt, ncell4));
+  TRY(dalloc(&E->d_start, ncell4 + 4));
   TRY(dalloc(&E->d_sidx, N));
   TRY(dalloc(&E->d_flags, 4));
   // counters that each evaluation leaves cleared for the next one
-  HIP_TRY(hipMemset(E->d_cnt, 0, E->P.ncell * sizeof(int)));
+  HIP_TRY(hipMemset(E->d_cnt, 0, ncell4 * sizeof(int)));
   HIP_TRY(hipMemset(E->d_flags, 0, 4 * sizeof(int)));
   HIP_TRY(hipMemset(E->d_ovfn, 0, 2 * sizeof(int)));
   TRY(dalloc(&E->d_wpos, N));

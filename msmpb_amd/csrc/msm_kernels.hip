@@ -208,15 +208,47 @@ __global__ __launch_bounds__(kBinBS) void k_bin(DevParams P, const double *__res
 
 constexpr int kScanBS = 1024;
 
+// Inclusive prefix sums over the wavefront by DPP: row shifts by 1, 2, 4, 8
+// lanes (within each row of 16), then lane 15 of each row broadcast into
+// the next row and lane 31 into the upper half (no LDS traffic; a lane with
+// no source adds 0).
+__device__ __forceinline__ int wave_incl_scan(int v) {
+  v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xF, 0xF, false);  // row_shr:1
+  v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xF, 0xF, false);  // row_shr:2
+  v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xF, 0xF, false);  // row_shr:4
+  v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xF, 0xF, false);  // row_shr:8
+  v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xA, 0xF, false);  // row_bcast:15
+  v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xC, 0xF, false);  // row_bcast:31
+  return v;
+}
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double dpp_take(double v) {
+  const unsigned long long u = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)u, CTRL, ROWS, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(u >> 32), CTRL, ROWS, 0xF, false);
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+__device__ __forceinline__ double wave_incl_scan(double v) {
+  v += dpp_take<0x111, 0xF>(v);
+  v += dpp_take<0x112, 0xF>(v);
+  v += dpp_take<0x114, 0xF>(v);
+  v += dpp_take<0x118, 0xF>(v);
+  v += dpp_take<0x142, 0xA>(v);
+  v += dpp_take<0x143, 0xC>(v);
+  return v;
+}
+
 // Exclusive scan of the cell counts (start[0..n]) by one workgroup, in
 // passes of kScanTiles tiles of 4 x kScanBS counts: thread t loads counts
 // 4t .. 4t + 3 of every tile of the pass (coalesced, all loads in flight
-// together), the per-thread sums are scanned per tile across the workgroup
-// (wavefront shuffles, then the wavefront totals), and the prefixes are
-// written back four at a time.  The counts are cleared as they are read,
-// ready for the next evaluation's binning, and so is the overflow count
-// (moved to ovfn[1] for k_place).  Also reduces the per-block charge sums
-// of the binning pass.
+// together; cnt and start are padded to whole int4 past n, the padding of
+// cnt zero), the per-thread sums are scanned per tile across each
+// wavefront, every wavefront scans the wavefront totals of each tile
+// itself (one barrier per pass), and the prefixes are written back four at
+// a time.  The counts are cleared as they are read, ready for the next
+// evaluation's binning, and so is the overflow count (moved to ovfn[1] for
+// k_place).  Also sums the per-block charge partials of the binning pass,
+// in a fixed order.
 constexpr int kScanTiles = 8;
 __global__ __launch_bounds__(kScanBS) void k_scan(int n, int *__restrict__ cnt,
                                                   int *__restrict__ start,
@@ -225,6 +257,7 @@ __global__ __launch_bounds__(kScanBS) void k_scan(int n, int *__restrict__ cnt,
                                                   int *__restrict__ ovfn) {
   constexpr int NW = kScanBS / 64, TILE = 4 * kScanBS;
   __shared__ int wtot[kScanTiles][NW];
+  __shared__ double wq[2][NW];
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   double q1 = 0, q2 = 0;
   for (int k = t; k < nq; k += kScanBS) q1 += qpart[2 * k], q2 += qpart[2 * k + 1];
@@ -235,62 +268,48 @@ __global__ __launch_bounds__(kScanBS) void k_scan(int n, int *__restrict__ cnt,
 #pragma unroll
     for (int j = 0; j < kScanTiles; ++j) {
       const int i = base + j * TILE + 4 * t;
-      if (i + 3 < n) {
-        v[j] = *reinterpret_cast<const int4 *>(cnt + i);
-        *reinterpret_cast<int4 *>(cnt + i) = make_int4(0, 0, 0, 0);
-      } else {
-        v[j].x = i < n ? cnt[i] : 0;
-        v[j].y = i + 1 < n ? cnt[i + 1] : 0;
-        v[j].z = i + 2 < n ? cnt[i + 2] : 0;
-        v[j].w = 0;
-        for (int k = i; k < min(i + 3, n); ++k) cnt[k] = 0;
-      }
+      v[j] = i < n ? *reinterpret_cast<const int4 *>(cnt + i) : make_int4(0, 0, 0, 0);
       sum[j] = v[j].x + v[j].y + v[j].z + v[j].w;
-      incl[j] = sum[j];
     }
-    for (int off = 1; off < 64; off <<= 1)
 #pragma unroll
-      for (int j = 0; j < kScanTiles; ++j) {
-        const int o = __shfl_up(incl[j], off, 64);
-        if (lane >= off) incl[j] += o;
-      }
+    for (int j = 0; j < kScanTiles; ++j) {
+      const int i = base + j * TILE + 4 * t;
+      if (i < n) *reinterpret_cast<int4 *>(cnt + i) = make_int4(0, 0, 0, 0);
+      incl[j] = wave_incl_scan(sum[j]);
+    }
     if (lane == 63)
 #pragma unroll
       for (int j = 0; j < kScanTiles; ++j) wtot[j][w] = incl[j];
     __syncthreads();
 #pragma unroll
     for (int j = 0; j < kScanTiles; ++j) {
-      int woff = 0, tile = 0;
-#pragma unroll
-      for (int k = 0; k < NW; ++k) {
-        const int x = wtot[j][k];
-        woff += k < w ? x : 0;
-        tile += x;
-      }
+      // this wavefront's offset in the tile and the tile's total
+      const int wi = wave_incl_scan(lane < NW ? wtot[j][lane] : 0);
+      const int woff = w > 0 ? __builtin_amdgcn_readlane(wi, w - 1) : 0;
+      const int tile = __builtin_amdgcn_readlane(wi, NW - 1);
       const int p = carry + woff + incl[j] - sum[j];
-      const int4 o = make_int4(p, p + v[j].x, p + v[j].x + v[j].y, p + v[j].x + v[j].y + v[j].z);
       const int i = base + j * TILE + 4 * t;
-      if (i + 3 < n) {
-        *reinterpret_cast<int4 *>(start + i) = o;
-      } else {
-        if (i < n) start[i] = o.x;
-        if (i + 1 < n) start[i + 1] = o.y;
-        if (i + 2 < n) start[i + 2] = o.z;
-      }
+      if (i < n)
+        *reinterpret_cast<int4 *>(start + i) =
+            make_int4(p, p + v[j].x, p + v[j].x + v[j].y, p + v[j].x + v[j].y + v[j].z);
       carry += tile;  // the same value in every thread
     }
     __syncthreads();  // wtot is rewritten by the next pass
   }
+  // charge sums for the constant energy terms: wavefront scans, then the
+  // wavefront totals in order
+  q1 = wave_incl_scan(q1);
+  q2 = wave_incl_scan(q2);
+  if (lane == 63) wq[0][w] = q1, wq[1][w] = q2;
+  __syncthreads();
   if (t == 0) {
-    start[n] = carry;
+    start[n] = carry;  // after the barriers: the padded int4 stores above may cover it
     ovfn[1] = ovfn[0];
     ovfn[0] = 0;
+    double a = 0, c2 = 0;
+    for (int k = 0; k < NW; ++k) a += wq[0][k], c2 += wq[1][k];
+    scal[0] = a, scal[1] = c2;
   }
-  // charge sums for the constant energy terms
-  const double a = block_sum<kScanBS>(q1);
-  __syncthreads();
-  const double c2 = block_sum<kScanBS>(q2);
-  if (t == 0) scal[0] = a, scal[1] = c2;
 }
 
 // Placement of a sorted atom on the level-1 grid for the spreading

@@ -372,6 +372,42 @@ def test_pair_range_table_overflow_dense(oracle_mod, torch_cuda):
     assert rel_err(Em, Eo) < 1e-5
 
 
+def test_binning_bucket_overflow_clusters(oracle_mod, torch_cuda):
+    """3,000 random charges in a 30 A box plus two clusters of 300 and 200
+    charges in balls of 1.5 A: cells of a0/3 = 3 A hold 3.5 atoms on average,
+    their bucket rows 32, so the two cluster cells overflow their rows and
+    their atoms are ranked through the overflow list (both clusters' atoms
+    on one list).  FP64 against the oracle; the sorted order is that of every
+    other case (by original index inside each cell)."""
+    print("[parity] clusters3500", flush=True)
+    r = np.random.default_rng(21)
+    box = np.array([30.0, 30.0, 30.0])
+    parts = [r.uniform(0, 1, (3000, 3)) * box]
+    for centre, n in (((7.5, 7.5, 7.5), 300), ((22.5, 16.5, 10.5), 200)):
+        v = r.normal(size=(n, 3))
+        v *= (1.5 * r.uniform(0, 1, n) ** (1 / 3) / np.linalg.norm(v, axis=1))[:, None]
+        parts.append(np.asarray(centre) + v)
+    pos = np.concatenate(parts)
+    perm = r.permutation(len(pos))  # cluster atoms spread over the index range
+    pos = pos[perm]
+    q = r.uniform(-1, 1, len(pos))
+    q -= q.mean()
+    d = pos[:, None, :] - pos[None, :, :]
+    assert np.sum(d * d, axis=2)[np.triu_indices(len(pos), 1)].min() > 1e-6
+    s = inputs.System("clusters3500", pos, q, np.diag(box), None,
+                      dict(nu=6, levels=2, top_grid=6, cutoff=9.0))
+    m, o = _pair(s, oracle_mod)
+    try:
+        E, F, phi = m.energy(s.pos, potential=True)
+        Eo, Fo, phio, _ = o.energy(s.pos)
+        assert rel_err(E, Eo) < TOL, (E, Eo)
+        assert force_err(F, Fo, force_scale(s.q, s.edges)) < TOL
+        assert np.max(np.abs(phi - phio)) / np.sqrt(np.mean(phio ** 2)) < TOL
+        assert m.pair_count == o.pair_count()
+    finally:
+        m.close()
+
+
 def _dft_pair(name, layout, oracle_mod):
     """Systems for the DFT build (FF_set_FFT(false)): automatic levels, grid
     and cutoff at a fixed order, and explicit grids that are not 7-smooth
