@@ -22,7 +22,8 @@ of A^-1, forcefield0.c:618-620) drops images in the sheared cell: the as-is
 oracle misses the Madelung energy by far more than the estimate at L >= 2.
 The GPU (marked gpu): MSM and PME energies, forces and potentials against
 the oracles in skewed boxes (FP64 to 1e-10, mixed to 1e-5; MSM also with a
-cutoff beyond half the smallest width), the Madelung energy on the GPU,
+cutoff beyond half the smallest width), the Madelung energy on the GPU (PME
+at order 10; MSM at order 8, L = 1 and 2, within its error estimate),
 pair counts, and FF_rebuild into a changed skewed box.
 """
 import ctypes
@@ -292,6 +293,35 @@ def test_msm_madelung_sheared_cell_on_gpu(n, nu, L, M, frac, oracle_mod, torch_c
     m = MSM(pos, q, A, layout="asis", **kw)
     try:
         assert rel_err(m.energy(pos)[0], Eao) < TOL
+    finally:
+        m.close()
+
+
+# order 8 (the order-10 multilevel case above stays off the GPU); the last
+# cutoff beyond half the smallest width
+MSM_MADELUNG_NU8 = [(4, 8, 1, 48, 0.49), (4, 8, 2, 12, 0.49), (4, 8, 2, 12, 0.8)]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n,nu,L,M,frac", MSM_MADELUNG_NU8)
+def test_msm_madelung_sheared_cell_nu8_on_gpu(n, nu, L, M, frac, oracle_mod, torch_cuda):
+    """An absolute pin of the GPU's triclinic MSM energy: the CsCl crystal
+    written in the sheared cell, FP64 energy against the oracle to 1e-10
+    and against the exact lattice sum within the a-priori estimate
+    deltaF/Fref (observed 6e-8 - 8e-6 against 5e-6 - 8e-4 on the oracle),
+    zero forces and potentials -q alpha/d likewise."""
+    from msmpb_amd import MSM
+    pos, q, A = _cscl_sheared(U2, n)
+    a0 = frac * _widths(A).min()
+    print(f"[triclinic] MSM Madelung nu={nu} L={L} cutoff/width={frac}", flush=True)
+    o = oracle_mod.Oracle(pos, q, A, nu=nu, L=L, Mtop=M, a0=a0)
+    Eo = o.energy(pos, want_phi=False)[0]
+    rel_est = o.deltaF() / force_scale(q, A)
+    m = MSM(pos, q, A, nu=nu, levels=L, top_grid=M, cutoff=a0)
+    try:
+        E, F, phi = m.energy(pos, potential=True)
+        assert rel_err(E, Eo) < TOL, (E, Eo)
+        _madelung_check(E, F, phi, q, A, rel_est)
     finally:
         m.close()
 
