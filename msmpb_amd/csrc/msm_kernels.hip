@@ -756,12 +756,8 @@ __global__ __launch_bounds__(64) void k_pair_cell(DevParams P, const double4 *__
           }
           // non-empty ranges only, in column order: (ranges << 24) + atoms
           const int mine = ((int)(cnt0 > 0) + (int)(cnt1 > 0)) * (1 << 24) + cnt0 + cnt1;
-          int incl = mine;
-          for (int off = 1; off < 64; off <<= 1) {
-            const int t = __shfl_up(incl, off, 64);
-            if (lane >= off) incl += t;
-          }
-          const int excl = incl - mine, last = __shfl(incl, 63, 64);
+          const int incl = wave_incl_scan(mine);
+          const int excl = incl - mine, last = __builtin_amdgcn_readlane(incl, 63);
           const int kfirst = excl >> 24;
           int k = kfirst;
           const int o = excl & 0xffffff;
@@ -854,8 +850,8 @@ __global__ __launch_bounds__(64) void k_pair_cell(DevParams P, const double4 *__
       }
     }
   }
-  for (int off = 32; off > 0; off >>= 1) esum += __shfl_down(esum, off, 64);
-  if (lane == 0) epart[c] = esum;
+  esum = wave_incl_scan(esum);  // lane 63: the wavefront total
+  if (lane == 63) epart[c] = esum;
 }
 
 // Pairs i < j (cell-sorted order) whose nearest image lies within radius
@@ -919,12 +915,8 @@ __global__ __launch_bounds__(64) void k_pair_count(DevParams P, const double4 *_
         }
       }
       const int mine = ((int)(cnt0 > 0) + (int)(cnt1 > 0)) * (1 << 24) + cnt0 + cnt1;
-      int incl = mine;
-      for (int off = 1; off < 64; off <<= 1) {
-        const int t = __shfl_up(incl, off, 64);
-        if (lane >= off) incl += t;
-      }
-      const int excl = incl - mine, last = __shfl(incl, 63, 64);
+      const int incl = wave_incl_scan(mine);
+      const int excl = incl - mine, last = __builtin_amdgcn_readlane(incl, 63);
       int k = excl >> 24;
       const int o = excl & 0xffffff;
       if (cnt0 > 0) {
@@ -1181,11 +1173,7 @@ __global__ __launch_bounds__(64 * WPB) void k_spread_mfma(DevParams P, GridDims 
         run += k < nr ? roff[k + 1] : 0;
         loc[u] = run;
       }
-      int incl = run;
-      for (int off = 1; off < 64; off <<= 1) {
-        int t = __shfl_up(incl, off, 64);
-        if (lane >= off) incl += t;
-      }
+      const int incl = wave_incl_scan(run);
       const int excl = incl - run;
 #pragma unroll
       for (int u = 0; u < 8; ++u) {
@@ -1193,7 +1181,7 @@ __global__ __launch_bounds__(64 * WPB) void k_spread_mfma(DevParams P, GridDims 
         if (k < nr) roff[k + 1] = excl + loc[u];
       }
       if (lane == 0) roff[0] = 0;
-      const int tot = __shfl(incl, 63, 64);
+      const int tot = __builtin_amdgcn_readlane(incl, 63);
       if (lane < kSJump) roff[nr + 1 + lane] = tot;  // the jumps of spread_index stop here
     }
     __syncthreads();
@@ -1384,11 +1372,7 @@ __global__ __launch_bounds__(64 * WPB) void k_spread_bricks(DevParams P, GridDim
         run += k < nr ? roff[k + 1] : 0;
         loc[u] = run;
       }
-      int incl = run;
-      for (int off = 1; off < 64; off <<= 1) {
-        int t = __shfl_up(incl, off, 64);
-        if (lane >= off) incl += t;
-      }
+      const int incl = wave_incl_scan(run);
       const int excl = incl - run;
 #pragma unroll
       for (int u = 0; u < 8; ++u) {
@@ -1396,7 +1380,7 @@ __global__ __launch_bounds__(64 * WPB) void k_spread_bricks(DevParams P, GridDim
         if (k < nr) roff[k + 1] = excl + loc[u];
       }
       if (lane == 0) roff[0] = 0;
-      const int tot = __shfl(incl, 63, 64);
+      const int tot = __builtin_amdgcn_readlane(incl, 63);
       if (lane < kSJump) roff[nr + 1 + lane] = tot;  // the jumps of spread_index stop here
     }
     __syncthreads();
@@ -2320,11 +2304,7 @@ __global__ __launch_bounds__(256) void k_gather_tile(DevParams P, GridDims g, Ti
       beg = start[base + c0[2]];
       cnt = start[base + c1[2]] - beg;
     }
-    int incl = cnt;
-    for (int off = 1; off < 64; off <<= 1) {
-      const int v = __shfl_up(incl, off, 64);
-      if (t >= off) incl += v;
-    }
+    const int incl = wave_incl_scan(cnt);
     if (t < ncol) {
       seg_off[t] = incl - cnt;
       seg_beg[t] = beg;
