@@ -703,6 +703,13 @@ __global__ __launch_bounds__(64) void k_pair_cell(DevParams P, const double4 *__
     for (int k = 0; k < kPassAtoms; ++k) s[k][0] = s[k][1] = s[k][2] = s[k][3] = (R)0;
     double tot = 0.0;  // this lane's share of the pass totals
     const bool wide = ni > 4;
+    // lane l will hold value vi = 4 g + d of home atom g: its writer's
+    // charge and original index are fetched now, used after the last run
+    const int vi = wide ? (lane >> 1) & 31 : (lane >> 2) & 15;
+    const bool writer = (lane & (wide ? 1 : 3)) == 0 && (vi >> 2) < ni;
+    const int si = b0 + ip + min(vi >> 2, ni - 1);
+    const double qi = reinterpret_cast<const double *>(spos + si)[3];
+    const int oi = sidx[si];
     int np = 0;        // staged atoms in jbuf
     int c0 = 0, g0 = 0, total = 0, sl = 0;
     bool fresh = true, done = false;
@@ -835,12 +842,8 @@ __global__ __launch_bounds__(64) void k_pair_cell(DevParams P, const double4 *__
       }
     }
     // ---- lane l holds value vi = 4 g + d of home atom g
-    const int vi = wide ? (lane >> 1) & 31 : (lane >> 2) & 15;
-    const bool writer = (lane & (wide ? 1 : 3)) == 0 && (vi >> 2) < ni;
     if (writer) {
-      const int si = b0 + ip + (vi >> 2), d = vi & 3;
-      const double qi = spos[si].w;
-      const int oi = sidx[si];
+      const int d = vi & 3;
       if (d == 0) {
         const double pv = tot - qi * P.self_img;
         if (phi) phi[oi] = pv;
