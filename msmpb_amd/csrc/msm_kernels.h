@@ -11,6 +11,38 @@
 
 namespace msm {
 
+// ---- wavefront scans shared by the kernel files
+// Inclusive prefix sums over the wavefront by DPP: row shifts by 1, 2, 4, 8
+// lanes (within each row of 16), then lane 15 of each row broadcast into
+// the next row and lane 31 into the upper half (no LDS traffic; a lane with
+// no source adds 0).
+__device__ __forceinline__ int wave_incl_scan(int v) {
+  v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xF, 0xF, false);  // row_shr:1
+  v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xF, 0xF, false);  // row_shr:2
+  v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xF, 0xF, false);  // row_shr:4
+  v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xF, 0xF, false);  // row_shr:8
+  v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xA, 0xF, false);  // row_bcast:15
+  v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xC, 0xF, false);  // row_bcast:31
+  return v;
+}
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double dpp_take(double v) {
+  const unsigned long long u = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)u, CTRL, ROWS, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(u >> 32), CTRL, ROWS, 0xF, false);
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+__device__ __forceinline__ double wave_incl_scan(double v) {
+  v += dpp_take<0x111, 0xF>(v);
+  v += dpp_take<0x112, 0xF>(v);
+  v += dpp_take<0x114, 0xF>(v);
+  v += dpp_take<0x118, 0xF>(v);
+  v += dpp_take<0x142, 0xA>(v);
+  v += dpp_take<0x143, 0xC>(v);
+  return v;
+}
+
+
 // The grid chain runs beside the pair kernel and bounds the evaluation: its
 // wavefronts take instruction issue ahead of the pair kernel's on a shared
 // SIMD (wave priority 3; the pair kernel keeps the default 0).

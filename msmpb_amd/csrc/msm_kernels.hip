@@ -33,9 +33,9 @@ __device__ __forceinline__ int floordiv(int v, int m) {
 template <int BS>
 __device__ __forceinline__ double block_sum(double v) {
   __shared__ double red[BS / 64];
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  v = wave_incl_scan(v);  // lane 63: the wavefront total
   int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  if (lane == 0) red[w] = v;
+  if (lane == 63) red[w] = v;
   __syncthreads();
   double t = 0;
   if (threadIdx.x == 0)
@@ -207,36 +207,6 @@ __global__ __launch_bounds__(kBinBS) void k_bin(DevParams P, const double *__res
 }
 
 constexpr int kScanBS = 1024;
-
-// Inclusive prefix sums over the wavefront by DPP: row shifts by 1, 2, 4, 8
-// lanes (within each row of 16), then lane 15 of each row broadcast into
-// the next row and lane 31 into the upper half (no LDS traffic; a lane with
-// no source adds 0).
-__device__ __forceinline__ int wave_incl_scan(int v) {
-  v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xF, 0xF, false);  // row_shr:1
-  v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xF, 0xF, false);  // row_shr:2
-  v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xF, 0xF, false);  // row_shr:4
-  v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xF, 0xF, false);  // row_shr:8
-  v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xA, 0xF, false);  // row_bcast:15
-  v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xC, 0xF, false);  // row_bcast:31
-  return v;
-}
-template <int CTRL, int ROWS>
-__device__ __forceinline__ double dpp_take(double v) {
-  const unsigned long long u = __double_as_longlong(v);
-  const int lo = __builtin_amdgcn_update_dpp(0, (int)u, CTRL, ROWS, 0xF, false);
-  const int hi = __builtin_amdgcn_update_dpp(0, (int)(u >> 32), CTRL, ROWS, 0xF, false);
-  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
-}
-__device__ __forceinline__ double wave_incl_scan(double v) {
-  v += dpp_take<0x111, 0xF>(v);
-  v += dpp_take<0x112, 0xF>(v);
-  v += dpp_take<0x114, 0xF>(v);
-  v += dpp_take<0x118, 0xF>(v);
-  v += dpp_take<0x142, 0xA>(v);
-  v += dpp_take<0x143, 0xC>(v);
-  return v;
-}
 
 // Exclusive scan of the cell counts (start[0..n]) by one workgroup, in
 // passes of kScanTiles tiles of 4 x kScanBS counts: thread t loads counts
@@ -972,8 +942,8 @@ __global__ __launch_bounds__(64) void k_pair_count(DevParams P, const double4 *_
       __syncthreads();  // the tables are rewritten by the next round
     }
   }
-  for (int off = 32; off > 0; off >>= 1) cnt += __shfl_down(cnt, off, 64);
-  if (lane == 0) cell_count[c] = cnt;  // one count per cell: no contended atomic
+  cnt = wave_incl_scan((int)cnt);  // lane 63: the wavefront total
+  if (lane == 63) cell_count[c] = cnt;  // one count per cell: no contended atomic
 }
 
 // ---------------------------------------------------------------- spread
@@ -1948,33 +1918,6 @@ __device__ __forceinline__ void csum_add(double &s, double &c, double s2, double
   c += c2 + err;
 }
 
-// Compensated partial sums (s, c) over the wavefront: a butterfly in a
-// fixed order, then lane 0's copy is used (deterministic run to run).
-__device__ __forceinline__ void wave_csum(double &s, double &c) {
-#pragma unroll
-  for (int m = 1; m < 64; m <<= 1) {
-    const double s2 = __shfl_xor(s, m, 64), c2 = __shfl_xor(c, m, 64);
-    csum_add(s, c, s2, c2);
-  }
-}
-
-// ... and over a workgroup of BS threads: the wavefronts' sums in order;
-// thread 0 receives the total (the others' values are unspecified).
-template <int BS>
-__device__ __forceinline__ void block_csum(double &s, double &c) {
-  constexpr int NW = BS / 64;
-  __shared__ double ws[NW], wc[NW];
-  wave_csum(s, c);
-  const int t = threadIdx.x;
-  if ((t & 63) == 0) ws[t >> 6] = s, wc[t >> 6] = c;
-  __syncthreads();
-  if (t == 0) {
-    s = ws[0], c = wc[0];
-    for (int w = 1; w < NW; ++w) csum_add(s, c, ws[w], wc[w]);
-  }
-  __syncthreads();  // ws, wc free for the next call
-}
-
 // Compensated sums of the pair and the grid partials over the threads of a
 // BS-thread workgroup: thread t takes p[t], p[t + BS], ... of each in that
 // order.  The partials sit in memory written by other kernels, so every
@@ -2000,22 +1943,33 @@ __device__ __forceinline__ void strided_csum2(const double *__restrict__ ep, int
   }
 }
 
-// Two compensated sums (s1, c1), (s2, c2) over a workgroup at once (the
-// butterflies of block_csum interleaved, one barrier); thread 0 receives
-// the totals.
+// One step of a compensated wavefront scan: (s, c) combined with the pair
+// DPP control CTRL brings (rows ROWS only; elsewhere it adds (0, 0), which
+// changes neither).
+template <int CTRL, int ROWS>
+__device__ __forceinline__ void csum_step(double &s1, double &c1, double &s2, double &c2) {
+  const double a = dpp_take<CTRL, ROWS>(s1), b = dpp_take<CTRL, ROWS>(c1);
+  const double d = dpp_take<CTRL, ROWS>(s2), e = dpp_take<CTRL, ROWS>(c2);
+  csum_add(s1, c1, a, b);
+  csum_add(s2, c2, d, e);
+}
+
+// Two compensated sums (s1, c1), (s2, c2) over a workgroup at once: an
+// inclusive scan of each over the wavefront by DPP in a fixed order (lane
+// 63 ends with the wavefront total), then the wavefront totals in order
+// (one barrier); thread 0 receives the totals.
 template <int BS>
 __device__ __forceinline__ void block_csum2(double &s1, double &c1, double &s2, double &c2) {
   constexpr int NW = BS / 64;
   __shared__ double ws[4][NW];
-#pragma unroll
-  for (int m = 1; m < 64; m <<= 1) {
-    const double a = __shfl_xor(s1, m, 64), b = __shfl_xor(c1, m, 64);
-    const double d = __shfl_xor(s2, m, 64), e = __shfl_xor(c2, m, 64);
-    csum_add(s1, c1, a, b);
-    csum_add(s2, c2, d, e);
-  }
+  csum_step<0x111, 0xF>(s1, c1, s2, c2);  // row_shr:1
+  csum_step<0x112, 0xF>(s1, c1, s2, c2);  // row_shr:2
+  csum_step<0x114, 0xF>(s1, c1, s2, c2);  // row_shr:4
+  csum_step<0x118, 0xF>(s1, c1, s2, c2);  // row_shr:8
+  csum_step<0x142, 0xA>(s1, c1, s2, c2);  // row_bcast:15
+  csum_step<0x143, 0xC>(s1, c1, s2, c2);  // row_bcast:31
   const int t = threadIdx.x;
-  if ((t & 63) == 0) ws[0][t >> 6] = s1, ws[1][t >> 6] = c1, ws[2][t >> 6] = s2, ws[3][t >> 6] = c2;
+  if ((t & 63) == 63) ws[0][t >> 6] = s1, ws[1][t >> 6] = c1, ws[2][t >> 6] = s2, ws[3][t >> 6] = c2;
   __syncthreads();
   if (t == 0) {
     s1 = ws[0][0], c1 = ws[1][0], s2 = ws[2][0], c2 = ws[3][0];

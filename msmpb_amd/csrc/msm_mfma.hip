@@ -400,8 +400,8 @@ __global__ __launch_bounds__(256) void k_prolong_sum(DevParams P, GridDims g, Gr
     if (q1) es += q1[id] * acc;
   }
   if (q1) {  // fixed-order block sum of q1 e1
-    for (int off = 32; off > 0; off >>= 1) es += __shfl_down(es, off, 64);
-    if ((tid & 63) == 0) wsum[tid >> 6] = es;
+    es = wave_incl_scan(es);  // lane 63: the wavefront total
+    if ((tid & 63) == 63) wsum[tid >> 6] = es;
     __syncthreads();
     if (tid == 0) epart[blockIdx.x] = ((wsum[0] + wsum[1]) + wsum[2]) + wsum[3];
   }
