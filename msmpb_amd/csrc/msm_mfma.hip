@@ -199,7 +199,12 @@ __global__ __launch_bounds__(64) void k_g2g_tiles(G2GArgs args) {
 // of a tile's 16 are dropped (nz + s_z - 1 fits the chunks).  The four
 // partial tiles are summed through LDS in a fixed order.
 constexpr int kRowWaves = 1;
-constexpr int kSlabU = 16;  // slab elements per thread and batch (loads in flight)
+constexpr int kSlabRows = 8;  // slab rows per batch: kSlabRows x kSlabCols loads in flight
+// slab columns per thread: the widest slab, (kG2GMaxTpw - 1) 16 + 4 x 32
+// columns (tile width <= 16, at most four K chunks), fits 64 kRowWaves
+// kSlabCols
+constexpr int kSlabCols = 3;
+static_assert((kG2GMaxTpw - 1) * 16 + 4 * 32 <= 64 * kRowWaves * kSlabCols, "slab columns");
 
 template <int NCH, int T>
 __global__ __launch_bounds__(64 * kRowWaves) void k_g2g_rows(G2GArgs args) {
@@ -229,8 +234,15 @@ __global__ __launch_bounds__(64 * kRowWaves) void k_g2g_rows(G2GArgs args) {
   uint16_t *const shi = reinterpret_cast<uint16_t *>(lds_raw);
   uint16_t *const slo = shi + R * Wp;
   const int yb = y0 - s.y / 2, zb = z0 - s.z / 2;
-  const float igy = 1.0f / g.y, igz = 1.0f / g.z, iW = 1.0f / W;
+  const float igy = 1.0f / g.y, igz = 1.0f / g.z;
   auto wrapf = [](int v, int m, float im) { return v - m * (int)floorf((v + 0.5f) * im); };
+  // slab columns: lane l takes columns l + 64 j (the same for every kx and
+  // row), their wrapped z index computed once; rows go kSlabRows at a time,
+  // the row's wrapped y offset uniform over the wavefront
+  const int nj = (W + BS - 1) / BS;  // <= kSlabCols (checked at launch)
+  int zo[kSlabCols];
+#pragma unroll
+  for (int j = 0; j < kSlabCols; ++j) zo[j] = wrapf(zb + min(tid + BS * j, W - 1), g.z, igz);
   f32x4 acc[T];
 #pragma unroll
   for (int tt = 0; tt < T; ++tt) acc[tt] = f32x4{0.f, 0.f, 0.f, 0.f};
@@ -238,22 +250,25 @@ __global__ __launch_bounds__(64 * kRowWaves) void k_g2g_rows(G2GArgs args) {
     const int xi = wrapm(x + kx - s.x / 2, g.x);
     const unsigned *plane = L.qb + (long)xi * g.y * g.z;
     __syncthreads();  // previous slab consumed
-    const int nel = R * W;
-    for (int e0 = tid; e0 < nel; e0 += BS * kSlabU) {
-      unsigned v[kSlabU];
-      int at[kSlabU];
+    for (int r0 = 0; r0 < R; r0 += kSlabRows) {
+      unsigned v[kSlabRows][kSlabCols];
 #pragma unroll
-      for (int u = 0; u < kSlabU; ++u) {
-        const int e = min(e0 + BS * u, nel - 1);  // past the end: a valid address, not stored
-        const int r = (int)((e + 0.5f) * iW), cc = e - r * W;  // e / W (exact: e < 2^20)
-        at[u] = r * Wp + cc;
-        v[u] = plane[(long)wrapf(yb + r, g.y, igy) * g.z + wrapf(zb + cc, g.z, igz)];
+      for (int u = 0; u < kSlabRows; ++u) {
+        const int yo = wrapf(yb + min(r0 + u, R - 1), g.y, igy) * g.z;  // past the end: valid, not stored
+#pragma unroll
+        for (int j = 0; j < kSlabCols; ++j)
+          if (j < nj) v[u][j] = plane[yo + zo[j]];
       }
 #pragma unroll
-      for (int u = 0; u < kSlabU; ++u) {
-        if (e0 + BS * u < nel) {
-          shi[at[u]] = (uint16_t)(v[u] & 0xFFFFu);
-          slo[at[u]] = (uint16_t)(v[u] >> 16);
+      for (int u = 0; u < kSlabRows; ++u) {
+#pragma unroll
+        for (int j = 0; j < kSlabCols; ++j) {
+          const int cc = tid + BS * j;
+          if (j < nj && r0 + u < R && cc < W) {
+            const int at = (r0 + u) * Wp + cc;
+            shi[at] = (uint16_t)(v[u][j] & 0xFFFFu);
+            slo[at] = (uint16_t)(v[u][j] >> 16);
+          }
         }
       }
     }
