@@ -103,7 +103,7 @@ struct Engine {
   int bcap = 0;
   double *d_Fp = nullptr, *d_phip = nullptr;  // short-range parts of F and phi
   int *d_flags = nullptr;
-  double4 *d_wpos = nullptr, *d_spos = nullptr;
+  double4 *d_spos = nullptr;
   // per-atom spreading placements (packed support origin; t and q), sorted order
   unsigned *d_sbase = nullptr;
   double4 *d_st = nullptr;
@@ -136,7 +136,7 @@ struct Engine {
   void release() {
     T.clear();
     void *ptrs[] = {d_pos,  d_q,    d_F,    d_phi,   d_out,  d_qpart, d_scal,  d_epair,
-                    d_egrid, d_cell_of, d_cnt, d_start, d_sidx, d_flags, d_wpos,
+                    d_egrid, d_cell_of, d_cnt, d_start, d_sidx, d_flags,
                     d_bucket, d_ovf, d_ovfn, d_Fp, d_phip,
                     d_spos, d_sten, d_count, d_qsplit, d_tw, d_topwork, d_stab, d_sscr,
                     d_sbase, d_st};
@@ -164,7 +164,7 @@ struct Engine {
     d_Fp = d_phip = nullptr;
     d_sbase = nullptr;
     d_st = nullptr;
-    d_wpos = d_spos = nullptr;
+    d_spos = nullptr;
     d_sten = nullptr;
     d_count = nullptr;
     d_qsplit = nullptr;
@@ -625,7 +625,6 @@ int create_engine(FFX *x, const msm::Plan &plan) {
   HIP_TRY(hipMemset(E->d_cnt, 0, ncell4 * sizeof(int)));
   HIP_TRY(hipMemset(E->d_flags, 0, 4 * sizeof(int)));
   HIP_TRY(hipMemset(E->d_ovfn, 0, 2 * sizeof(int)));
-  TRY(dalloc(&E->d_wpos, N));
   TRY(dalloc(&E->d_spos, N));
   TRY(dalloc(&E->d_count, E->P.ncell));
   if (msm::spread_by_placements(E->P, E->gd[1])) {
@@ -653,12 +652,12 @@ void stage_prepare(Engine &E, const double *pos, const double *q, hipStream_t s,
   // d_cnt and d_ovfn[0] are zero here: set at allocation, cleared again by
   // every scan
   msm::launch_bin(E.P, pos, q, E.d_cell_of, E.d_bucket, E.bcap, E.d_ovfn, E.d_ovf, E.d_cnt,
-                  E.d_wpos, E.d_qpart, E.d_flags, s);
+                  E.d_qpart, E.d_flags, s);
   msm::launch_scan(E.P, E.d_cnt, E.d_start, E.d_qpart, msm::bin_blocks(E.P.N), E.d_scal,
                    E.d_ovfn, s);
   // the placement pass also writes the atoms' spreading placements
   const bool place = msm::spread_by_placements(E.P, E.gd[1]);
-  msm::launch_place(E.P, E.d_cell_of, E.d_wpos, E.d_start, E.d_bucket, E.bcap, E.d_ovfn,
+  msm::launch_place(E.P, pos, q, E.d_cell_of, E.d_start, E.d_bucket, E.bcap, E.d_ovfn,
                     E.d_ovf, E.d_spos, E.d_sidx, s, fork, E.gd[1],
                     place ? E.d_sbase : nullptr, place ? E.d_st : nullptr);
 }

@@ -126,21 +126,21 @@ void launch_fill_dbl(double *p, long n, double v, hipStream_t s);
 // inputs counted in flags[0].
 int bin_blocks(int N);
 void launch_bin(const DevParams &P, const double *pos, const double *q, int *cell_of, int *bucket,
-                int cap, int *ovfn, int *ovf, int *cnt, double4 *wpos, double *qpart, int *flags,
-                hipStream_t s);
+                int cap, int *ovfn, int *ovf, int *cnt, double *qpart, int *flags, hipStream_t s);
 // Exclusive scan of cnt into start[0..ncell], cnt and ovfn[0] cleared for
 // the next binning (the overflow count moved to ovfn[1]), and reduction of
 // qpart into scal[0] = sum q, scal[1] = sum q^2.
 void launch_scan(const DevParams &P, int *cnt, int *start, const double *qpart, int nqpart,
                  double *scal, int *ovfn, hipStream_t s);
-// The cell-sorted order: sorted positions and charges (spos) and original
-// indices (sidx), atoms ordered inside each cell by original index
-// (deterministic results).  With sbase, st (N entries each) it also writes
-// every atom's spreading placement on the grid g1 (as the placement pass of
+// The cell-sorted order: sorted wrapped positions and charges (spos) and
+// original indices (sidx), atoms ordered inside each cell by original index
+// (deterministic results); pos, q are the binning's inputs (the wrap is
+// evaluated again).  With sbase, st (N entries each) it also writes every
+// atom's spreading placement on the grid g1 (as the placement pass of
 // launch_spread does).
-void launch_place(const DevParams &P, const int *cell_of, const double4 *wpos, const int *start,
-                  const int *bucket, int cap, const int *ovfn, const int *ovf, double4 *spos,
-                  int *sidx, hipStream_t s, hipEvent_t done = nullptr,
+void launch_place(const DevParams &P, const double *pos, const double *q, const int *cell_of,
+                  const int *start, const int *bucket, int cap, const int *ovfn, const int *ovf,
+                  double4 *spos, int *sidx, hipStream_t s, hipEvent_t done = nullptr,
                   GridDims g1 = GridDims{0, 0, 0}, unsigned *sbase = nullptr,
                   double4 *st = nullptr);
 

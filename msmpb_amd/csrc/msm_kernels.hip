@@ -149,50 +149,63 @@ __device__ __forceinline__ void lattice_rt(const DevParams &P, int kx, int ky, i
 // counting atomic gives the atom's arrival rank r in its cell, and the atom
 // is recorded in the cell's bucket row (cap entries) at r, or, past the row,
 // on the overflow list (ovfn[0] entries).  Also the per-workgroup charge sums.
+// Atom i's cell and wrapped position with its charge (w); false for a
+// non-finite position or charge (then cell 0 and a zero position: no index
+// is computed from it).  The binning and the placement pass both evaluate
+// it, with the same result.
+__device__ __forceinline__ bool bin_atom(const DevParams &P, const double *__restrict__ pos,
+                                         const double *__restrict__ q, int i, int &cell,
+                                         double4 &w) {
+  const double r[3] = {pos[3 * (long)i], pos[3 * (long)i + 1], pos[3 * (long)i + 2]};
+  const double qi = q[i];
+  int c[3];
+  double rw[3], sf[3], fl[3];
+  bool ok = isfinite(qi);
+  to_frac_rt(P, r, sf);
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    double s = sf[a];
+    double f = floor(s);
+    double sw = s - f;
+    fl[a] = f;
+    ok = ok && isfinite(s);
+    // s - floor(s) may round up to 1 for tiny negative s; keep in range
+    int ci = (int)(sw * P.nc[a]);
+    if (!(sw >= 0.0) || ci < 0) ci = 0;
+    if (ci >= P.nc[a]) ci = P.nc[a] - 1;
+    c[a] = ci;
+  }
+  if (P.tri) {  // r - A floor(A^-1 r)
+    rw[0] = r[0] - (P.A[0] * fl[0] + P.A[1] * fl[1] + P.A[2] * fl[2]);
+    rw[1] = r[1] - (P.A[3] * fl[0] + P.A[4] * fl[1] + P.A[5] * fl[2]);
+    rw[2] = r[2] - (P.A[6] * fl[0] + P.A[7] * fl[1] + P.A[8] * fl[2]);
+  } else {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) rw[a] = r[a] - fl[a] * P.box[a];
+  }
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+    if (!isfinite(rw[a])) rw[a] = 0.0;
+  cell = (c[0] * P.nc[1] + c[1]) * P.nc[2] + c[2];
+  w = make_double4(rw[0], rw[1], rw[2], qi);
+  return ok;
+}
+
 __global__ __launch_bounds__(kBinBS) void k_bin(DevParams P, const double *__restrict__ pos,
                                                 const double *__restrict__ q,
                                                 int *__restrict__ cell_of, int *__restrict__ bucket,
                                                 int cap, int *__restrict__ ovfn,
                                                 int *__restrict__ ovf,
-                                                int *__restrict__ cnt, double4 *__restrict__ wpos,
+                                                int *__restrict__ cnt,
                                                 double *__restrict__ qpart, int *__restrict__ flags) {
   int i = blockIdx.x * kBinBS + threadIdx.x;
   double qi = 0.0;
   if (i < P.N) {
-    double r[3] = {pos[3 * (long)i], pos[3 * (long)i + 1], pos[3 * (long)i + 2]};
-    qi = q[i];
-    int c[3];
-    double rw[3], sf[3], fl[3];
-    bool ok = isfinite(qi);
-    to_frac_rt(P, r, sf);
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      double s = sf[a];
-      double f = floor(s);
-      double sw = s - f;
-      fl[a] = f;
-      ok = ok && isfinite(s);
-      // s - floor(s) may round up to 1 for tiny negative s; keep in range
-      int ci = (int)(sw * P.nc[a]);
-      if (!(sw >= 0.0) || ci < 0) ci = 0;
-      if (ci >= P.nc[a]) ci = P.nc[a] - 1;
-      c[a] = ci;
-    }
-    if (P.tri) {  // r - A floor(A^-1 r)
-      rw[0] = r[0] - (P.A[0] * fl[0] + P.A[1] * fl[1] + P.A[2] * fl[2]);
-      rw[1] = r[1] - (P.A[3] * fl[0] + P.A[4] * fl[1] + P.A[5] * fl[2]);
-      rw[2] = r[2] - (P.A[6] * fl[0] + P.A[7] * fl[1] + P.A[8] * fl[2]);
-    } else {
-#pragma unroll
-      for (int a = 0; a < 3; ++a) rw[a] = r[a] - fl[a] * P.box[a];
-    }
-#pragma unroll
-    for (int a = 0; a < 3; ++a)
-      if (!isfinite(rw[a])) rw[a] = 0.0;
-    if (!ok) atomicAdd(flags, 1);
-    int cell = (c[0] * P.nc[1] + c[1]) * P.nc[2] + c[2];
+    int cell;
+    double4 w;
+    if (!bin_atom(P, pos, q, i, cell, w)) atomicAdd(flags, 1);
+    qi = w.w;
     cell_of[i] = cell;
-    wpos[i] = make_double4(rw[0], rw[1], rw[2], qi);
     const int rk = atomicAdd(cnt + cell, 1);  // arrival order; k_place fixes the order
     if (rk < cap) bucket[(long)cell * cap + rk] = i;
     else ovf[atomicAdd(ovfn, 1)] = i;
@@ -319,8 +332,9 @@ __device__ __forceinline__ void place_atom(const DevParams &P, GridDims g, const
 // CUs.
 constexpr int kPlaceAhead = 8;  // bucket entries in flight per thread
 template <bool TRI>
-__global__ __launch_bounds__(256) void k_place(DevParams P, const int *__restrict__ cell_of,
-                                               const double4 *__restrict__ wpos,
+__global__ __launch_bounds__(256) void k_place(DevParams P, const double *__restrict__ pos,
+                                               const double *__restrict__ q,
+                                               const int *__restrict__ cell_of,
                                                const int *__restrict__ start,
                                                const int *__restrict__ bucket, int cap,
                                                const int *__restrict__ ovfn,
@@ -330,8 +344,9 @@ __global__ __launch_bounds__(256) void k_place(DevParams P, const int *__restric
                                                double4 *__restrict__ st) {
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= P.N) return;
-  const double4 v = wpos[i];
-  const int c = cell_of[i];
+  int c;
+  double4 v;
+  bin_atom(P, pos, q, i, c, v);  // as the binning found it
   const int *row = bucket + (long)c * cap;
   const int b = start[c], n = start[c + 1] - b;
   int rank = 0;
@@ -2313,10 +2328,9 @@ void launch_fill_dbl(double *p, long n, double v, hipStream_t s) {
 int bin_blocks(int N) { return (int)nblk(N, kBinBS); }
 
 void launch_bin(const DevParams &P, const double *pos, const double *q, int *cell_of, int *bucket,
-                int cap, int *ovfn, int *ovf, int *cnt, double4 *wpos, double *qpart, int *flags,
-                hipStream_t s) {
-  k_bin<<<bin_blocks(P.N), kBinBS, 0, s>>>(P, pos, q, cell_of, bucket, cap, ovfn, ovf, cnt, wpos,
-                                           qpart, flags);
+                int cap, int *ovfn, int *ovf, int *cnt, double *qpart, int *flags, hipStream_t s) {
+  k_bin<<<bin_blocks(P.N), kBinBS, 0, s>>>(P, pos, q, cell_of, bucket, cap, ovfn, ovf, cnt, qpart,
+                                           flags);
 }
 
 void launch_scan(const DevParams &P, int *cnt, int *start, const double *qpart, int nqpart,
@@ -2324,11 +2338,11 @@ void launch_scan(const DevParams &P, int *cnt, int *start, const double *qpart, 
   k_scan<<<1, kScanBS, 0, s>>>(P.ncell, cnt, start, qpart, nqpart, scal, ovfn);
 }
 
-void launch_place(const DevParams &P, const int *cell_of, const double4 *wpos, const int *start,
-                  const int *bucket, int cap, const int *ovfn, const int *ovf, double4 *spos,
-                  int *sidx, hipStream_t s, hipEvent_t done, GridDims g1, unsigned *sbase,
-                  double4 *st) {
-#define A P, cell_of, wpos, start, bucket, cap, ovfn, ovf, spos, sidx, g1, sbase, st
+void launch_place(const DevParams &P, const double *pos, const double *q, const int *cell_of,
+                  const int *start, const int *bucket, int cap, const int *ovfn, const int *ovf,
+                  double4 *spos, int *sidx, hipStream_t s, hipEvent_t done, GridDims g1,
+                  unsigned *sbase, double4 *st) {
+#define A P, pos, q, cell_of, start, bucket, cap, ovfn, ovf, spos, sidx, g1, sbase, st
   if (P.tri) launch_done(k_place<true>, nblk(P.N, 256), 256, 0, s, done, A);
   else launch_done(k_place<false>, nblk(P.N, 256), 256, 0, s, done, A);
 #undef A
