@@ -736,14 +736,23 @@ int stage_grids(Engine &E, int precision, int layout, Timing &T, hipStream_t s) 
     msm::launch_g2g_tiles(pr, ac, g2g_launch_lds(pr, ac), s);
     T.close(t, "grid2grid", s);
   }
-  t = T.open(s);
   const int top_map = layout != MSM_LAYOUT_ASIS ? 0
                       : L >= 2                  ? (E.plan.fft ? msm::kTopAsisStride : 0)
                                                 : (E.plan.fft ? 0 : msm::kTopAsisFlat);
-  msm::launch_top(E.P, E.gd[L], E.d_ql[L], E.d_khat[L], E.d_tw, E.d_topwork, E.d_el[L], top_map,
-                  s);
-  T.close(t, "grid2grid_top", s);
-  for (int l = L - 1; l >= 1; --l) {
+  // a small top level rides on the prolongation onto level L-1 (>= 2)
+  const bool fused_top = L >= 3 && msm::prolong_top_fits(E.gd[L - 1], E.gd[L]);
+  if (fused_top) {
+    t = T.open(s);
+    msm::launch_prolong_top(E.P, E.gd[L - 1], E.gd[L], E.g2g[pr].lv[L - 2].G, E.d_part[pr][L - 1],
+                            E.d_ql[L], E.d_khat[L], E.d_tw, E.d_el[L], E.d_el[L - 1], top_map, s);
+    T.close(t, "grid2grid_top", s);
+  } else {
+    t = T.open(s);
+    msm::launch_top(E.P, E.gd[L], E.d_ql[L], E.d_khat[L], E.d_tw, E.d_topwork, E.d_el[L],
+                    top_map, s);
+    T.close(t, "grid2grid_top", s);
+  }
+  for (int l = fused_top ? L - 2 : L - 1; l >= 1; --l) {
     if (l == 1) HIP_TRY(hipStreamWaitEvent(s, E.ev_g1, 0));
     t = T.open(s);
     // the finest level also writes the grid-energy partials (sum q1 e1)

@@ -182,6 +182,65 @@ void launch_restrict(const DevParams &P, GridDims fine, GridDims coarse, const d
 // (kTopAsisStride) or the DFT build's flat read of the padded level-1 array
 // at L = 1 (kTopAsisFlat).
 constexpr int kTopAsisStride = 1, kTopAsisFlat = 2;
+
+// ---- top level in one workgroup (shared by the kernel files)
+// FFTg2g (forcefield1.c:499-528): separable DFT along z, y, x (sign -1),
+// pointwise product with khat, inverse (sign +1), real part / M^3.
+// tw holds cos/sin(2 pi j / M_axis) for the three axes back to back.
+// The top-level input at (x, y, z), with the as-is read maps.
+__device__ __forceinline__ double top_input(GridDims g, const double *qL, int x, int y, int z,
+                                            int asis, int nu) {
+  if (asis == kTopAsisStride) {
+    // reference HEAD, FFT build (forcefield1.c:504-515 vs :406-415): unpadded
+    // data read with the padded stride M + nu - 1 and offset nu/2 - 1
+    int o = nu / 2 - 1;
+    long f = ((long)(x + o) * (g.y + nu - 1) + (y + o)) * (g.z + nu - 1) + (z + o);
+    return f < g.size() ? qL[f] : 0.0;
+  }
+  if (asis == kTopAsisFlat) {
+    // reference HEAD, DFT build at L = 1 (forcefield1.c:483-486 on the array
+    // of :338-346): the padded level-1 array (dims M + nu - 1, interior at
+    // nu/2 - 1, zero ghosts) read at the flat unpadded index
+    const long f = ((long)x * g.y + y) * g.z + z;
+    const int P1 = g.y + nu - 1, P2 = g.z + nu - 1, o = nu / 2 - 1;
+    const int a = (int)(f / ((long)P1 * P2)) - o, b = (int)((f / P2) % P1) - o,
+              c = (int)(f % P2) - o;
+    return a >= 0 && a < g.x && b >= 0 && b < g.y && c >= 0 && c < g.z
+               ? qL[((long)a * g.y + b) * g.z + c]
+               : 0.0;
+  }
+  return qL[((long)x * g.y + y) * g.z + z];
+}
+
+// One DFT pass along `axis` from (ire, iim) into (ore, oim), all in LDS.
+template <int BS>
+__device__ inline void dft_pass_lds(GridDims g, int axis, double sign, const double *ire,
+                             const double *iim, double *ore, double *oim, const double *tw) {
+  // (kept lean in registers: it runs beside the pair kernel's wavefronts)
+  const int M = axis == 0 ? g.x : axis == 1 ? g.y : g.z;
+  const double *c = tw + 2 * (axis == 0 ? 0 : axis == 1 ? g.x : g.x + g.y);
+  const int stride = axis == 0 ? g.y * g.z : axis == 1 ? g.z : 1;
+  const int n = (int)g.size();
+#pragma unroll 1
+  for (int e = threadIdx.x; e < n; e += BS) {
+    const int k = axis == 0 ? e / (g.y * g.z) : axis == 1 ? (e / g.z) % g.y : e % g.z;
+    const int base = e - k * stride;
+    double sr = 0, si = 0;
+    int km = 0;  // k m mod M
+#pragma unroll 1
+    for (int m = 0, o = base; m < M; ++m, o += stride) {
+      const double cr = c[2 * km], ci = sign * c[2 * km + 1];
+      const double xr = ire[o], xi = iim[o];
+      sr += xr * cr - xi * ci;
+      si += xr * ci + xi * cr;
+      km += k;
+      km -= km >= M ? M : 0;
+    }
+    ore[e] = sr, oim[e] = si;
+  }
+  __syncthreads();
+}
+
 void launch_top(const DevParams &P, GridDims g, const double *qL, const double *khat,
                 const double *tw, double *work, double *eL, int asis, hipStream_t s);
 // e^l = P e^{l+1} + K_l * q^l (prolongation fused with the stencil pass);
@@ -246,6 +305,16 @@ void launch_g2g_tiles(int prec, G2GArgs args, size_t lds, hipStream_t s, hipEven
 // e^l = sum of the G partials + prolongation of e^{l+1}; with q1 != null
 // also one partial of sum q1 e1 per workgroup into epart (prolong_blocks(g)).
 int prolong_blocks(GridDims g);
+// The top level folded into the prolongation onto the next level (gc: the
+// top grid, at most kTopFuse points; g: the next level, at most
+// kTopFuseBricks prolongation bricks): e_L (also stored into eL) from qL as
+// launch_top computes it, then el as launch_prolong_sum computes it.
+constexpr long kTopFuse = 512;
+constexpr int kTopFuseBricks = 64;
+bool prolong_top_fits(GridDims g, GridDims gc);
+void launch_prolong_top(const DevParams &P, GridDims g, GridDims gc, int G, const double *part,
+                        const double *qL, const double *khat, const double *tw, double *eL,
+                        double *el, int asis, hipStream_t s);
 void launch_prolong_sum(const DevParams &P, GridDims g, GridDims gc, int G, const double *part,
                         const double *ec, double *el, const double *q1, double *epart,
                         hipStream_t s);

@@ -1525,62 +1525,10 @@ __global__ __launch_bounds__(kRBS) void k_restrict(DevParams P, GridDims f, Grid
 // FFTg2g (forcefield1.c:499-528): separable DFT along z, y, x (sign -1),
 // pointwise product with khat, inverse (sign +1), real part / M^3.
 // tw holds cos/sin(2 pi j / M_axis) for the three axes back to back.
-__device__ __forceinline__ double top_input(GridDims g, const double *qL, int x, int y, int z,
-                                            int asis, int nu) {
-  if (asis == kTopAsisStride) {
-    // reference HEAD, FFT build (forcefield1.c:504-515 vs :406-415): unpadded
-    // data read with the padded stride M + nu - 1 and offset nu/2 - 1
-    int o = nu / 2 - 1;
-    long f = ((long)(x + o) * (g.y + nu - 1) + (y + o)) * (g.z + nu - 1) + (z + o);
-    return f < g.size() ? qL[f] : 0.0;
-  }
-  if (asis == kTopAsisFlat) {
-    // reference HEAD, DFT build at L = 1 (forcefield1.c:483-486 on the array
-    // of :338-346): the padded level-1 array (dims M + nu - 1, interior at
-    // nu/2 - 1, zero ghosts) read at the flat unpadded index
-    const long f = ((long)x * g.y + y) * g.z + z;
-    const int P1 = g.y + nu - 1, P2 = g.z + nu - 1, o = nu / 2 - 1;
-    const int a = (int)(f / ((long)P1 * P2)) - o, b = (int)((f / P2) % P1) - o,
-              c = (int)(f % P2) - o;
-    return a >= 0 && a < g.x && b >= 0 && b < g.y && c >= 0 && c < g.z
-               ? qL[((long)a * g.y + b) * g.z + c]
-               : 0.0;
-  }
-  return qL[((long)x * g.y + y) * g.z + z];
-}
-
 // One workgroup of 256 threads with LDS sized to the grid (32 bytes per
 // point), so that it finds room beside the concurrently running pair kernel.
 constexpr int kTopBS = 256;
 constexpr int kTopSmall = 2048;  // largest M^3 done in one workgroup (64 KB of LDS)
-
-// One DFT pass along `axis` from (ire, iim) into (ore, oim), all in LDS.
-__device__ void dft_pass_lds(GridDims g, int axis, double sign, const double *ire,
-                             const double *iim, double *ore, double *oim, const double *tw) {
-  // (kept lean in registers: it runs beside the pair kernel's wavefronts)
-  const int M = axis == 0 ? g.x : axis == 1 ? g.y : g.z;
-  const double *c = tw + 2 * (axis == 0 ? 0 : axis == 1 ? g.x : g.x + g.y);
-  const int stride = axis == 0 ? g.y * g.z : axis == 1 ? g.z : 1;
-  const int n = (int)g.size();
-#pragma unroll 1
-  for (int e = threadIdx.x; e < n; e += kTopBS) {
-    const int k = axis == 0 ? e / (g.y * g.z) : axis == 1 ? (e / g.z) % g.y : e % g.z;
-    const int base = e - k * stride;
-    double sr = 0, si = 0;
-    int km = 0;  // k m mod M
-#pragma unroll 1
-    for (int m = 0, o = base; m < M; ++m, o += stride) {
-      const double cr = c[2 * km], ci = sign * c[2 * km + 1];
-      const double xr = ire[o], xi = iim[o];
-      sr += xr * cr - xi * ci;
-      si += xr * ci + xi * cr;
-      km += k;
-      km -= km >= M ? M : 0;
-    }
-    ore[e] = sr, oim[e] = si;
-  }
-  __syncthreads();
-}
 
 __global__ __launch_bounds__(kTopBS) void k_top_small(DevParams P, GridDims g,
                                                       const double *__restrict__ qL,
@@ -1599,14 +1547,14 @@ __global__ __launch_bounds__(kTopBS) void k_top_small(DevParams P, GridDims g,
     im[e] = 0.0;
   }
   __syncthreads();
-  dft_pass_lds(g, 2, -1.0, re, im, re2, im2, twl);
-  dft_pass_lds(g, 1, -1.0, re2, im2, re, im, twl);
-  dft_pass_lds(g, 0, -1.0, re, im, re2, im2, twl);
+  dft_pass_lds<kTopBS>(g, 2, -1.0, re, im, re2, im2, twl);
+  dft_pass_lds<kTopBS>(g, 1, -1.0, re2, im2, re, im, twl);
+  dft_pass_lds<kTopBS>(g, 0, -1.0, re, im, re2, im2, twl);
   for (long e = threadIdx.x; e < n; e += kTopBS) re2[e] *= khat[e], im2[e] *= khat[e];
   __syncthreads();
-  dft_pass_lds(g, 0, 1.0, re2, im2, re, im, twl);
-  dft_pass_lds(g, 1, 1.0, re, im, re2, im2, twl);
-  dft_pass_lds(g, 2, 1.0, re2, im2, re, im, twl);
+  dft_pass_lds<kTopBS>(g, 0, 1.0, re2, im2, re, im, twl);
+  dft_pass_lds<kTopBS>(g, 1, 1.0, re, im, re2, im2, twl);
+  dft_pass_lds<kTopBS>(g, 2, 1.0, re2, im2, re, im, twl);
   for (long e = threadIdx.x; e < n; e += kTopBS) eL[e] = re[e] / (double)n;
 }
 

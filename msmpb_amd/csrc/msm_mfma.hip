@@ -366,13 +366,11 @@ __device__ __forceinline__ double prolong_taps(const double *J, int v, F c) {
 constexpr int kPB = 8;  // fine brick edge of the prolongation
 
 template <int NU>
-__global__ __launch_bounds__(256) void k_prolong_sum(DevParams P, GridDims g, GridDims gc, int G,
-                                                     const double *__restrict__ part,
-                                                     const double *__restrict__ ec,
-                                                     double *__restrict__ el,
-                                                     const double *__restrict__ q1,
-                                                     double *__restrict__ epart) {
-  chain_issue_priority();
+__device__ __forceinline__ void prolong_brick(const DevParams &P, GridDims g, GridDims gc, int G,
+                                              const double *__restrict__ part,
+                                              const double *ec, double *__restrict__ el,
+                                              const double *__restrict__ q1,
+                                              double *__restrict__ epart) {
   constexpr int H = NU / 2, CW = H + 4, B = kPB;
   __shared__ double cs[CW * CW * CW];
   __shared__ double t1[CW * CW * B];  // [cx][cy][fz]
@@ -420,6 +418,60 @@ __global__ __launch_bounds__(256) void k_prolong_sum(DevParams P, GridDims g, Gr
     __syncthreads();
     if (tid == 0) epart[blockIdx.x] = ((wsum[0] + wsum[1]) + wsum[2]) + wsum[3];
   }
+}
+
+template <int NU>
+__global__ __launch_bounds__(256) void k_prolong_sum(DevParams P, GridDims g, GridDims gc, int G,
+                                                     const double *__restrict__ part,
+                                                     const double *__restrict__ ec,
+                                                     double *__restrict__ el,
+                                                     const double *__restrict__ q1,
+                                                     double *__restrict__ epart) {
+  chain_issue_priority();
+  prolong_brick<NU>(P, g, gc, G, part, ec, el, q1, epart);
+}
+
+// The coarsest prolongation with the top level folded in: on a small top
+// grid whose next level has few prolongation bricks, every workgroup
+// computes the whole top-level potential itself -- the DFT convolution of
+// the one-workgroup top-level kernel, in its own LDS -- and prolongates its
+// brick from it, so that the chain runs one dependent launch fewer;
+// workgroup 0 also stores the top-level potential.
+template <int NU>
+__global__ __launch_bounds__(256) void k_prolong_top(DevParams P, GridDims g, GridDims gc, int G,
+                                                     const double *__restrict__ part,
+                                                     const double *__restrict__ qL,
+                                                     const double *__restrict__ khat,
+                                                     const double *__restrict__ tw,
+                                                     double *__restrict__ eL,
+                                                     double *__restrict__ el, int asis) {
+  chain_issue_priority();
+  extern __shared__ double top_lds[];  // 4 n doubles: two complex buffers; the twiddles
+  const int n = (int)gc.size();
+  double *re = top_lds, *im = top_lds + n, *re2 = top_lds + 2 * n, *im2 = top_lds + 3 * n;
+  double *const twl = top_lds + 4 * n;
+  for (int k = threadIdx.x; k < 2 * (gc.x + gc.y + gc.z); k += 256) twl[k] = tw[k];
+  for (int e = threadIdx.x; e < n; e += 256) {
+    const int z = e % gc.z, y = (e / gc.z) % gc.y, x = e / (gc.y * gc.z);
+    re[e] = top_input(gc, qL, x, y, z, asis, P.nu);
+    im[e] = 0.0;
+  }
+  __syncthreads();
+  dft_pass_lds<256>(gc, 2, -1.0, re, im, re2, im2, twl);
+  dft_pass_lds<256>(gc, 1, -1.0, re2, im2, re, im, twl);
+  dft_pass_lds<256>(gc, 0, -1.0, re, im, re2, im2, twl);
+  for (int e = threadIdx.x; e < n; e += 256) re2[e] *= khat[e], im2[e] *= khat[e];
+  __syncthreads();
+  dft_pass_lds<256>(gc, 0, 1.0, re2, im2, re, im, twl);
+  dft_pass_lds<256>(gc, 1, 1.0, re, im, re2, im2, twl);
+  dft_pass_lds<256>(gc, 2, 1.0, re2, im2, re, im, twl);
+  for (int e = threadIdx.x; e < n; e += 256) {
+    const double v = re[e] / (double)n;
+    re[e] = v;
+    if (blockIdx.x == 0) eL[e] = v;
+  }
+  __syncthreads();
+  prolong_brick<NU>(P, g, gc, G, part, re, el, nullptr, nullptr);
 }
 
 }  // namespace
@@ -620,6 +672,26 @@ void launch_g2g_tiles(int prec, G2GArgs args, size_t lds, hipStream_t s, hipEven
 
 int prolong_blocks(GridDims g) {
   return ((g.x + kPB - 1) / kPB) * ((g.y + kPB - 1) / kPB) * ((g.z + kPB - 1) / kPB);
+}
+
+bool prolong_top_fits(GridDims g, GridDims gc) {
+  return gc.size() <= kTopFuse && prolong_blocks(g) <= kTopFuseBricks;
+}
+
+void launch_prolong_top(const DevParams &P, GridDims g, GridDims gc, int G, const double *part,
+                        const double *qL, const double *khat, const double *tw, double *eL,
+                        double *el, int asis, hipStream_t s) {
+  const unsigned nb = (unsigned)prolong_blocks(g);
+  const size_t lds = (4 * gc.size() + 2 * (gc.x + gc.y + gc.z)) * sizeof(double);
+#define CALL(NU) \
+  k_prolong_top<NU><<<nb, 256, lds, s>>>(P, g, gc, G, part, qL, khat, tw, eL, el, asis)
+  switch (P.nu) {
+    case 4: CALL(4); break;
+    case 6: CALL(6); break;
+    case 8: CALL(8); break;
+    default: CALL(10); break;
+  }
+#undef CALL
 }
 
 void launch_prolong_sum(const DevParams &P, GridDims g, GridDims gc, int G, const double *part,
